@@ -1,0 +1,251 @@
+"""Benchmark of the ray-tracing hot path on one MI355X.
+
+Workload (the large single-scene configuration): a 1448 x 1448-cell fBm
+heightfield (4,193,408 triangles) plus 2,000 level-3 icospheres (2,560,000
+triangles), 6,753,408 fp32 triangles; GPU LBVH build; 1920 x 1080 path tracing,
+16 spp, depth 8, Lambertian albedo U(0.2, 0.8) per triangle, 1% emissive.
+
+A step is one path-traced frame (all 16 spp of every pixel).  `value` is the
+incoherent (bounce >= 1) closest-hit rays per second of the timed frames, in
+Mrays/s.  LBVH build time is measured separately on the same scene.  The
+roofline object is for the path-tracing kernel (HIP events on its stream);
+the CPU baseline is the reference's own tiled_sched path (oracle/_ref) on a
+bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "incoherent Mrays/s @4M tris; LBVH build ms; adjoint vs FD speedup"
+HBM_PEAK_GBS = 8000.0
+RAY_IN, HIT_OUT, PAIR_BYTES, TRI_BYTES = 32, 16, 64, 48
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1448)
+    ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--builds", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--out", type=str, default="")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(the path shards by image tiles or scenes, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import visionaray_amd as V
+    from visionaray_amd import scenes as S
+
+    t0 = time.time()
+    tris = S.config1_scene(cells=args.cells, spheres=args.spheres)
+    mats = S.config1_materials(tris.size)
+    log(f"scene: {tris.size} triangles generated in {time.time() - t0:.1f}s")
+
+    V.init(0)
+    stream = V.Stream()
+    d_tris = V.DeviceArray.from_host(tris)
+    d_mats = V.DeviceArray.from_host(mats)
+
+    # ---- LBVH build timing (same scene) ----
+    builder = V.lbvh_builder()
+    build_ms = []
+    bvh = None
+    for i in range(args.builds + 1):
+        if bvh is not None:
+            bvh.destroy()
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        bvh = builder.build(d_tris, keep_nodes=False, stream=stream)
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1)
+        if i > 0:
+            build_ms.append(ms)
+    info = bvh.info()
+    lbvh_ms = float(np.median(build_ms))
+    log(f"LBVH: {lbvh_ms:.3f} ms median of {len(build_ms)} (depth {info['depth']})")
+
+    # ---- path tracing ----
+    eye, center, up, fovy, aspect = S.config1_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, args.width / args.height)
+    kp = V.make_kernel_params(num_bounces=args.depth, epsilon=1e-4, bg_color=(0.0, 0.0, 0.0, 1.0),
+                              ambient_color=(0.6, 0.7, 0.9, 1.0))
+    kernel = V.pathtracing.kernel(bvh, d_mats, kp)
+    rt = V.render_target(args.width, args.height)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=args.spp, sfactor=1.0, dfactor=0.0), cam, rt)
+
+    for _ in range(args.warmup):
+        sched.frame(kernel, sp, stream=stream)
+    stream.synchronize()
+    sched.reset_counters()
+
+    frame_ms = []
+    V.synchronize()
+    wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        sched.frame(kernel, sp, stream=stream)
+        e1.record(stream)
+        frame_ms.append(e0.elapsed_ms(e1))
+    V.synchronize()
+    wall = time.perf_counter() - wall0
+    n_primary, n_secondary = sched.rays()
+    prim_per, sec_per = n_primary / args.steps, n_secondary / args.steps
+    ms_step = wall * 1e3 / args.steps
+    kernel_ms = float(np.mean(frame_ms))
+    incoherent_mrays = n_secondary / (wall * 1e6)
+    total_mrays = (n_primary + n_secondary) / (wall * 1e6)
+    log(f"frames: {ms_step:.2f} ms/step wall, {kernel_ms:.2f} ms kernel; {prim_per:.0f} primary + "
+        f"{sec_per:.0f} secondary rays per frame; {incoherent_mrays:.1f} incoherent Mrays/s")
+
+    # ---- algorithmic bytes of one frame (counting variant, untimed) ----
+    stats = V.DeviceArray(4, np.uint64, zero=True)
+    sched.frame_id -= 1  # replay the last timed frame's seeds
+    sched.frame(kernel, sp, stream=stream, stats=stats)
+    stream.synchronize()
+    st = stats.to_host().astype(np.float64)
+    rays_frame = st[0] + st[1]
+    alg_bytes = rays_frame * (RAY_IN + HIT_OUT) + st[2] * PAIR_BYTES + st[3] * TRI_BYTES
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    log(f"per ray: {st[2] / rays_frame:.2f} node pairs, {st[3] / rays_frame:.2f} triangles; "
+        f"{alg_bytes / 1e9:.2f} GB algorithmic per frame -> {achieved:.0f} GB/s")
+
+    traffic = None
+    tfile = ROOT / "profiles" / "pathtrace_c1_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    # ---- CPU baseline: the reference's own tiled_sched path ----
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream)
+
+    result = {
+        "metric": METRIC,
+        "value": round(incoherent_mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "C1 large scene: fBm heightfield 1448x1448 cells + 2000 level-3 icospheres, "
+                        f"{tris.size} fp32 triangles; GPU LBVH; {args.width}x{args.height} path tracing, "
+                        f"{args.spp} spp, depth {args.depth}",
+            "triangles": int(tris.size), "width": args.width, "height": args.height, "spp": args.spp,
+            "depth": args.depth,
+        },
+        "lbvh_build_ms": round(lbvh_ms, 4),
+        "lbvh_depth": info["depth"],
+        "total_mrays_per_s": round(total_mrays, 3),
+        "rays_per_frame": {"primary": int(prim_per), "secondary": int(sec_per)},
+        "adjoint_vs_fd_speedup": None,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "kernel": "k_pathtrace",
+            "kernel_ms": round(kernel_ms, 4),
+            "algorithmic_bytes_per_launch": int(alg_bytes),
+            "per_ray": {"node_pairs": round(st[2] / rays_frame, 3), "triangles": round(st[3] / rays_frame, 3)},
+        },
+        "cpu_baseline": cpu,
+    }
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.out:
+        Path(args.out).write_text(line + "\n")
+
+
+def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
+    """Time the reference's tiled_sched + pathtracing::kernel (compiled from
+    the reference headers into oracle/_ref) on a bounded sample: the same
+    scene, camera and depth at --cpu-spp samples per pixel.  Rays/s uses the
+    query count of the GPU run of the identical sample (same per-pixel seeds)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    try:
+        import oracle as O
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "error": f"oracle unavailable: {e}"}
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    w, h = args.width, args.height
+    eye, center, up, fovy, _ = S.config1_camera()
+    cam_in = O.camera_input(eye, center, up, fovy, w / h)
+    kpv = kp[0]
+    # GPU query count for the same sample
+    cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+    rt = V.render_target(w, h)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=args.cpu_spp), cam, rt)
+    sched.frame(V.pathtracing.kernel(bvh, d_mats, kp), sp, stream=stream)
+    stream.synchronize()
+    n_primary, n_secondary = sched.rays()
+    if O.ref_available():
+        params = O.ref_params(w, h, spp=args.cpu_spp, num_bounces=int(kpv["num_bounces"]),
+                              epsilon=float(kpv["epsilon"]), bg=tuple(kpv["bg"]), ambient=tuple(kpv["ambient"]),
+                              threads=threads)
+        log(f"cpu baseline: reference tiled_sched, {threads} threads, {w}x{h} at {args.cpu_spp} spp")
+        t0 = time.time()
+        r = O.ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, timeout=900)
+        log(f"cpu baseline done in {time.time() - t0:.1f}s: frame {r['frame_s_median']:.3f}s, "
+            f"build {r['build_s']:.2f}s")
+        frame_s = r["frame_s_median"]
+        return {
+            "value": round(n_secondary / frame_s / 1e6, 4),
+            "unit": "Mrays/s",
+            "cores": threads,
+            "kind": "reference",
+            "sample": f"same scene and camera, {w}x{h} at {args.cpu_spp} spp, depth {int(kpv['num_bounces'])}; "
+                      f"tiled_sched<basic_ray<simd::float4>> + pathtracing::kernel over the reference's CPU "
+                      f"lbvh_builder tree (leaf size 4); median of 3 frames after 1 warm-up; rays counted "
+                      f"on the GPU run of the identical sample",
+            "frame_s": round(frame_s, 4),
+            "total_mrays_per_s": round((n_primary + n_secondary) / frame_s / 1e6, 4),
+            "cpu_lbvh_build_s": round(r["build_s"], 3),
+            "binary": O.ref_binary().name,
+        }
+    return {"value": None, "error": "oracle/_ref not built"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,229 @@
+/*
+ * visionaray_amd.h -- C ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * The reference exposes this path only as C++ templates (no ABI).  Each entry
+ * point below replaces one reference interface, cited as file:line relative to
+ * the reference repository root (headers under include/visionaray/).  The C++
+ * facade in visionaray_amd.hpp re-presents the reference's names on top.
+ *
+ * Conventions
+ *   - Every function returns an int status: VR_OK (0) or a VR_ERR_* code; the
+ *     message of the last failure on the calling thread is vr_last_error().
+ *     (The reference prints through *_SAFE_CALL and continues,
+ *     hip/safe_call.h:17-39; errors here are returned, never printed.)
+ *   - Pointers named d_* are device memory (vr_malloc or any hipMalloc);
+ *     h_* are host memory.  Sizes are element counts unless named *_bytes.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Work is enqueued
+ *     asynchronously unless the function says it synchronizes.
+ *   - Handles (vr_bvh) own their device memory; destroy them with
+ *     vr_bvh_destroy.  A handle may be used from one host thread at a time.
+ */
+#ifndef VISIONARAY_AMD_H
+#define VISIONARAY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VR_OK 0
+#define VR_ERR_INVALID_ARGUMENT 1
+#define VR_ERR_OUT_OF_MEMORY 2
+#define VR_ERR_HIP 3
+#define VR_ERR_NOT_INITIALIZED 4
+#define VR_ERR_UNSUPPORTED 5
+
+/* basic_triangle<3, float, unsigned> (math/primitive.h:14-20,
+ * math/triangle.h:34-36): 44 bytes, e1 = v2 - v1, e2 = v3 - v1. */
+typedef struct vr_triangle {
+    uint32_t geom_id;
+    uint32_t prim_id;
+    float v1[3];
+    float e1[3];
+    float e2[3];
+} vr_triangle;
+
+/* bvh_node (bvh.h:70-141): 32 bytes; num_prims == 0 marks an inner node whose
+ * children are nodes first, first + 1; a leaf covers indices
+ * [first, first + num_prims). */
+typedef struct vr_bvh_node {
+    float lo[3];
+    float hi[3];
+    uint32_t first;
+    uint16_t num_prims;
+    uint8_t axis;
+    uint8_t sign;
+} vr_bvh_node;
+
+/* basic_ray<float> (math/ray.h:16-40): 32 bytes. */
+typedef struct vr_ray {
+    float ori[3];
+    float dir[3];
+    float tmin;
+    float tmax;
+} vr_ray;
+
+/* hit_record<ray, primitive<unsigned>> reduced to what callers read
+ * (math/intersect.h:151-169): prim_id (-1 on a miss), t, u = b1, v = b2. */
+typedef struct vr_hit {
+    int32_t prim_id;
+    float t;
+    float u;
+    float v;
+} vr_hit;
+
+/* pinhole_camera after begin_frame (detail/pinhole_camera.inl:87-97). */
+typedef struct vr_camera {
+    float eye[3];
+    float U[3];
+    float V[3];
+    float W[3];
+} vr_camera;
+
+/* generic_material<emissive, matte>: type 0 = matte (cd, kd;
+ * detail/material/matte.inl:43-56), 1 = emissive (ce, ls;
+ * detail/material/emissive.inl:31-40).  36 bytes. */
+typedef struct vr_material {
+    int32_t type;
+    float cd[3];
+    float kd;
+    float ce[3];
+    float ls;
+} vr_material;
+
+/* kernel_params subset used without a light list (kernels.h:38-132):
+ * bounces, epsilon, background and ambient intensity (cl * kl). */
+typedef struct vr_kernel_params {
+    uint32_t num_bounces;
+    float epsilon;
+    float bg[3];
+    float ambient[3];
+} vr_kernel_params;
+
+/* jittered_blend pixel sampler plus the frame counter of the scheduler
+ * (pixel_sampler_types.h:51-69, detail/cuda_sched.inl:226-246). */
+typedef struct vr_sched_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;
+    uint32_t frame_id;
+    float sfactor;
+    float dfactor;
+} vr_sched_params;
+
+typedef struct vr_bvh_info {
+    uint64_t num_prims;
+    uint64_t num_nodes;      /* reference layout: 2n - 1 for an LBVH */
+    uint32_t depth;          /* levels of the deepest leaf (root = 1) */
+    uint32_t max_leaf_size;
+} vr_bvh_info;
+
+typedef struct vr_bvh_t* vr_bvh;
+
+/* vr_lbvh_build flags */
+#define VR_BVH_KEEP_NODES 1u   /* also write the reference bvh_node[2n-1] layout */
+
+/* ---- runtime ---------------------------------------------------------- */
+/* Select device `device` for this thread and create its state. */
+int vr_init(int device);
+const char* vr_last_error(void);
+int vr_version(int* major, int* minor);
+int vr_malloc(void** d_ptr, size_t bytes);
+int vr_free(void* d_ptr);
+int vr_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int vr_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int vr_memset(void* d_dst, int value, size_t bytes);
+int vr_synchronize(void);
+int vr_stream_create(void** stream);
+int vr_stream_destroy(void* stream);
+int vr_stream_synchronize(void* stream);
+int vr_event_create(void** event);
+int vr_event_destroy(void* event);
+int vr_event_record(void* event, void* stream);
+int vr_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- camera ------------------------------------------------------------- */
+/* pinhole_camera::perspective + look_at + begin_frame
+ * (detail/pinhole_camera.inl:12-29, 31-64, 87-97). Host only. */
+int vr_camera_look_at(const float eye[3], const float center[3], const float up[3],
+                      float fovy, float aspect, vr_camera* out);
+
+/* ---- BVH construction ------------------------------------------------- */
+/* lbvh_builder::build(cuda_index_bvh<P>{}, P* prims, n)
+ * (detail/bvh/lbvh.h:681-844): Morton codes, stable radix sort, Karras
+ * hierarchy, bottom-up boxes.  d_prims stays owned by the caller and is
+ * copied into the BVH.  n == 0 yields an empty BVH. */
+int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* stream, vr_bvh* out);
+
+/* assign_morton_codes (detail/bvh/lbvh.h:242-269) alone: codes[i] for
+ * primitive i, unsorted. */
+int vr_lbvh_morton(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, void* stream);
+
+/* The sorted prim_refs of the last build (lbvh.h:740-755 output):
+ * codes and original ids in sorted order, n each. */
+int vr_bvh_sorted_refs(vr_bvh bvh, uint32_t* d_codes, int32_t* d_ids, void* stream);
+
+/* index_bvh_t built elsewhere (e.g. binned_sah_builder on the host,
+ * detail/bvh/sah.h:176-807): adopt a reference-layout node array and index
+ * list (bvh.h:870-1010).  h_nodes/h_indices are host arrays. */
+int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh_node* h_nodes,
+                      size_t num_nodes, const uint32_t* h_indices, vr_bvh* out);
+
+/* bvh_refitter::refit (detail/bvh/refit.h:28-114): recompute every box for
+ * moved vertices (same count and order as at build time). */
+int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream);
+
+int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info);
+/* Reference-layout nodes (tree.nodes(), bvh.h:910) into d_nodes[num_nodes]. */
+int vr_bvh_nodes(vr_bvh bvh, vr_bvh_node* d_nodes, void* stream);
+/* tree.indices() (bvh.h:913) into d_indices[num_prims]. */
+int vr_bvh_indices(vr_bvh bvh, uint32_t* d_indices, void* stream);
+int vr_bvh_destroy(vr_bvh bvh);
+
+/* Host binned SAH build (binned_sah_builder::build, detail/bvh/sah.h:176-807,
+ * 16 bins, no spatial splits).  h_nodes must hold 2n - 1 nodes; the node count
+ * used is returned in *num_nodes. */
+int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes,
+                 size_t* num_nodes, uint32_t* h_indices);
+
+/* ---- traversal -------------------------------------------------------- */
+/* closest_hit / any_hit over one BVH (detail/traverse_linear.inl:149-214 ->
+ * intersect<ClosestHit|AnyHit>, detail/bvh/intersect.inl:41-357). */
+int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
+
+/* Same, also counting per ray the node-pair and triangle tests
+ * (d_counts[2 * i] = pairs, d_counts[2 * i + 1] = triangles); for the
+ * algorithmic-bytes model. */
+int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits,
+                     uint32_t* d_counts, void* stream);
+
+/* ---- kernels under the scheduler -------------------------------------- */
+/* Ambient occlusion kernel of src/examples/ao/main.cpp:192-260 under the
+ * jittered_blend sampler; d_accum/d_color are RGBA32F width*height. */
+int vr_render_ao(vr_bvh bvh, const vr_camera* cam, const vr_sched_params* sp, uint32_t ao_samples,
+                 float ao_radius, const float bg[3], float* d_accum, float* d_color, void* stream);
+
+/* pathtracing::kernel (detail/pathtracing.inl:43-233) under the
+ * jittered_blend sampler and per-pixel seeds make_random_seed(y*w+x, frame)
+ * (detail/cuda_sched.inl:32-69), on a persistent wavefront that regenerates
+ * paths.  d_materials is indexed by the triangle's geom_id.  d_ray_counts
+ * (optional, 2 x uint64, accumulated) receives primary and secondary
+ * closest-hit queries. */
+int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                 const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
+                 float* d_accum, float* d_color, uint64_t* d_ray_counts, void* stream);
+
+/* vr_render_pt with per-frame traversal statistics for the algorithmic-bytes
+ * model: d_stats (4 x uint64, accumulated) = primary queries, secondary
+ * queries, node-pair tests, triangle tests.  Same image as vr_render_pt. */
+int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                         const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
+                         float* d_accum, float* d_color, uint64_t* d_stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,386 @@
+"""Python access to the oracle (test infrastructure only).
+
+* `liboracle.so` -- our C restatement of the reference algorithms
+  (vr_oracle.c), loaded with ctypes.
+* `_ref/vr_ref` -- the reference's own CPU code compiled from its headers
+  (ref_driver.cpp), run as a subprocess on raw binary files.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_BIN = HERE / "_ref" / "vr_ref"
+REF_BIN_V3 = HERE / "_ref" / "vr_ref_v3"
+
+TRI_DTYPE = np.dtype([("geom_id", "<u4"), ("prim_id", "<u4"), ("v1", "<f4", (3,)), ("e1", "<f4", (3,)),
+                      ("e2", "<f4", (3,))])
+NODE_DTYPE = np.dtype([("lo", "<f4", (3,)), ("hi", "<f4", (3,)), ("first", "<u4"), ("num_prims", "<u2"),
+                       ("axis", "u1"), ("sign", "u1")])
+RAY_DTYPE = np.dtype([("ori", "<f4", (3,)), ("dir", "<f4", (3,)), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+CAMERA_DTYPE = np.dtype([("eye", "<f4", (3,)), ("U", "<f4", (3,)), ("V", "<f4", (3,)), ("W", "<f4", (3,))])
+MAT_DTYPE = np.dtype([("type", "<i4"), ("cd", "<f4", (3,)), ("kd", "<f4"), ("ce", "<f4", (3,)), ("ls", "<f4")])
+RPARAMS_DTYPE = np.dtype([
+    ("width", "<u4"), ("height", "<u4"), ("spp", "<u4"), ("frame_id", "<u4"), ("num_bounces", "<u4"),
+    ("epsilon", "<f4"), ("bg", "<f4", (3,)), ("ambient", "<f4", (3,)), ("sfactor", "<f4"), ("dfactor", "<f4"),
+    ("ao_samples", "<u4"), ("ao_radius", "<f4"), ("arg_order", "<u4"),
+])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            subprocess.run(["make", "-C", str(HERE), "liboracle.so"], check=True, capture_output=True)
+        l = C.CDLL(str(LIB))
+        P, SZ = C.c_void_p, C.c_size_t
+        l.vro_morton_encode3d.restype = C.c_uint32
+        l.vro_morton_encode3d.argtypes = [C.c_uint32] * 3
+        l.vro_make_random_seed.restype = C.c_uint32
+        l.vro_make_random_seed.argtypes = [C.c_int32, C.c_int32]
+        l.vro_lbvh_build.restype = C.c_int
+        l.vro_lbvh_build.argtypes = [P, SZ, P, P, P, P]
+        l.vro_morton_codes.argtypes = [P, SZ, P]
+        l.vro_centroid_bounds.argtypes = [P, SZ, P, P]
+        l.vro_sort_prim_refs.argtypes = [P, SZ, P, P]
+        l.vro_determine_range.argtypes = [P, C.c_int, C.c_int, P, P, P]
+        l.vro_trace.argtypes = [P, P, P, SZ, P, SZ, C.c_int, P]
+        l.vro_trace_brute.argtypes = [P, SZ, P, SZ, C.c_int, P]
+        l.vro_rng_stream.argtypes = [C.c_uint32, SZ, P]
+        l.vro_refit.argtypes = [P, SZ, P, P]
+        l.vro_render_ao.argtypes = [P, P, P, SZ, P, P, P, P, C.c_int, C.c_int]
+        l.vro_render_pt.argtypes = [P, P, P, SZ, P, P, P, P, P, C.c_int, C.c_int, P]
+        _lib = l
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------------------
+# restatement
+# ---------------------------------------------------------------------------
+
+def morton_encode3d(x: int, y: int, z: int) -> int:
+    return int(lib().vro_morton_encode3d(x, y, z))
+
+
+def morton_codes(tris: np.ndarray) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    out = np.zeros(tris.size, np.uint32)
+    lib().vro_morton_codes(_p(tris), tris.size, _p(out))
+    return out
+
+
+def centroid_bounds(tris: np.ndarray):
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    lib().vro_centroid_bounds(_p(tris), tris.size, _p(lo), _p(hi))
+    return lo, hi
+
+
+def sort_prim_refs(codes: np.ndarray):
+    codes = np.ascontiguousarray(codes, np.uint32)
+    sc = np.zeros(codes.size, np.uint32)
+    si = np.zeros(codes.size, np.int32)
+    lib().vro_sort_prim_refs(_p(codes), codes.size, _p(sc), _p(si))
+    return sc, si
+
+
+def determine_range(sorted_codes: np.ndarray, i: int):
+    sorted_codes = np.ascontiguousarray(sorted_codes, np.uint32)
+    f = np.zeros(1, np.int32)
+    l = np.zeros(1, np.int32)
+    s = np.zeros(1, np.int32)
+    lib().vro_determine_range(_p(sorted_codes), sorted_codes.size, i, _p(f), _p(l), _p(s))
+    return int(f[0]), int(l[0]), int(s[0])
+
+
+def lbvh_build(tris: np.ndarray):
+    """-> (nodes[2n-1], indices[n], sorted_codes[n], sorted_ids[n])"""
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    n = tris.size
+    nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
+    idx = np.zeros(max(n, 1), np.uint32)
+    sc = np.zeros(max(n, 1), np.uint32)
+    si = np.zeros(max(n, 1), np.int32)
+    rc = lib().vro_lbvh_build(_p(tris), n, _p(nodes), _p(idx), _p(sc), _p(si))
+    assert rc == 0
+    return nodes[: max(2 * n - 1, 0)], idx[:n], sc[:n], si[:n]
+
+
+def refit(nodes: np.ndarray, tris: np.ndarray, indices: np.ndarray) -> np.ndarray:
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE).copy()
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    lib().vro_refit(_p(nodes), nodes.size, _p(tris), _p(indices))
+    return nodes
+
+
+def trace(nodes, tris, indices, rays, any_hit=False) -> np.ndarray:
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(rays.size, HIT_DTYPE)
+    lib().vro_trace(_p(nodes), _p(tris), _p(indices), tris.size, _p(rays), rays.size, int(any_hit), _p(hits))
+    return hits
+
+
+def trace_brute(tris, rays, any_hit=False) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(rays.size, HIT_DTYPE)
+    lib().vro_trace_brute(_p(tris), tris.size, _p(rays), rays.size, int(any_hit), _p(hits))
+    return hits
+
+
+def make_random_seed(v0: int, v1: int) -> int:
+    return int(lib().vro_make_random_seed(v0, v1))
+
+
+def rng_stream(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, np.float32)
+    lib().vro_rng_stream(seed, count, _p(out))
+    return out
+
+
+def render_params(width, height, spp=1, frame_id=0, num_bounces=5, epsilon=float(np.finfo(np.float32).eps),
+                  bg=(0, 0, 0), ambient=(0, 0, 0), sfactor=1.0, dfactor=0.0, ao_samples=8, ao_radius=0.1,
+                  arg_order=1) -> np.ndarray:
+    p = np.zeros(1, RPARAMS_DTYPE)
+    p["width"], p["height"], p["spp"], p["frame_id"] = width, height, spp, frame_id
+    p["num_bounces"], p["epsilon"] = num_bounces, epsilon
+    p["bg"], p["ambient"] = bg, ambient
+    p["sfactor"], p["dfactor"] = sfactor, dfactor
+    p["ao_samples"], p["ao_radius"], p["arg_order"] = ao_samples, ao_radius, arg_order
+    return p
+
+
+def render_ao(nodes, tris, indices, cam, params, rows=None) -> np.ndarray:
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    cam = np.ascontiguousarray(cam, CAMERA_DTYPE)
+    w, h = int(params["width"][0]), int(params["height"][0])
+    acc = np.zeros((h, w, 4), np.float32)
+    col = np.zeros((h, w, 4), np.float32)
+    y0, y1 = rows if rows else (0, h)
+    lib().vro_render_ao(_p(nodes), _p(tris), _p(indices), tris.size, _p(cam), _p(params), _p(acc), _p(col), y0, y1)
+    return col
+
+
+def render_pt(nodes, tris, indices, mats, cam, params, rows=None):
+    """-> (color image, [primary rays, secondary rays])"""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    mats = np.ascontiguousarray(mats, MAT_DTYPE)
+    cam = np.ascontiguousarray(cam, CAMERA_DTYPE)
+    w, h = int(params["width"][0]), int(params["height"][0])
+    acc = np.zeros((h, w, 4), np.float32)
+    col = np.zeros((h, w, 4), np.float32)
+    counts = np.zeros(2, np.uint64)
+    y0, y1 = rows if rows else (0, h)
+    lib().vro_render_pt(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(cam), _p(params), _p(acc),
+                        _p(col), y0, y1, _p(counts))
+    return col, counts
+
+
+# ---------------------------------------------------------------------------
+# the reference's own code (oracle/_ref/vr_ref), where it was built
+# ---------------------------------------------------------------------------
+
+def ref_available() -> bool:
+    return REF_BIN.exists()
+
+
+def ref_binary() -> Path:
+    """The baseline build the host CPU supports (AVX2 when present)."""
+    try:
+        flags = Path("/proc/cpuinfo").read_text()
+        if " avx2 " in flags and " fma " in flags and REF_BIN_V3.exists():
+            return REF_BIN_V3
+    except OSError:
+        pass
+    return REF_BIN
+
+
+class _Tmp:
+    def __init__(self):
+        self.dir = tempfile.TemporaryDirectory(prefix="vr_ref_")
+
+    def put(self, name: str, a: np.ndarray) -> str:
+        p = os.path.join(self.dir.name, name)
+        np.ascontiguousarray(a).tofile(p)
+        return p
+
+    def path(self, name: str) -> str:
+        return os.path.join(self.dir.name, name)
+
+    def close(self):
+        self.dir.cleanup()
+
+
+def _run(args, binary=None):
+    b = str(binary or REF_BIN)
+    res = subprocess.run([b, *map(str, args)], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"vr_ref {args[0]} failed: {res.stderr.strip()}")
+    return res.stdout
+
+
+def camera_input(eye, center, up, fovy, aspect) -> np.ndarray:
+    return np.array([*eye, *center, *up, fovy, aspect], np.float32)
+
+
+def ref_params(width, height, spp=1, frame_id=0, num_bounces=5, epsilon=float(np.finfo(np.float32).eps),
+               bg=(0, 0, 0), ambient=(0, 0, 0), sfactor=1.0, dfactor=0.0, ao_samples=8, ao_radius=0.1,
+               threads=0) -> np.ndarray:
+    return np.array([width, height, spp, frame_id, num_bounces, epsilon, *bg, *ambient, sfactor, dfactor,
+                     ao_samples, ao_radius, threads], np.float32)
+
+
+def ref_morton(tris):
+    t = _Tmp()
+    try:
+        _run(["morton", t.put("tris.bin", tris), tris.size, t.path("out.bin")])
+        out = np.fromfile(t.path("out.bin"), np.uint32).reshape(-1, 2)
+        return out[:, 0].copy(), out[:, 1].astype(np.int32)
+    finally:
+        t.close()
+
+
+def ref_karras(sorted_codes):
+    t = _Tmp()
+    try:
+        _run(["karras", t.put("codes.bin", np.asarray(sorted_codes, np.uint32)), len(sorted_codes),
+              t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), np.int32).reshape(-1, 3)
+    finally:
+        t.close()
+
+
+def ref_trace(nodes, tris, indices, rays, any_hit=False):
+    t = _Tmp()
+    try:
+        _run(["trace", t.put("nodes.bin", nodes), nodes.size, t.put("tris.bin", tris), tris.size,
+              t.put("idx.bin", np.asarray(indices, np.uint32)), t.put("rays.bin", rays), rays.size,
+              int(any_hit), t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), HIT_DTYPE)
+    finally:
+        t.close()
+
+
+def ref_brute(tris, rays, any_hit=False):
+    t = _Tmp()
+    try:
+        _run(["brute", t.put("tris.bin", tris), tris.size, t.put("rays.bin", rays), rays.size, int(any_hit),
+              t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), HIT_DTYPE)
+    finally:
+        t.close()
+
+
+def ref_seed(pairs):
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    t = _Tmp()
+    try:
+        _run(["seed", t.put("pairs.bin", pairs), pairs.shape[0], t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), np.uint32)
+    finally:
+        t.close()
+
+
+def ref_rng(seed, count):
+    t = _Tmp()
+    try:
+        _run(["rng", seed, count, t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), np.float32)
+    finally:
+        t.close()
+
+
+def ref_probe(seed):
+    t = _Tmp()
+    try:
+        _run(["probe", seed, t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), np.float32)
+    finally:
+        t.close()
+
+
+def ref_camera(cam_in):
+    t = _Tmp()
+    try:
+        _run(["camera", t.put("cam.bin", np.asarray(cam_in, np.float32)), t.path("out.bin")])
+        return np.fromfile(t.path("out.bin"), np.float32)
+    finally:
+        t.close()
+
+
+def ref_ao(nodes, tris, indices, cam_in, params):
+    t = _Tmp()
+    try:
+        _run(["ao", t.put("nodes.bin", nodes), nodes.size, t.put("tris.bin", tris), tris.size,
+              t.put("idx.bin", np.asarray(indices, np.uint32)), t.put("cam.bin", cam_in),
+              t.put("params.bin", params), t.path("out.bin")])
+        w, h = int(params[0]), int(params[1])
+        return np.fromfile(t.path("out.bin"), np.float32).reshape(h, w, 4)
+    finally:
+        t.close()
+
+
+def ref_pt(nodes, tris, indices, mats, cam_in, params):
+    t = _Tmp()
+    try:
+        _run(["pt", t.put("nodes.bin", nodes), nodes.size, t.put("tris.bin", tris), tris.size,
+              t.put("idx.bin", np.asarray(indices, np.uint32)), t.put("mats.bin", mats), mats.size,
+              t.put("cam.bin", cam_in), t.put("params.bin", params), t.path("out.bin")])
+        w, h = int(params[0]), int(params[1])
+        return np.fromfile(t.path("out.bin"), np.float32).reshape(h, w, 4)
+    finally:
+        t.close()
+
+
+def ref_sah(tris):
+    t = _Tmp()
+    try:
+        _run(["sah", t.put("tris.bin", tris), tris.size, t.path("nodes.bin"), t.path("idx.bin")])
+        return np.fromfile(t.path("nodes.bin"), NODE_DTYPE), np.fromfile(t.path("idx.bin"), np.uint32)
+    finally:
+        t.close()
+
+
+def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, binary=None, timeout=600):
+    """Time the reference CPU path (builder + tiled_sched frames)."""
+    t = _Tmp()
+    try:
+        out = t.path("bench.json")
+        args = ["bench", t.put("tris.bin", tris), tris.size, t.put("mats.bin", mats), mats.size,
+                t.put("cam.bin", cam_in), t.put("params.bin", params), packet, build, frames, out]
+        b = str(binary or ref_binary())
+        res = subprocess.run([b, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+        if res.returncode != 0:
+            raise RuntimeError(f"vr_ref bench failed: {res.stderr.strip()}")
+        with open(out) as f:
+            return json.load(f)
+    finally:
+        t.close()

@@ -1,0 +1,127 @@
+"""GPU closest-hit / any-hit traversal vs the oracle and the reference's
+golden hits: prim ids, t, u, v bit-exact on the same tree
+(detail/bvh/intersect.inl:166-356, math/intersect.h:176-229)."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from visionaray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+FMAX = np.finfo(np.float32).max
+RAY_DTYPE = np.dtype([("ori", "<f4", (3,)), ("dir", "<f4", (3,)), ("tmin", "<f4"), ("tmax", "<f4")])
+
+
+def random_rays(n, seed, lo=-0.9, hi=0.9, tmin=0.0, tmax=FMAX):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY_DTYPE)
+    r["ori"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+def assert_hits_equal(a, b):
+    assert np.array_equal(a["prim_id"], b["prim_id"])
+    hit = a["prim_id"] >= 0
+    assert np.array_equal(a["t"][hit], b["t"][hit])
+    assert np.array_equal(a["u"][hit], b["u"][hit])
+    assert np.array_equal(a["v"][hit], b["v"][hit])
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_golden_lbvh_hits(V, any_hit):
+    """The reference's own traversal on the same node array."""
+    g = load_golden("c0_small")
+    bvh = V.bvh_from_nodes(g["tris"], g["lbvh_nodes"], g["lbvh_indices"])
+    hits = V.trace(bvh, g["rays"], any_hit)
+    ref = g["ref_any"] if any_hit else g["ref_closest"]
+    if any_hit:
+        assert np.array_equal(hits["prim_id"] >= 0, ref["prim_id"] >= 0)
+    else:
+        assert_hits_equal(hits, ref)
+        assert np.array_equal(hits["prim_id"], g["ref_brute"]["prim_id"])
+
+
+def test_golden_sah_hits(V):
+    """The reference's binned-SAH tree (leaves of several primitives)."""
+    g = load_golden("c0_small")
+    bvh = V.bvh_from_nodes(g["tris"], g["sah_nodes"], g["sah_indices"])
+    assert bvh.info()["max_leaf_size"] > 1
+    hits = V.closest_hit(bvh, g["rays"])
+    assert_hits_equal(hits, g["ref_sah_closest"])
+
+
+@pytest.mark.parametrize("scene", ["c0_level5", "clusters", "heightfield"])
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_lbvh_trace_matches_oracle(V, O, scene, any_hit):
+    tris = {"c0_level5": lambda: S.config0_scene(level=5),
+            "clusters": lambda: S.random_clusters(40_000, seed=2),
+            "heightfield": lambda: S.heightfield(96, seed=1)}[scene]()
+    print(f"[case] trace {scene} any={any_hit} n={tris.size}", flush=True)
+    bvh = V.lbvh_builder().build(tris)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    rays = random_rays(30_000, seed=4)
+    got = V.trace(bvh, rays, any_hit)
+    want = O.trace(nodes, tris, idx, rays, any_hit)
+    assert_hits_equal(got, want)
+
+
+def test_ray_limits_and_axis_aligned_dirs(V, O):
+    """tmin/tmax windows, zero direction components (inv_dir = FLT_MAX) and
+    the axis-aligned room walls (flat boxes)."""
+    tris = S.config0_scene(level=3)
+    bvh = V.lbvh_builder().build(tris)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    rng = np.random.default_rng(8)
+    rays = random_rays(8192, seed=9, tmin=0.0, tmax=FMAX)
+    axis = rng.integers(0, 3, size=rays.size)
+    d = np.zeros((rays.size, 3), np.float32)
+    d[np.arange(rays.size), axis] = rng.choice([-1.0, 1.0], size=rays.size)
+    rays["dir"][: rays.size // 2] = d[: rays.size // 2]
+    rays["tmin"] = rng.uniform(0, 0.5, rays.size).astype(np.float32)
+    rays["tmax"] = rng.uniform(0.5, 3.0, rays.size).astype(np.float32)
+    for any_hit in (False, True):
+        got = V.trace(bvh, rays, any_hit)
+        want = O.trace(nodes, tris, idx, rays, any_hit)
+        assert_hits_equal(got, want)
+
+
+def test_intersect_kat_ray1tri4(V):
+    """test/unittests/math/intersect.cpp:60-121: ray along +x from (-2,0,0)
+    through four x = const triangles; the hits are prim 1 (t 0), 2 (t 1),
+    3 (t 2).  Traced here one triangle at a time through 1-leaf trees."""
+    xs = [-3.0, -2.0, -1.0, -0.0]
+    want = {1: 0.0, 2: 1.0, 3: 2.0}
+    for i, x in enumerate(xs):
+        tri = S.make_triangles(np.array([[x, -1, -1]]), np.array([[x, -1, 1]]), np.array([[x, 1, 0]]))
+        tri["prim_id"] = i
+        bvh = V.lbvh_builder().build(tri)
+        r = np.zeros(1, V.RAY_DTYPE)
+        r["ori"] = [-2, 0, 0]
+        r["dir"] = [1, 0, 0]
+        r["tmax"] = FMAX
+        h = V.closest_hit(bvh, r)[0]
+        if i in want:
+            assert h["prim_id"] == i and h["t"] == np.float32(want[i])
+        else:
+            assert h["prim_id"] == -1
+
+
+def test_get_normal_kat(V):
+    """test/unittests/get_normal.cpp:17-75: two triangles at z = +-1, ray from
+    (0.5,-0.5,2) along -z hits prim 0 at t = 1."""
+    a = np.array([[-1, -1, 1], [1, -1, -1]], np.float32)
+    b = np.array([[1, -1, 1], [-1, -1, -1]], np.float32)
+    c = np.array([[1, 1, 1], [-1, 1, -1]], np.float32)
+    tris = S.make_triangles(a, b, c)
+    tris["geom_id"] = [0, 1]
+    for bvh in (V.lbvh_builder().build(tris),):
+        r = np.zeros(1, V.RAY_DTYPE)
+        r["ori"] = [0.5, -0.5, 2.0]
+        r["dir"] = [0, 0, -1]
+        r["tmax"] = FMAX
+        h = V.closest_hit(bvh, r)[0]
+        assert h["prim_id"] == 0 and h["t"] == np.float32(1.0)

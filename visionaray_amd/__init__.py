@@ -1,0 +1,395 @@
+"""visionaray_amd -- MI355X (gfx950) implementation of Visionaray's ray-tracing
+hot path behind a C ABI, with a host mirror of the reference's interface.
+
+Reference names kept (reference files cited relative to include/visionaray/):
+    lbvh_builder().build(tree_type, prims, n)     detail/bvh/lbvh.h:681-844
+    bvh.refit(prims)                              detail/bvh/refit.h:28-114
+    closest_hit / any_hit                         detail/traverse_linear.inl:149-214
+    pinhole_camera                                pinhole_camera.h, detail/pinhole_camera.inl
+    make_kernel_params                            kernels.h:87-132
+    make_sched_params + jittered_blend_type       scheduler.h:103-194, pixel_sampler_types.h:51-69
+    pathtracing.kernel                            detail/pathtracing.inl:33-244
+    hip_sched(...).frame(kernel, sparams)         detail/cuda_sched.inl:226-246 (CUDA analogue)
+
+Device memory is handled through `DeviceArray` (vr_malloc / vr_memcpy); every
+compute call runs the HIP kernels of libvisionaray_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BVH_INFO_DTYPE, CAMERA_DTYPE, HIT_DTYPE, KERNEL_PARAMS_DTYPE, NODE_DTYPE, RAY_DTYPE,
+                   SCHED_PARAMS_DTYPE, VR_BVH_KEEP_NODES, VRError, check, ptr)
+from .scenes import MAT_DTYPE, TRI_DTYPE
+
+__all__ = [
+    "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build",
+    "closest_hit", "any_hit", "trace", "pinhole_camera", "make_kernel_params", "make_sched_params",
+    "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
+    "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "synchronize",
+]
+
+_initialized = False
+
+
+def lib():
+    return _lib.load()
+
+
+def init(device: int = 0):
+    """Select the HIP device (vr_init)."""
+    global _initialized
+    check(lib().vr_init(device))
+    _initialized = True
+
+
+def _ensure():
+    if not _initialized:
+        init(0)
+
+
+def synchronize():
+    check(lib().vr_synchronize())
+
+
+class DeviceArray:
+    """Owning device allocation holding `count` records of `dtype`."""
+
+    def __init__(self, count: int, dtype, zero: bool = False):
+        _ensure()
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        self._p = C.c_void_p()
+        check(lib().vr_malloc(C.byref(self._p), self.nbytes))
+        if zero and self.nbytes:
+            check(lib().vr_memset(self._p, 0, self.nbytes))
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if d.nbytes:
+            check(lib().vr_memcpy_htod(d._p, ptr(a), d.nbytes))
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.count
+        if self.nbytes:
+            check(lib().vr_memcpy_htod(self._p, ptr(a), self.nbytes))
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        if self.nbytes:
+            check(lib().vr_memcpy_dtoh(ptr(out), self._p, self.nbytes))
+        return out
+
+    @property
+    def ptr(self) -> C.c_void_p:
+        return self._p
+
+    def free(self):
+        if self._p and self._p.value:
+            lib().vr_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        _ensure()
+        self.handle = C.c_void_p()
+        check(lib().vr_stream_create(C.byref(self.handle)))
+
+    def synchronize(self):
+        check(lib().vr_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            lib().vr_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        _ensure()
+        self.handle = C.c_void_p()
+        check(lib().vr_event_create(C.byref(self.handle)))
+
+    def record(self, stream: Stream | None = None):
+        check(lib().vr_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().vr_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().vr_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _stream(s):
+    return s.handle if isinstance(s, Stream) else s
+
+
+def _as_device(a, dtype) -> DeviceArray:
+    if isinstance(a, DeviceArray):
+        return a
+    return DeviceArray.from_host(np.ascontiguousarray(a, dtype=dtype))
+
+
+class BVH:
+    """Handle of a device BVH (the analogue of hip_index_bvh<P>, bvh.h:1100)."""
+
+    def __init__(self, handle: C.c_void_p, prims: DeviceArray | None = None):
+        self.handle = handle
+        self.prims = prims
+
+    def info(self) -> dict:
+        a = np.zeros(1, BVH_INFO_DTYPE)
+        check(lib().vr_bvh_get_info(self.handle, ptr(a)))
+        return {k: int(a[k][0]) for k in a.dtype.names}
+
+    def nodes(self) -> np.ndarray:
+        """tree.nodes() in the reference bvh_node layout."""
+        n = self.info()["num_nodes"]
+        d = DeviceArray(n, NODE_DTYPE)
+        check(lib().vr_bvh_nodes(self.handle, d.ptr, None))
+        return d.to_host()
+
+    def indices(self) -> np.ndarray:
+        n = self.info()["num_prims"]
+        d = DeviceArray(n, np.uint32)
+        check(lib().vr_bvh_indices(self.handle, d.ptr, None))
+        return d.to_host()
+
+    def sorted_refs(self):
+        """(sorted Morton codes, original ids) of an LBVH build."""
+        n = self.info()["num_prims"]
+        c = DeviceArray(n, np.uint32)
+        i = DeviceArray(n, np.int32)
+        check(lib().vr_bvh_sorted_refs(self.handle, c.ptr, i.ptr, None))
+        return c.to_host(), i.to_host()
+
+    def refit(self, prims, stream=None):
+        """bvh_refitter::refit (detail/bvh/refit.h:28-114) for moved vertices."""
+        p = _as_device(prims, TRI_DTYPE)
+        check(lib().vr_bvh_refit(self.handle, p.ptr, _stream(stream)))
+        self.prims = p
+
+    def destroy(self):
+        if self.handle and self.handle.value:
+            lib().vr_bvh_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class lbvh_builder:
+    """GPU LBVH builder (lbvh_builder::build, detail/bvh/lbvh.h:681-844)."""
+
+    def build(self, prims, n: int | None = None, keep_nodes: bool = True, stream=None) -> BVH:
+        p = _as_device(prims, TRI_DTYPE)
+        n = p.count if n is None else int(n)
+        h = C.c_void_p()
+        check(lib().vr_lbvh_build(p.ptr, n, VR_BVH_KEEP_NODES if keep_nodes else 0, _stream(stream), C.byref(h)))
+        return BVH(h, p)
+
+    @staticmethod
+    def morton_codes(prims, stream=None) -> np.ndarray:
+        """assign_morton_codes alone (detail/bvh/lbvh.h:242-269), unsorted."""
+        p = _as_device(prims, TRI_DTYPE)
+        d = DeviceArray(p.count, np.uint32)
+        check(lib().vr_lbvh_morton(p.ptr, p.count, d.ptr, _stream(stream)))
+        return d.to_host()
+
+
+def bvh_from_nodes(prims: np.ndarray, nodes: np.ndarray, indices: np.ndarray) -> BVH:
+    """Adopt a reference-layout tree built on the host (index_bvh_t, bvh.h:870-1010)."""
+    _ensure()
+    prims = np.ascontiguousarray(prims, TRI_DTYPE)
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    h = C.c_void_p()
+    check(lib().vr_bvh_from_nodes(ptr(prims), prims.size, ptr(nodes), nodes.size, ptr(indices), C.byref(h)))
+    return BVH(h)
+
+
+def sah_build(prims: np.ndarray, max_leaf_size: int = 4):
+    """Host binned SAH build (binned_sah_builder, detail/bvh/sah.h:176-807):
+    returns (nodes, indices) in the reference layout."""
+    prims = np.ascontiguousarray(prims, TRI_DTYPE)
+    n = prims.size
+    nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
+    idx = np.zeros(max(n, 1), np.uint32)
+    nn = C.c_size_t()
+    check(lib().vr_sah_build(ptr(prims), n, max_leaf_size, ptr(nodes), C.byref(nn), ptr(idx)))
+    return nodes[: nn.value].copy(), idx[:n].copy()
+
+
+def trace(bvh: BVH, rays, any_hit_: bool = False, counted: bool = False, stream=None):
+    """Trace rays (numpy RAY_DTYPE or DeviceArray); returns host hits (and
+    per-ray [pairs, triangles] test counts when counted)."""
+    r = _as_device(rays, RAY_DTYPE)
+    hits = DeviceArray(r.count, HIT_DTYPE)
+    if counted:
+        cnt = DeviceArray(2 * r.count, np.uint32)
+        check(lib().vr_trace_counted(bvh.handle, int(any_hit_), r.ptr, r.count, hits.ptr, cnt.ptr, _stream(stream)))
+        return hits.to_host(), cnt.to_host().reshape(-1, 2)
+    check(lib().vr_trace(bvh.handle, int(any_hit_), r.ptr, r.count, hits.ptr, _stream(stream)))
+    return hits.to_host()
+
+
+def closest_hit(bvh: BVH, rays, stream=None) -> np.ndarray:
+    return trace(bvh, rays, False, stream=stream)
+
+
+def any_hit(bvh: BVH, rays, stream=None) -> np.ndarray:
+    return trace(bvh, rays, True, stream=stream)
+
+
+def pinhole_camera(eye, center, up, fovy: float, aspect: float) -> np.ndarray:
+    """perspective + look_at + begin_frame (detail/pinhole_camera.inl); returns
+    the CAMERA_DTYPE record {eye, U, V, W}."""
+    cam = np.zeros(1, CAMERA_DTYPE)
+    e = np.asarray(eye, np.float32)
+    c = np.asarray(center, np.float32)
+    u = np.asarray(up, np.float32)
+    check(lib().vr_camera_look_at(ptr(e), ptr(c), ptr(u), float(fovy), float(aspect), ptr(cam)))
+    return cam
+
+
+def make_kernel_params(num_bounces: int = 5, epsilon: float = float(np.finfo(np.float32).eps),
+                       bg_color=(0.0, 0.0, 0.0, 0.0), ambient_color=(0.0, 0.0, 0.0, 0.0)) -> np.ndarray:
+    """kernels.h:87-132 without lights; colors are (r, g, b, kl) and the
+    intensity passed on is rgb * kl (ambient_light::intensity)."""
+    kp = np.zeros(1, KERNEL_PARAMS_DTYPE)
+    kp["num_bounces"] = num_bounces
+    kp["epsilon"] = np.float32(epsilon)
+    bg = np.asarray(bg_color, np.float32)
+    am = np.asarray(ambient_color, np.float32)
+    kp["bg"] = bg[:3] * bg[3]
+    kp["ambient"] = am[:3] * am[3]
+    return kp
+
+
+@dataclass
+class jittered_blend_type:
+    """pixel_sampler::jittered_blend_type (pixel_sampler_types.h:51-69)."""
+    spp: int = 1
+    sfactor: float = 1.0
+    dfactor: float = 0.0
+
+
+@dataclass
+class render_target:
+    """RGBA32F color + accum render target on the device."""
+    width: int
+    height: int
+    color: DeviceArray = field(init=False)
+    accum: DeviceArray = field(init=False)
+
+    def __post_init__(self):
+        self.color = DeviceArray(self.width * self.height * 4, np.float32, zero=True)
+        self.accum = DeviceArray(self.width * self.height * 4, np.float32, zero=True)
+
+    def color_host(self) -> np.ndarray:
+        return self.color.to_host().reshape(self.height, self.width, 4)
+
+
+@dataclass
+class sched_params:
+    sample_params: jittered_blend_type
+    cam: np.ndarray
+    rt: render_target
+
+
+def make_sched_params(sample_params: jittered_blend_type, cam: np.ndarray, rt: render_target) -> sched_params:
+    """scheduler.h:103-194 (pixel sampler, camera, render target)."""
+    return sched_params(sample_params, cam, rt)
+
+
+class pathtracing:
+    @dataclass
+    class kernel:
+        """pathtracing::kernel (detail/pathtracing.inl:33-244): BVH, materials
+        indexed by geom_id, kernel params."""
+        bvh: BVH
+        materials: DeviceArray
+        params: np.ndarray
+
+
+@dataclass
+class ao_kernel:
+    """The AO kernel of src/examples/ao/main.cpp:192-260."""
+    bvh: BVH
+    samples: int = 8
+    radius: float = 0.1
+    bg: tuple = (0.0, 0.0, 0.0)
+
+
+class hip_sched:
+    """Scheduler launching one frame of a kernel (cuda_sched<R>::frame,
+    detail/cuda_sched.inl:226-246): per-pixel seeds make_random_seed(y*w+x,
+    frame_id), frame_id incremented after every frame."""
+
+    def __init__(self):
+        _ensure()
+        self.frame_id = 0
+        self.ray_counts = DeviceArray(2, np.uint64, zero=True)
+
+    def _sp(self, sp: sched_params) -> np.ndarray:
+        a = np.zeros(1, SCHED_PARAMS_DTYPE)
+        a["width"] = sp.rt.width
+        a["height"] = sp.rt.height
+        a["spp"] = sp.sample_params.spp
+        a["frame_id"] = self.frame_id
+        a["sfactor"] = sp.sample_params.sfactor
+        a["dfactor"] = sp.sample_params.dfactor
+        return a
+
+    def frame(self, kernel, sp: sched_params, stream=None, stats: "DeviceArray | None" = None):
+        """One frame; with `stats` (4 x uint64 device array) the counting
+        variant also accumulates node-pair and triangle tests."""
+        s = self._sp(sp)
+        if isinstance(kernel, pathtracing.kernel) and stats is not None:
+            check(lib().vr_render_pt_counted(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,
+                                             ptr(kernel.params), ptr(sp.cam), ptr(s), sp.rt.accum.ptr,
+                                             sp.rt.color.ptr, stats.ptr, _stream(stream)))
+        elif isinstance(kernel, pathtracing.kernel):
+            check(lib().vr_render_pt(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,
+                                     ptr(kernel.params), ptr(sp.cam), ptr(s), sp.rt.accum.ptr, sp.rt.color.ptr,
+                                     self.ray_counts.ptr, _stream(stream)))
+        elif isinstance(kernel, ao_kernel):
+            bg = np.asarray(kernel.bg, np.float32)
+            check(lib().vr_render_ao(kernel.bvh.handle, ptr(sp.cam), ptr(s), kernel.samples, float(kernel.radius),
+                                     ptr(bg), sp.rt.accum.ptr, sp.rt.color.ptr, _stream(stream)))
+        else:
+            raise TypeError(f"unsupported kernel {type(kernel).__name__}")
+        self.frame_id += 1
+
+    def rays(self) -> tuple[int, int]:
+        """(primary, secondary) closest-hit queries accumulated so far."""
+        c = self.ray_counts.to_host()
+        return int(c[0]), int(c[1])
+
+    def reset_counters(self):
+        check(lib().vr_memset(self.ray_counts.ptr, 0, self.ray_counts.nbytes))

@@ -1,0 +1,510 @@
+// capi.hip -- the C ABI (include/visionaray_amd.h): status codes, handles,
+// device memory helpers and the host-side parts of the path (camera basis,
+// conversion of reference-layout BVHs into the traversal layout).
+#include "vr_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace vr {
+
+int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, hipStream_t st);
+int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st);
+int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st);
+int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
+                hipStream_t st);
+int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+              const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
+              unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st);
+int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& sp, uint32_t samples, float radius,
+              const float bg[3], float* accum, float* color, hipStream_t st);
+int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                   uint32_t* indices);
+
+namespace {
+thread_local std::string g_error;
+
+struct device_state {
+    bool ready = false;
+    uint32_t* work = nullptr;  // persistent-kernel work counter
+};
+std::mutex g_mutex;
+device_state g_dev[64];
+
+device_state* dev_state()
+{
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return nullptr;
+    return &g_dev[d];
+}
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int current_device()
+{
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+
+}  // namespace vr
+
+using namespace vr;
+
+struct vr_bvh_t {
+    bvh_state s;
+};
+
+#define VR_TRY_BEGIN try {
+#define VR_TRY_END                                                   \
+    }                                                                \
+    catch (const std::bad_alloc&) { return fail(VR_ERR_OUT_OF_MEMORY, "host allocation failed"); } \
+    catch (const std::exception& e) { return fail(VR_ERR_INVALID_ARGUMENT, e.what()); }
+
+static int ensure_device()
+{
+    device_state* ds = dev_state();
+    VR_CHECK(ds != nullptr, VR_ERR_HIP, "no current HIP device");
+    if (ds->ready) return VR_OK;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ds->ready) {
+        VR_HIP_CHECK(hipMalloc(&ds->work, 256));
+        ds->ready = true;
+    }
+    return VR_OK;
+}
+
+extern "C" {
+
+int vr_init(int device)
+{
+    int count = 0;
+    VR_HIP_CHECK(hipGetDeviceCount(&count));
+    VR_CHECK(device >= 0 && device < count, VR_ERR_INVALID_ARGUMENT, "vr_init: no such device");
+    VR_HIP_CHECK(hipSetDevice(device));
+    return ensure_device();
+}
+
+const char* vr_last_error(void) { return g_error.c_str(); }
+
+int vr_version(int* major, int* minor)
+{
+    VR_CHECK(major && minor, VR_ERR_INVALID_ARGUMENT, "vr_version: null argument");
+    *major = 0;
+    *minor = 1;
+    return VR_OK;
+}
+
+int vr_malloc(void** d_ptr, size_t bytes)
+{
+    VR_CHECK(d_ptr, VR_ERR_INVALID_ARGUMENT, "vr_malloc: null out pointer");
+    *d_ptr = nullptr;
+    if (bytes == 0) return VR_OK;
+    hipError_t e = hipMalloc(d_ptr, bytes);
+    if (e == hipErrorOutOfMemory) return fail(VR_ERR_OUT_OF_MEMORY, "vr_malloc: out of device memory");
+    VR_HIP_CHECK(e);
+    return VR_OK;
+}
+
+int vr_free(void* d_ptr)
+{
+    if (d_ptr) VR_HIP_CHECK(hipFree(d_ptr));
+    return VR_OK;
+}
+
+int vr_memcpy_htod(void* d_dst, const void* h_src, size_t bytes)
+{
+    if (bytes == 0) return VR_OK;
+    VR_CHECK(d_dst && h_src, VR_ERR_INVALID_ARGUMENT, "vr_memcpy_htod: null pointer");
+    VR_HIP_CHECK(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    return VR_OK;
+}
+
+int vr_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes)
+{
+    if (bytes == 0) return VR_OK;
+    VR_CHECK(h_dst && d_src, VR_ERR_INVALID_ARGUMENT, "vr_memcpy_dtoh: null pointer");
+    VR_HIP_CHECK(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_memset(void* d_dst, int value, size_t bytes)
+{
+    if (bytes == 0) return VR_OK;
+    VR_CHECK(d_dst, VR_ERR_INVALID_ARGUMENT, "vr_memset: null pointer");
+    VR_HIP_CHECK(hipMemset(d_dst, value, bytes));
+    return VR_OK;
+}
+
+int vr_synchronize(void)
+{
+    VR_HIP_CHECK(hipDeviceSynchronize());
+    return VR_OK;
+}
+
+int vr_stream_create(void** stream)
+{
+    VR_CHECK(stream, VR_ERR_INVALID_ARGUMENT, "vr_stream_create: null out pointer");
+    hipStream_t s;
+    VR_HIP_CHECK(hipStreamCreate(&s));
+    *stream = s;
+    return VR_OK;
+}
+
+int vr_stream_destroy(void* stream)
+{
+    if (stream) VR_HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_stream_synchronize(void* stream)
+{
+    VR_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_event_create(void** event)
+{
+    VR_CHECK(event, VR_ERR_INVALID_ARGUMENT, "vr_event_create: null out pointer");
+    hipEvent_t e;
+    VR_HIP_CHECK(hipEventCreate(&e));
+    *event = e;
+    return VR_OK;
+}
+
+int vr_event_destroy(void* event)
+{
+    if (event) VR_HIP_CHECK(hipEventDestroy((hipEvent_t)event));
+    return VR_OK;
+}
+
+int vr_event_record(void* event, void* stream)
+{
+    VR_CHECK(event, VR_ERR_INVALID_ARGUMENT, "vr_event_record: null event");
+    VR_HIP_CHECK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    VR_CHECK(start && stop && ms, VR_ERR_INVALID_ARGUMENT, "vr_event_elapsed_ms: null argument");
+    VR_HIP_CHECK(hipEventSynchronize((hipEvent_t)stop));
+    VR_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return VR_OK;
+}
+
+// pinhole_camera::look_at + perspective + begin_frame
+// (detail/pinhole_camera.inl:12-29, 31-64, 87-97), host float arithmetic
+int vr_camera_look_at(const float eye[3], const float center[3], const float up[3], float fovy, float aspect,
+                      vr_camera* out)
+{
+    VR_CHECK(eye && center && up && out, VR_ERR_INVALID_ARGUMENT, "vr_camera_look_at: null argument");
+    const f3 e = mk3(eye[0], eye[1], eye[2]);
+    const f3 c = mk3(center[0], center[1], center[2]);
+    const f3 u0 = mk3(up[0], up[1], up[2]);
+    const f3 f = normalize3(sub3(e, c));
+    const f3 s = normalize3(cross3(u0, f));
+    const f3 u = cross3(f, s);
+    const float th = std::tan(fovy / 2.0f);
+    const f3 U = muls3(muls3(s, th), aspect);
+    const f3 V = muls3(u, std::tan(fovy / 2.0f));
+    const f3 W = neg3(f);
+    const float vals[12] = {e.x, e.y, e.z, U.x, U.y, U.z, V.x, V.y, V.z, W.x, W.y, W.z};
+    std::memcpy(out, vals, sizeof(vals));
+    return VR_OK;
+}
+
+int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* stream, vr_bvh* out)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(out, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build: null out handle");
+    VR_CHECK(n == 0 || d_prims, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build: null primitives");
+    int rc = ensure_device();
+    if (rc) return rc;
+    vr_bvh h = new vr_bvh_t;
+    h->s.device = current_device();
+    rc = lbvh_build_into(h->s, d_prims, n, (flags & VR_BVH_KEEP_NODES) != 0, (hipStream_t)stream);
+    if (rc) {
+        vr_bvh_destroy(h);
+        return rc;
+    }
+    // a Karras tree has at most 65 levels: the common prefix of a node's range
+    // (30 code bits + 32 index bits) grows strictly from parent to child
+    h->s.depth = n <= 1 ? (uint32_t)n : 65u;
+    *out = h;
+    return VR_OK;
+    VR_TRY_END
+}
+
+int vr_lbvh_morton(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, void* stream)
+{
+    VR_CHECK(n == 0 || (d_prims && d_codes), VR_ERR_INVALID_ARGUMENT, "vr_lbvh_morton: null pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return lbvh_morton_only(d_prims, n, d_codes, (hipStream_t)stream);
+}
+
+int vr_bvh_sorted_refs(vr_bvh bvh, uint32_t* d_codes, int32_t* d_ids, void* stream)
+{
+    VR_CHECK(bvh, VR_ERR_INVALID_ARGUMENT, "vr_bvh_sorted_refs: null handle");
+    VR_CHECK(bvh->s.is_lbvh, VR_ERR_UNSUPPORTED, "vr_bvh_sorted_refs: not an LBVH");
+    const size_t n = bvh->s.num_prims;
+    if (n == 0) return VR_OK;
+    if (d_codes)
+        VR_HIP_CHECK(hipMemcpyAsync(d_codes, bvh->s.sorted_codes, n * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    if (d_ids)
+        VR_HIP_CHECK(hipMemcpyAsync(d_ids, bvh->s.sorted_ids, n * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return VR_OK;
+}
+
+// Adopt a reference-layout tree: inner nodes become pair records in
+// depth-first order, leaf triangles are copied into traversal order.
+int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh_node* h_nodes, size_t num_nodes,
+                      const uint32_t* h_indices, vr_bvh* out)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(out, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: null out handle");
+    VR_CHECK(num_prims == 0 || (h_prims && h_nodes && h_indices && num_nodes > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_bvh_from_nodes: null array");
+    VR_CHECK(num_prims <= (size_t)MAX_LEAF_FIRST, VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: too many primitives");
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::vector<pair_node> pairs;
+    std::vector<tri_node> tris;
+    tris.reserve(num_prims);
+    uint32_t depth = 0, max_leaf = 1, root = 0;
+    if (num_prims > 0) {
+        // iterative DFS over (node, depth); inner nodes get pair indices in visit order
+        struct item {
+            uint32_t node;
+            uint32_t depth;
+            int64_t parent_pair;  // pair whose child slot receives this node
+            uint32_t side;
+        };
+        std::vector<item> stack{{0u, 1u, -1, 0u}};
+        // append the leaf's triangles in index order (ranges checked by the caller)
+        auto add_leaf = [&](const vr_bvh_node& nd) -> uint32_t {
+            const uint32_t first = (uint32_t)tris.size();
+            for (uint32_t k = nd.first; k < nd.first + nd.num_prims; ++k) {
+                const uint32_t id = h_indices[k];
+                const vr_triangle& t = h_prims[id];
+                float a4, b4, c4;
+                std::memcpy(&a4, &t.prim_id, 4);
+                std::memcpy(&b4, &t.geom_id, 4);
+                std::memcpy(&c4, &id, 4);
+                tri_node tn;
+                tn.a = make_float4(t.v1[0], t.v1[1], t.v1[2], a4);
+                tn.b = make_float4(t.e1[0], t.e1[1], t.e1[2], b4);
+                tn.c = make_float4(t.e2[0], t.e2[1], t.e2[2], c4);
+                tris.push_back(tn);
+            }
+            max_leaf = std::max<uint32_t>(max_leaf, nd.num_prims);
+            return leaf_entry(first, nd.num_prims);
+        };
+        while (!stack.empty()) {
+            item it = stack.back();
+            stack.pop_back();
+            VR_CHECK(it.node < num_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: child index out of range");
+            VR_CHECK(it.depth <= 4096, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: tree too deep (cycle?)");
+            const vr_bvh_node& nd = h_nodes[it.node];
+            depth = std::max(depth, it.depth);
+            uint32_t entry;
+            if (nd.num_prims) {
+                VR_CHECK(nd.num_prims <= MAX_LEAF_COUNT, VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: leaf larger than 32");
+                VR_CHECK((size_t)nd.first + nd.num_prims <= num_prims, VR_ERR_INVALID_ARGUMENT,
+                         "vr_bvh_from_nodes: leaf range out of bounds");
+                for (uint32_t k = nd.first; k < nd.first + nd.num_prims; ++k)
+                    VR_CHECK(h_indices[k] < num_prims, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: bad index");
+                entry = add_leaf(nd);
+            } else {
+                VR_CHECK(nd.first + 1 < num_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: child index out of range");
+                const uint32_t p = (uint32_t)pairs.size();
+                pair_node pn;
+                std::memset(&pn, 0, sizeof(pn));
+                for (int side = 0; side < 2; ++side) {
+                    const vr_bvh_node& c = h_nodes[nd.first + side];
+                    for (int k = 0; k < 3; ++k) {
+                        pn.f[side * 6 + k] = c.lo[k];
+                        pn.f[side * 6 + 3 + k] = c.hi[k];
+                    }
+                }
+                pn.e[2] = it.parent_pair < 0 ? 0xFFFFFFFFu : (uint32_t)((it.parent_pair << 1) | it.side);
+                pairs.push_back(pn);
+                entry = p;
+                // push right first so the left subtree is numbered first
+                stack.push_back({nd.first + 1, it.depth + 1, (int64_t)p, 1u});
+                stack.push_back({nd.first, it.depth + 1, (int64_t)p, 0u});
+            }
+            if (it.parent_pair < 0) root = entry;
+            else pairs[(size_t)it.parent_pair].e[it.side] = entry;
+        }
+    }
+    vr_bvh h = new vr_bvh_t;
+    bvh_state& s = h->s;
+    s.device = current_device();
+    s.num_prims = num_prims;
+    s.num_pairs = pairs.size();
+    s.root = root;
+    s.depth = depth;
+    s.max_leaf = max_leaf;
+    s.is_lbvh = false;
+    auto bail = [&](int code) {
+        vr_bvh_destroy(h);
+        return code;
+    };
+    if (depth > (uint32_t)MAX_STACK + 1) return bail(fail(VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: tree deeper than 129 levels"));
+    if (num_prims > 0) {
+        if (hipMalloc(&s.pairs, std::max<size_t>(pairs.size(), 1) * sizeof(pair_node)) != hipSuccess ||
+            hipMalloc(&s.tris, tris.size() * sizeof(tri_node)) != hipSuccess ||
+            hipMalloc(&s.prims, num_prims * sizeof(vr_triangle)) != hipSuccess ||
+            hipMalloc(&s.ref_nodes, num_nodes * sizeof(vr_bvh_node)) != hipSuccess ||
+            hipMalloc(&s.indices, num_prims * 4) != hipSuccess)
+            return bail(fail(VR_ERR_OUT_OF_MEMORY, "vr_bvh_from_nodes: device allocation failed"));
+        s.num_ref_nodes = num_nodes;
+        if ((pairs.size() && hipMemcpy(s.pairs, pairs.data(), pairs.size() * sizeof(pair_node), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(s.tris, tris.data(), tris.size() * sizeof(tri_node), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s.prims, h_prims, num_prims * sizeof(vr_triangle), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s.ref_nodes, h_nodes, num_nodes * sizeof(vr_bvh_node), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s.indices, h_indices, num_prims * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(VR_ERR_HIP, "vr_bvh_from_nodes: upload failed"));
+    }
+    *out = h;
+    return VR_OK;
+    VR_TRY_END
+}
+
+int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream)
+{
+    VR_CHECK(bvh && (bvh->s.num_prims == 0 || d_prims), VR_ERR_INVALID_ARGUMENT, "vr_bvh_refit: null argument");
+    VR_CHECK(bvh->s.is_lbvh, VR_ERR_UNSUPPORTED, "vr_bvh_refit: only LBVHs are refitted on the device");
+    return lbvh_refit(bvh->s, d_prims, (hipStream_t)stream);
+}
+
+int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info)
+{
+    VR_CHECK(bvh && info, VR_ERR_INVALID_ARGUMENT, "vr_bvh_get_info: null argument");
+    const bvh_state& s = bvh->s;
+    info->num_prims = s.num_prims;
+    info->num_nodes = s.is_lbvh ? (s.num_prims ? 2 * s.num_prims - 1 : 0) : s.num_ref_nodes;
+    info->max_leaf_size = s.max_leaf;
+    info->depth = s.depth;
+    if (s.is_lbvh && s.num_prims > 1) {
+        uint32_t d = 0;
+        VR_HIP_CHECK(hipDeviceSynchronize());
+        VR_HIP_CHECK(hipMemcpy(&d, s.root_box + 6, 4, hipMemcpyDeviceToHost));
+        info->depth = d;
+    }
+    return VR_OK;
+}
+
+int vr_bvh_nodes(vr_bvh bvh, vr_bvh_node* d_nodes, void* stream)
+{
+    VR_CHECK(bvh && d_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_nodes: null argument");
+    const bvh_state& s = bvh->s;
+    if (s.num_prims == 0) return VR_OK;
+    VR_CHECK(s.ref_nodes, VR_ERR_UNSUPPORTED, "vr_bvh_nodes: build with VR_BVH_KEEP_NODES to keep the node layout");
+    const size_t nn = s.is_lbvh ? 2 * s.num_prims - 1 : s.num_ref_nodes;
+    VR_HIP_CHECK(hipMemcpyAsync(d_nodes, s.ref_nodes, nn * sizeof(vr_bvh_node), hipMemcpyDeviceToDevice,
+                                (hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_bvh_indices(vr_bvh bvh, uint32_t* d_indices, void* stream)
+{
+    VR_CHECK(bvh && d_indices, VR_ERR_INVALID_ARGUMENT, "vr_bvh_indices: null argument");
+    const bvh_state& s = bvh->s;
+    if (s.num_prims == 0) return VR_OK;
+    VR_HIP_CHECK(hipMemcpyAsync(d_indices, s.indices, s.num_prims * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_bvh_destroy(vr_bvh bvh)
+{
+    if (!bvh) return VR_OK;
+    bvh_state& s = bvh->s;
+    void* ptrs[] = {s.pairs, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents, s.flags,
+                    s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete bvh;
+    return VR_OK;
+}
+
+int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes, size_t* num_nodes,
+                 uint32_t* h_indices)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(num_nodes && (n == 0 || (h_prims && h_nodes && h_indices)), VR_ERR_INVALID_ARGUMENT,
+             "vr_sah_build: null argument");
+    return sah_build_host(h_prims, n, max_leaf_size, h_nodes, num_nodes, h_indices);
+    VR_TRY_END
+}
+
+int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)
+{
+    VR_CHECK(bvh && (n == 0 || (d_rays && d_hits)), VR_ERR_INVALID_ARGUMENT, "vr_trace: null argument");
+    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, nullptr, (hipStream_t)stream);
+}
+
+int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, uint32_t* d_counts,
+                     void* stream)
+{
+    VR_CHECK(bvh && (n == 0 || (d_rays && d_hits && d_counts)), VR_ERR_INVALID_ARGUMENT,
+             "vr_trace_counted: null argument");
+    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, d_counts, (hipStream_t)stream);
+}
+
+int vr_render_ao(vr_bvh bvh, const vr_camera* cam, const vr_sched_params* sp, uint32_t ao_samples, float ao_radius,
+                 const float bg[3], float* d_accum, float* d_color, void* stream)
+{
+    VR_CHECK(bvh && cam && sp && bg && d_accum && d_color, VR_ERR_INVALID_ARGUMENT, "vr_render_ao: null argument");
+    VR_CHECK(sp->spp >= 1 && ao_samples >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_ao: spp and samples must be >= 1");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_ao(bvh->s, *cam, *sp, ao_samples, ao_radius, bg, d_accum, d_color, (hipStream_t)stream);
+}
+
+int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                 const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                 uint64_t* d_ray_counts, void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_accum && d_color, VR_ERR_INVALID_ARGUMENT, "vr_render_pt: null argument");
+    VR_CHECK(bvh->s.num_prims == 0 || (d_materials && num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_pt: spp must be >= 1");
+    VR_CHECK((uint64_t)sp->width * sp->height < (1ull << 31), VR_ERR_INVALID_ARGUMENT, "vr_render_pt: image too large");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
+                     reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work, false,
+                     (hipStream_t)stream);
+}
+
+int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                         const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                         uint64_t* d_stats, void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_accum && d_color && d_stats, VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_counted: null argument");
+    VR_CHECK(bvh->s.num_prims == 0 || (d_materials && num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_counted: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_counted: spp must be >= 1");
+    VR_CHECK((uint64_t)sp->width * sp->height < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_counted: image too large");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
+                     reinterpret_cast<unsigned long long*>(d_stats), dev_state()->work, true, (hipStream_t)stream);
+}
+
+}  // extern "C"

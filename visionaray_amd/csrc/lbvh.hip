@@ -1,0 +1,665 @@
+// lbvh.hip -- LBVH construction on gfx950.
+//
+// Same tree as the reference GPU builder (detail/bvh/lbvh.h:681-844), built
+// MI355X-first:
+//   1. centroid bounds: wave64 shuffle reduction, one ordered-int atomic per
+//      component per block (the reference does 6 float CAS loops per
+//      primitive on 6 global words, lbvh.h:63-91, 233-238)
+//   2. 30-bit Morton codes (lbvh.h:242-269), bit-exact
+//   3. stable LSD radix sort of (code, id), 4 passes of 8 bits with wave64
+//      ballot-match ranking and LDS staging so the scatter writes runs of one
+//      digit contiguously (the reference calls CUB's merge sort, lbvh.h:740-755;
+//      a stable sort of ids 0..n-1 has exactly one result)
+//   4. Karras 2012 hierarchy (lbvh.h:120-185, 281-330) with exact integer
+//      step sizes, writing children straight into the traversal records
+//   5. bottom-up boxes where the second thread to reach a node computes its
+//      box (one atomic per node instead of lbvh.h:332-372's six float CAS
+//      atomics per level per leaf) -- the boxes are the same exact min/max
+//      unions -- fused with the collapse to the reference bvh_node layout
+//      (lbvh.h:374-451) and the copy of triangles into traversal order.
+#include "vr_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace vr {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+constexpr int RADIX = 256;
+constexpr int ITEMS = 8;
+constexpr int TILE = BLOCK * ITEMS;  // keys per sort tile
+
+// float <-> order-preserving unsigned (for integer atomic min/max)
+__device__ inline uint32_t f2o(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float o2f(uint32_t o)
+{
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __uint_as_float(u);
+}
+
+__device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// number of set bits of m below this lane
+__device__ inline uint32_t mbcnt64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// load a reference triangle (44-byte stride, 4-byte aligned)
+struct tri_in {
+    uint32_t geom_id, prim_id;
+    f3 v1, e1, e2;
+};
+__device__ inline tri_in load_tri(const vr_triangle* __restrict__ p, size_t i)
+{
+    const float* q = reinterpret_cast<const float*>(p + i);
+    tri_in t;
+    t.geom_id = __float_as_uint(q[0]);
+    t.prim_id = __float_as_uint(q[1]);
+    t.v1 = mk3(q[2], q[3], q[4]);
+    t.e1 = mk3(q[5], q[6], q[7]);
+    t.e2 = mk3(q[8], q[9], q[10]);
+    return t;
+}
+
+// get_bounds(triangle) (math/detail/triangle.inl:41-53)
+__device__ inline void tri_box(const tri_in& t, f3& lo, f3& hi)
+{
+    lo = mk3(FLT_MAX, FLT_MAX, FLT_MAX);
+    hi = mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    const f3 p0 = t.v1, p1 = add3(t.v1, t.e1), p2 = add3(t.v1, t.e2);
+    lo = min3v(lo, p0); hi = max3v(hi, p0);
+    lo = min3v(lo, p1); hi = max3v(hi, p1);
+    lo = min3v(lo, p2); hi = max3v(hi, p2);
+}
+
+// aabb::center (math/detail/aabb.inl:56-61)
+__device__ inline f3 tri_centroid(const tri_in& t)
+{
+    f3 lo, hi;
+    tri_box(t, lo, hi);
+    return muls3(add3(hi, lo), 0.5f);
+}
+
+__global__ void k_init_bounds(uint32_t* cb)
+{
+    if (threadIdx.x < 3) cb[threadIdx.x] = f2o(FLT_MAX);
+    else if (threadIdx.x < 6) cb[threadIdx.x] = f2o(-FLT_MAX);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_centroid_bounds(const vr_triangle* __restrict__ prims, size_t n,
+                                                           uint32_t* __restrict__ cb)
+{
+    float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        f3 c = tri_centroid(load_tri(prims, i));
+        v[0] = fminf(v[0], c.x); v[1] = fminf(v[1], c.y); v[2] = fminf(v[2], c.z);
+        v[3] = fmaxf(v[3], c.x); v[4] = fmaxf(v[4], c.y); v[5] = fmaxf(v[5], c.z);
+    }
+    // min/max are exact, so any reduction order gives the same values
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], __shfl_xor(v[k], off));
+        for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], __shfl_xor(v[k], off));
+    }
+    __shared__ float s[WAVES][6];
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k) s[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float r = s[0][k];
+        for (int q = 1; q < WAVES; ++q) r = k < 3 ? fminf(r, s[q][k]) : fmaxf(r, s[q][k]);
+        if (k < 3) atomicMin(&cb[k], f2o(r));
+        else atomicMax(&cb[k], f2o(r));
+    }
+}
+
+// morton_encode3D (morton.h:33-47)
+__device__ inline uint32_t separate_bits3(uint32_t n)
+{
+    n &= 0x000003FFu;
+    n = (n ^ (n << 16)) & 0xFF0000FFu;
+    n = (n ^ (n << 8)) & 0x0300F00Fu;
+    n = (n ^ (n << 4)) & 0x030C30C3u;
+    n = (n ^ (n << 2)) & 0x09249249u;
+    return n;
+}
+
+// assign_morton_codes (detail/bvh/lbvh.h:242-269): centroid relative to the
+// centroid box, scaled by 1024, clamped to [0, 1023] with the ternary min/max
+// (a degenerate axis gives 0/0 = NaN, which the clamp maps to 1023), then
+// truncated and interleaved.
+__global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict__ prims, size_t n,
+                                                  const uint32_t* __restrict__ cb, uint32_t* __restrict__ codes)
+{
+    const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const f3 lo = mk3(o2f(cb[0]), o2f(cb[1]), o2f(cb[2]));
+    const f3 hi = mk3(o2f(cb[3]), o2f(cb[4]), o2f(cb[5]));
+    const f3 center = muls3(add3(hi, lo), 0.5f);
+    const f3 size = sub3(hi, lo);
+    f3 c = tri_centroid(load_tri(prims, i));
+    c = sub3(c, center);
+    c = add3(c, muls3(size, 0.5f));
+    c = f3{c.x / size.x, c.y / size.y, c.z / size.z};
+    c = min3v(max3v(muls3(c, 1024.0f), mk3(0.0f, 0.0f, 0.0f)), mk3(1023.0f, 1023.0f, 1023.0f));
+    codes[i] = separate_bits3((uint32_t)c.x) | (separate_bits3((uint32_t)c.y) << 1) |
+               (separate_bits3((uint32_t)c.z) << 2);
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits
+// ---------------------------------------------------------------------------
+
+// lanes of this wave holding the same 8-bit digit (among `valid` lanes)
+__device__ inline uint64_t match_digit(uint32_t d, bool valid)
+{
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// per-tile digit histogram, digit-major: hist[d * num_tiles + tile]
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, size_t n, int shift,
+                                                      uint32_t* __restrict__ hist, uint32_t num_tiles)
+{
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * TILE;
+    for (int k = 0; k < ITEMS; ++k) {
+        const size_t idx = base + (size_t)k * BLOCK + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
+        const uint64_t m = match_digit(d, valid);
+        if (valid && mbcnt64(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// block-wide exclusive scan of one value per thread (BLOCK threads)
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp /* WAVES + 1 */, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x / 64;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int q = 0; q < WAVES; ++q) {
+        const uint32_t s = s_tmp[q];
+        if (q < (int)w) wbase += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + x - v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
+                                                         const uint32_t* __restrict__ vals_in, int first_pass,
+                                                         size_t n, int shift, const uint32_t* __restrict__ hist,
+                                                         const uint32_t* __restrict__ offsets, uint32_t num_tiles,
+                                                         uint32_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ vals_out)
+{
+    __shared__ uint32_t s_key[TILE];
+    __shared__ uint32_t s_val[TILE];
+    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+    __shared__ uint32_t s_run[RADIX];
+    __shared__ uint32_t s_wcnt[WAVES][RADIX];
+    __shared__ uint32_t s_tmp[WAVES + 1];
+
+    const uint32_t tid = threadIdx.x, w = tid / 64;
+    const size_t base = (size_t)blockIdx.x * TILE;
+    const uint32_t cnt = hist[(size_t)tid * num_tiles + blockIdx.x];
+    s_goff[tid] = offsets[(size_t)tid * num_tiles + blockIdx.x];
+    uint32_t total;
+    s_dstart[tid] = block_exclusive_scan(cnt, s_tmp, total);
+    s_run[tid] = 0;
+    for (int q = 0; q < WAVES; ++q) s_wcnt[q][tid] = 0;
+    __syncthreads();
+
+    for (int k = 0; k < ITEMS; ++k) {
+        const size_t idx = base + (size_t)k * BLOCK + tid;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? keys_in[idx] : 0u;
+        const uint32_t val = valid ? (first_pass ? (uint32_t)idx : vals_in[idx]) : 0u;
+        const uint32_t d = (key >> shift) & 0xFFu;
+        const uint64_t m = match_digit(d, valid);
+        const uint32_t rank = mbcnt64(m);
+        if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_dstart[d] + s_run[d] + rank;
+            for (uint32_t q = 0; q < w; ++q) pos += s_wcnt[q][d];
+            s_key[pos] = key;
+            s_val[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int q = 0; q < WAVES; ++q) {
+            add += s_wcnt[q][tid];
+            s_wcnt[q][tid] = 0;
+        }
+        s_run[tid] += add;
+        __syncthreads();
+    }
+    const size_t tile_n = std::min<size_t>(TILE, n - base);
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t p = (uint32_t)k * BLOCK + tid;
+        if (p < tile_n) {
+            const uint32_t key = s_key[p];
+            const uint32_t d = (key >> shift) & 0xFFu;
+            const size_t dst = (size_t)s_goff[d] + (p - s_dstart[d]);
+            keys_out[dst] = key;
+            vals_out[dst] = s_val[p];
+        }
+    }
+}
+
+// exclusive scan of `count` uint32 values, 2048 per block
+__global__ __launch_bounds__(BLOCK) void k_scan_block(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      size_t count, uint32_t* __restrict__ block_sums)
+{
+    __shared__ uint32_t s_tmp[WAVES + 1];
+    const size_t base = (size_t)blockIdx.x * TILE + (size_t)threadIdx.x * ITEMS;
+    uint32_t v[ITEMS];
+    uint32_t sum = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        v[k] = base + k < count ? in[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, s_tmp, total);
+    for (int k = 0; k < ITEMS; ++k) {
+        if (base + k < count) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t* __restrict__ out, size_t count,
+                                                    const uint32_t* __restrict__ block_offsets)
+{
+    const size_t base = (size_t)blockIdx.x * TILE + (size_t)threadIdx.x * ITEMS;
+    const uint32_t add = block_offsets[blockIdx.x];
+    for (int k = 0; k < ITEMS; ++k)
+        if (base + k < count) out[base + k] += add;
+}
+
+// ---------------------------------------------------------------------------
+// Karras hierarchy (detail/bvh/lbvh.h:120-185, 281-330)
+// ---------------------------------------------------------------------------
+
+__device__ inline int kdelta(const uint32_t* __restrict__ k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint32_t x = k[i] ^ k[j];
+    if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
+    return (int)__clz((int)x);
+}
+
+// parent link of a node: (parent pair << 1) | side
+__global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ codes, int n,
+                                                  pair_node* __restrict__ pairs, uint32_t* __restrict__ leaf_parent)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = kdelta(codes, n, i, i + 1) >= kdelta(codes, n, i, i - 1) ? 1 : -1;
+    const int dmin = kdelta(codes, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(codes, n, i, j);
+    // split search with steps t = ceil(l / 2^k): the reference evaluates
+    // ceil(l / divf) in float (exact for l < 2^24) and keeps probing with
+    // t = 1 afterwards; those extra probes cannot move s (the prefix length
+    // is monotone in the distance from i), so stopping after the first t = 1
+    // gives the same split.
+    int s = 0;
+    for (int k = 1;; ++k) {
+        const int t = (l + (1 << k) - 1) >> k;
+        if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int split = i + s * d + (d < 0 ? d : 0);
+    const int first = d > 0 ? i : j;
+    const int last = d > 0 ? j : i;
+    const int lc = split, rc = split + 1;
+    if (lc < 0 || rc >= n) return;  // unreachable for sorted keys; never write out of bounds
+    pair_node& P = pairs[i];
+    if (lc == first) {
+        P.e[0] = leaf_entry((uint32_t)lc, 1u);
+        leaf_parent[lc] = ((uint32_t)i << 1) | 0u;
+    } else {
+        P.e[0] = (uint32_t)lc;
+        pairs[lc].e[2] = ((uint32_t)i << 1) | 0u;
+    }
+    if (rc == last) {
+        P.e[1] = leaf_entry((uint32_t)rc, 1u);
+        leaf_parent[rc] = ((uint32_t)i << 1) | 1u;
+    } else {
+        P.e[1] = (uint32_t)rc;
+        pairs[rc].e[2] = ((uint32_t)i << 1) | 1u;
+    }
+    if (i == 0) P.e[2] = 0xFFFFFFFFu;
+    P.e[3] = 0;
+}
+
+__device__ inline void store_box(float* f, f3 lo, f3 hi)
+{
+    f[0] = lo.x; f[1] = lo.y; f[2] = lo.z;
+    f[3] = hi.x; f[4] = hi.y; f[5] = hi.z;
+}
+
+__device__ inline void store_ref_node(vr_bvh_node* o, f3 lo, f3 hi, uint32_t first, uint32_t num)
+{
+    // bvh_node::set_inner / set_leaf (bvh.h:122-138); axis = sign = 0
+    uint4* q = reinterpret_cast<uint4*>(o);
+    q[0] = make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), __float_as_uint(hi.x));
+    q[1] = make_uint4(__float_as_uint(hi.y), __float_as_uint(hi.z), first, num);
+}
+
+// Leaves in sorted order: copy the triangle into traversal order, store its
+// box in the parent's pair record and climb.  The first thread to arrive at a
+// node stops; the second sees both child boxes and continues with their
+// union, so every node box is written exactly once.
+__global__ __launch_bounds__(BLOCK) void k_bottom_up(const vr_triangle* __restrict__ prims, size_t n,
+                                                     const int32_t* __restrict__ sorted_ids,
+                                                     const uint32_t* __restrict__ leaf_parent,
+                                                     pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
+                                                     vr_bvh_node* __restrict__ ref_nodes,
+                                                     float* __restrict__ root_box, uint32_t* __restrict__ depth_out)
+{
+    const size_t j = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const int32_t id = sorted_ids[j];
+    const tri_in t = load_tri(prims, (size_t)id);
+    tri_node tn;
+    tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
+    tn.b = make_float4(t.e1.x, t.e1.y, t.e1.z, __uint_as_float(t.geom_id));
+    tn.c = make_float4(t.e2.x, t.e2.y, t.e2.z, __uint_as_float((uint32_t)id));
+    tris[j] = tn;
+    f3 lo, hi;
+    tri_box(t, lo, hi);
+    if (n == 1) {
+        store_box(root_box, lo, hi);
+        if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, (uint32_t)id, 1u);
+        *depth_out = 1;
+        return;
+    }
+    uint32_t link = leaf_parent[j];
+    const uint32_t npairs = (uint32_t)(n - 1);
+    if ((link >> 1) >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
+    uint32_t p = link >> 1, side = link & 1u;
+    store_box(&pairs[p].f[side * 6], lo, hi);
+    heights[2 * p + side] = 1;
+    if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, (uint32_t)id, 1u);
+    // a Karras tree has at most 65 levels; the bound only guards against a
+    // corrupt hierarchy
+    for (int level = 0; level < 128; ++level) {
+        __threadfence();
+        const uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;
+        __threadfence();
+        const volatile float* f = pairs[p].f;
+        const f3 lo0 = mk3(f[0], f[1], f[2]), hi0 = mk3(f[3], f[4], f[5]);
+        const f3 lo1 = mk3(f[6], f[7], f[8]), hi1 = mk3(f[9], f[10], f[11]);
+        lo = min3v(lo0, lo1);
+        hi = max3v(hi0, hi1);
+        const volatile uint32_t* hv = heights;
+        const uint32_t h = max(hv[2 * p], hv[2 * p + 1]) + 1u;
+        if (p == 0) {
+            store_box(root_box, lo, hi);
+            if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, 1u, 0u);
+            *depth_out = h;
+            return;
+        }
+        link = pairs[p].e[2];
+        const uint32_t pp = link >> 1;
+        if (pp >= npairs) return;
+        side = link & 1u;
+        store_box(&pairs[pp].f[side * 6], lo, hi);
+        heights[2 * pp + side] = h;
+        if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * pp + side], lo, hi, 1u + 2u * p, 0u);
+        p = pp;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_iota_u32(uint32_t* p, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// scratch layout for one build of n primitives
+struct build_scratch {
+    uint32_t* keys0;
+    uint32_t* vals0;
+    uint32_t* keys1;
+    uint32_t* vals1;
+    uint32_t* hist;
+    uint32_t* offs;
+    uint32_t* sums0;  // level-1 block sums
+    uint32_t* sums1;  // level-2 block sums
+    uint32_t* sums2;
+    uint32_t* cb;     // 6 words
+    uint32_t* leaf_parent;
+    size_t bytes;
+};
+
+build_scratch plan_scratch(size_t n, char* base)
+{
+    const size_t tiles = std::max<size_t>(1, (n + TILE - 1) / TILE);
+    const size_t hist_n = tiles * RADIX;
+    const size_t l1 = (hist_n + TILE - 1) / TILE;
+    const size_t l2 = (l1 + TILE - 1) / TILE;
+    size_t off = 0;
+    auto take = [&](size_t elems) {
+        char* p = base ? base + off : nullptr;
+        off += align_up(std::max<size_t>(elems, 1) * 4, 256);
+        return reinterpret_cast<uint32_t*>(p);
+    };
+    build_scratch s;
+    s.keys0 = take(n);
+    s.vals0 = take(n);
+    s.keys1 = take(n);
+    s.vals1 = take(n);
+    s.hist = take(hist_n);
+    s.offs = take(hist_n);
+    s.sums0 = take(l1);
+    s.sums1 = take(l2);
+    s.sums2 = take(1);
+    s.cb = take(8);
+    s.leaf_parent = take(n);
+    s.bytes = off;
+    return s;
+}
+
+// exclusive scan of `count` values in `in` into `out` (may alias); up to
+// 2048^3 values through three levels
+int scan_exclusive(const uint32_t* in, uint32_t* out, size_t count, uint32_t* sums0, uint32_t* sums1,
+                   uint32_t* sums2, hipStream_t st)
+{
+    const size_t b0 = (count + TILE - 1) / TILE;
+    VR_CHECK(b0 <= (size_t)TILE * TILE, VR_ERR_UNSUPPORTED, "scan: too many elements");
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)b0), dim3(BLOCK), 0, st, in, out, count, sums0);
+    if (b0 > 1) {
+        const size_t b1 = (b0 + TILE - 1) / TILE;
+        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)b1), dim3(BLOCK), 0, st, sums0, sums0, b0, sums1);
+        if (b1 > 1) {
+            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(BLOCK), 0, st, sums1, sums1, b1, sums2);
+            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)b1), dim3(BLOCK), 0, st, sums0, b0, sums1);
+        }
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)b0), dim3(BLOCK), 0, st, out, count, sums0);
+    }
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+int launch_morton(const vr_triangle* prims, size_t n, uint32_t* cb, uint32_t* codes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, cb);
+    const unsigned grid = std::max(1u, std::min(div_up(n, BLOCK), 4096u));
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(grid), dim3(BLOCK), 0, st, prims, n, cb);
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, prims, n, cb, codes);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+}  // namespace
+
+// Build (or rebuild in place) the LBVH of `bs` from n primitives.
+int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, hipStream_t st)
+{
+    VR_CHECK(n <= (size_t)MAX_LEAF_FIRST, VR_ERR_UNSUPPORTED, "lbvh: more than 2^26 - 1 primitives");
+    const size_t npairs = n > 1 ? n - 1 : 0;
+    // (re)allocate owned arrays when the size changes
+    if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
+        hipFree(bs.pairs); hipFree(bs.tris); hipFree(bs.prims); hipFree(bs.sorted_codes);
+        hipFree(bs.sorted_ids); hipFree(bs.flags); hipFree(bs.heights); hipFree(bs.ref_nodes);
+        hipFree(bs.indices); hipFree(bs.root_box); hipFree(bs.scratch);
+        bs = bvh_state{};
+        bs.num_prims = n;
+        bs.num_pairs = npairs;
+        bs.is_lbvh = true;
+        if (n == 0) return VR_OK;
+        VR_HIP_CHECK(hipMalloc(&bs.pairs, std::max<size_t>(npairs, 1) * sizeof(pair_node)));
+        VR_HIP_CHECK(hipMalloc(&bs.tris, n * sizeof(tri_node)));
+        VR_HIP_CHECK(hipMalloc(&bs.prims, n * sizeof(vr_triangle)));
+        VR_HIP_CHECK(hipMalloc(&bs.sorted_codes, n * 4));
+        VR_HIP_CHECK(hipMalloc(&bs.sorted_ids, n * 4));
+        VR_HIP_CHECK(hipMalloc(&bs.flags, std::max<size_t>(npairs, 1) * 4));
+        VR_HIP_CHECK(hipMalloc(&bs.heights, std::max<size_t>(2 * npairs, 1) * 4));
+        VR_HIP_CHECK(hipMalloc(&bs.root_box, 8 * sizeof(float)));
+        VR_HIP_CHECK(hipMalloc(&bs.indices, n * 4));
+        if (keep_nodes) {
+            bs.num_ref_nodes = 2 * n - 1;
+            VR_HIP_CHECK(hipMalloc(&bs.ref_nodes, bs.num_ref_nodes * sizeof(vr_bvh_node)));
+        }
+        build_scratch plan = plan_scratch(n, nullptr);
+        bs.scratch_bytes = plan.bytes;
+        VR_HIP_CHECK(hipMalloc(&bs.scratch, plan.bytes));
+    }
+    if (n == 0) return VR_OK;
+    build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
+
+    // copy of the primitives (index_bvh_t(P*, n), bvh.h:891-896)
+    VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
+
+    // 1-2: centroid bounds and Morton codes
+    int rc = launch_morton(d_prims, n, s.cb, s.keys0, st);
+    if (rc) return rc;
+
+    // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
+    const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
+    const size_t hist_n = (size_t)tiles * RADIX;
+    uint32_t* kin = s.keys0;
+    uint32_t* vin = s.vals0;
+    uint32_t* kout = s.keys1;
+    uint32_t* vout = s.vals1;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = pass * 8;
+        if (pass == 3) {  // last pass writes straight into the BVH's arrays
+            kout = bs.sorted_codes;
+            vout = reinterpret_cast<uint32_t*>(bs.sorted_ids);
+        }
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(BLOCK), 0, st, kin, n, shift, s.hist, tiles);
+        rc = scan_exclusive(s.hist, s.offs, hist_n, s.sums0, s.sums1, s.sums2, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(BLOCK), 0, st, kin, vin, pass == 0 ? 1 : 0, n, shift,
+                           s.hist, s.offs, tiles, kout, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    VR_HIP_CHECK(hipGetLastError());
+
+    // 4: hierarchy
+    if (n > 1) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, s.leaf_parent, n, 0xFFFFFFFFu);
+        hipLaunchKernelGGL(k_karras, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
+                           bs.pairs, s.leaf_parent);
+        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags, npairs, 0u);
+    }
+    // 5: boxes, traversal-order triangles, reference layout
+    hipLaunchKernelGGL(k_bottom_up, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.prims, n,
+                       bs.sorted_ids, s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris,
+                       keep_nodes ? bs.ref_nodes : nullptr, bs.root_box,
+                       reinterpret_cast<uint32_t*>(bs.root_box + 6));
+    hipLaunchKernelGGL(k_iota_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.indices, n);
+    VR_HIP_CHECK(hipGetLastError());
+
+    bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
+    bs.max_leaf = 1;
+    return VR_OK;
+}
+
+// Refit after the vertices moved: same ids, new boxes (detail/bvh/refit.h).
+int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
+{
+    const size_t n = bs.num_prims;
+    if (n == 0) return VR_OK;
+    build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
+    VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
+    if (n > 1) {
+        // leaf parents are recomputed from the unchanged hierarchy
+        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, s.leaf_parent, n, 0xFFFFFFFFu);
+        hipLaunchKernelGGL(k_karras, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
+                           bs.pairs, s.leaf_parent);
+        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags,
+                           bs.num_pairs, 0u);
+    }
+    hipLaunchKernelGGL(k_bottom_up, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.prims, n, bs.sorted_ids,
+                       s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, bs.ref_nodes, bs.root_box,
+                       reinterpret_cast<uint32_t*>(bs.root_box + 6));
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st)
+{
+    if (n == 0) return VR_OK;
+    uint32_t* cb = nullptr;
+    VR_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&cb), 64, st));
+    int rc = launch_morton(d_prims, n, cb, d_codes, st);
+    hipFreeAsync(cb, st);
+    return rc;
+}
+
+}  // namespace vr

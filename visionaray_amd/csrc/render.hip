@@ -1,0 +1,443 @@
+// render.hip -- the path-tracing bounce loop and the AO kernel under the
+// jittered-blend pixel sampler.
+//
+// pathtracing::kernel (detail/pathtracing.inl:43-233) runs on a persistent
+// wavefront: each lane owns one pixel at a time and runs its spp samples in
+// sequence (the reference draws all samples of a pixel from one generator,
+// detail/sched_common.h:427-526, so a pixel cannot be split across lanes
+// without changing its random stream).  Every loop iteration traces one ray
+// per lane; a lane whose path ended starts the next sample, and a lane whose
+// pixel is done takes the next pixel, with one atomic per wave.  No lane idles
+// while the wave's longest path finishes.
+#include "vr_internal.h"
+
+namespace vr {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+// make_random_seed (make_random_seed.h:23-37)
+__device__ inline uint32_t make_random_seed(int32_t val0, int32_t val1)
+{
+    uint32_t v0 = (uint32_t)val0, v1 = (uint32_t)val1, s0 = 0;
+#pragma unroll
+    for (uint32_t n = 0; n < 4; ++n) {
+        s0 += 0x9E3779B9u;
+        v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xC8013EA4u);
+        v1 += ((v0 << 4) + 0xAd90777Du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7E95761Eu);
+    }
+    return v0;
+}
+
+// random_generator<float> as built for the host (random_generator.h:29-62):
+// minstd_rand0 (x <- 16807 x mod 2^31 - 1; seed mod 2^31 - 1, zero -> 1) with
+// uniform_real_distribution<float>(0, 1) through generate_canonical<float, 24>,
+// one engine draw per sample: float(x - 1) / 2^31, clamped below 1.
+struct rng_t {
+    uint32_t x;
+};
+__device__ inline void rng_seed(rng_t& g, uint32_t s)
+{
+    const uint32_t x = s % 2147483647u;
+    g.x = x == 0 ? 1u : x;
+}
+__device__ inline float rng_next(rng_t& g)
+{
+    const uint64_t t = (uint64_t)g.x * 16807u;
+    uint32_t r = (uint32_t)(t & 0x7FFFFFFFu) + (uint32_t)(t >> 31);  // 2^31 == 1 mod 2^31 - 1
+    if (r >= 0x7FFFFFFFu) r -= 0x7FFFFFFFu;
+    g.x = r;
+    const float f = (float)(r - 1u) * 4.656612873077392578125e-10f;  // exact 2^-31 scaling
+    return f >= 1.0f ? 0x1.fffffep-1f : f;
+}
+// two draws for one argument list, in the order the host compiler evaluated
+// the reference's `f(gen.next(), gen.next())`: arg_order 1 = right first
+__device__ inline void rng_pair(rng_t& g, uint32_t arg_order, float& first_arg, float& second_arg)
+{
+    const float a = rng_next(g);
+    const float b = rng_next(g);
+    first_arg = arg_order ? b : a;
+    second_arg = arg_order ? a : b;
+}
+
+// make_primary_ray with jitter (detail/sched_common.h:209-233) and
+// pinhole_camera::primary_ray (detail/pinhole_camera.inl:103-121), image region
+// (0,0)-(1,1)
+__device__ inline void primary_ray(const vr_camera& cam, rng_t& g, uint32_t arg_order, int x, int y, int w, int h,
+                                   f3& o, f3& d)
+{
+    float a, b;
+    rng_pair(g, arg_order, a, b);
+    const float jx = a - 0.5f, jy = b - 0.5f;
+    const float fx = (float)x + jx, fy = (float)y + jy;
+    float sx = (fx + 0.5f) / (float)w;
+    float sy = (fy + 0.5f) / (float)h;
+    sx = (1.0f - sx) * 0.0f + sx * 1.0f;
+    sy = (1.0f - sy) * 0.0f + sy * 1.0f;
+    const float u = 2.0f * sx - 1.0f;
+    const float v = 2.0f * sy - 1.0f;
+    const f3 U = mk3(cam.U[0], cam.U[1], cam.U[2]);
+    const f3 V = mk3(cam.V[0], cam.V[1], cam.V[2]);
+    const f3 W = mk3(cam.W[0], cam.W[1], cam.W[2]);
+    d = normalize3(add3(add3(muls3(U, u), muls3(V, v)), W));
+    o = mk3(cam.eye[0], cam.eye[1], cam.eye[2]);
+}
+
+// cosine_sample_hemisphere (sampling.h:117-127)
+__device__ inline f3 cosine_hemisphere(float u1, float u2)
+{
+    const float r = sqrtf(u1);
+    const float theta = 6.28318530717958623199592693709e+00f * u2;
+    float sn, cs;
+    sincosf(theta, &sn, &cs);
+    return mk3(r * cs, r * sn, sqrtf(1.0f - u1));
+}
+
+// make_orthonormal_basis (math/detail/vector3.inl:356-366; brdf.h:62-66)
+__device__ inline void onb(f3 w, f3& u, f3& v)
+{
+    v = fabsf(w.x) > fabsf(w.y) ? normalize3(mk3(-w.z, 0.0f, w.x)) : normalize3(mk3(0.0f, w.z, -w.y));
+    u = cross3(v, w);
+}
+
+__device__ inline f3 ldtri3(float4 q) { return mk3(q.x, q.y, q.z); }
+
+__device__ inline uint64_t lane_mask_lt_count(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+struct pt_args {
+    bvh_view b;
+    const vr_material* mats;
+    uint32_t num_mats;
+    vr_kernel_params kp;
+    vr_camera cam;
+    vr_sched_params sp;
+    float* accum;
+    float* color;
+    unsigned long long* ray_counts;
+    uint32_t* work;
+    uint32_t arg_order;
+};
+
+__device__ inline void write_pixel(const pt_args& A, uint32_t px, float4 c)
+{
+    // sample average, pixel_access::blend into accum, accum -> color
+    // (detail/sched_common.h:466-509, detail/pixel_access.h:1040-1131)
+    const float spp = (float)A.sp.spp;
+    c.x = c.x / spp; c.y = c.y / spp; c.z = c.z / spp; c.w = c.w / spp;
+    float4* acc = reinterpret_cast<float4*>(A.accum) + px;
+    const float4 old = *acc;
+    const float sf = A.sp.sfactor, df = A.sp.dfactor;
+    const float4 r = make_float4(c.x * sf + old.x * df, c.y * sf + old.y * df, c.z * sf + old.z * df,
+                                 c.w * sf + old.w * df);
+    *acc = r;
+    reinterpret_cast<float4*>(A.color)[px] = r;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
+{
+    __shared__ uint32_t stack[LDS_STACK * BLOCK];
+    uint32_t* st = stack + threadIdx.x;
+    const uint32_t w = A.sp.width, h = A.sp.height, npix = w * h;
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const f3 amb = mk3(A.kp.ambient[0], A.kp.ambient[1], A.kp.ambient[2]);
+    const float eps = A.kp.epsilon;
+
+    int32_t px = -1;
+    bool done = false;
+    rng_t g{1u};
+    uint32_t s = 0, bounce = 0;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
+    f3 tp = mk3(1, 1, 1), I = mk3(0, 0, 0);
+    bool rhit = false;
+    float4 sum = make_float4(0, 0, 0, 0);
+    unsigned long long n_primary = 0, n_secondary = 0;
+    trace_counters tc{0, 0};
+    unsigned long long n_pairs = 0, n_tris = 0;
+
+    for (;;) {
+        // regenerate: lanes without a pixel take the next ones, one atomic per wave
+        const bool need = !done && px < 0;
+        const uint64_t needm = __ballot(need);
+        if (needm) {
+            const int leader = __ffsll((unsigned long long)needm) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(needm));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint32_t my = base + (uint32_t)lane_mask_lt_count(needm);
+                if (my < npix) {
+                    px = (int32_t)my;
+                    const int x = (int)(my % w), y = (int)(my / w);
+                    rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)A.sp.frame_id));
+                    s = 0;
+                    sum = make_float4(0, 0, 0, 0);
+                    primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                    tp = mk3(1, 1, 1);
+                    I = mk3(0, 0, 0);
+                    bounce = 0;
+                    rhit = false;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (done) continue;
+
+        const hit_t hr = traverse<false, COUNT, LDS_STACK, BLOCK>(A.b, o, d, 0.0f, FLT_MAX, st, &tc);
+        if (COUNT) {
+            n_pairs += tc.pairs;
+            n_tris += tc.tris;
+            tc.pairs = 0;
+            tc.tris = 0;
+        }
+        if (bounce == 0) ++n_primary;
+        else ++n_secondary;
+
+        bool finished = false;
+        if (hr.tri < 0) {
+            // escaped: ambient light times throughput (pathtracing.inl:69-77)
+            I = add3(I, mul3(amb, tp));
+            finished = true;
+        } else {
+            if (bounce == 0) rhit = true;
+            const f3 view = neg3(d);
+            const f3 pos = add3(o, muls3(d, hr.t));
+            const float4* tq = reinterpret_cast<const float4*>(A.b.tris + hr.tri);
+            const float4 tb = tq[1], tc = tq[2];
+            const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tc)));
+            uint32_t mid = __float_as_uint(tb.w);
+            mid = mid < A.num_mats ? mid : A.num_mats - 1u;
+            const vr_material* m = A.mats + mid;
+            f3 refl = mk3(0.0f, 0.0f, 0.0f);
+            float pdf;
+            f3 src;
+            const bool emission = m->type == 1;
+            if (emission) {
+                // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
+                const float ls = m->ls;
+                src = mk3((1.0f * m->ce[0]) * ls, (1.0f * m->ce[1]) * ls, (1.0f * m->ce[2]) * ls);
+                pdf = 1.0f;
+            } else {
+                // matte::sample -> lambertian::sample_f (matte.inl:43-56, brdf.h:50-84)
+                const f3 n = faceforward3(gn, view, gn);
+                f3 u, v;
+                onb(n, u, v);
+                float a1, a2;
+                rng_pair(g, A.arg_order, a1, a2);
+                const f3 sp = cosine_hemisphere(a1, a2);
+                refl = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
+                pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
+                const float kd = m->kd;
+                src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
+                          1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
+                          1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
+            }
+            const bool zero_pdf = pdf <= 0.0f;
+            bool active = true;
+            if (emission) {
+                // no light list: MIS weight 1 (pathtracing.inl:134-146)
+                I = add3(I, mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src));
+                active = false;
+            }
+            if (zero_pdf) active = false;
+            const f3 n = faceforward3(gn, view, gn);
+            const float q = dot3(n, refl) / pdf;
+            tp = mk3(tp.x * (src.x * q), tp.y * (src.y * q), tp.z * (src.z * q));
+            if (zero_pdf) tp = mk3(0.0f, 0.0f, 0.0f);
+            if (bounce >= 2) {
+                // Russian roulette (pathtracing.inl:201-213)
+                float prob = tp.x;
+                if (tp.y > prob) prob = tp.y;
+                if (tp.z > prob) prob = tp.z;
+                const bool terminate = rng_next(g) > prob;
+                if (terminate) active = false;
+                tp = mk3(tp.x / prob, tp.y / prob, tp.z / prob);
+            }
+            if (!active) {
+                finished = true;
+            } else {
+                o = add3(pos, muls3(refl, eps));
+                d = refl;
+                ++bounce;
+                if (bounce >= A.kp.num_bounces) finished = true;
+            }
+        }
+        if (finished) {
+            if (rhit) {
+                sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
+            } else {
+                sum.x += A.kp.bg[0]; sum.y += A.kp.bg[1]; sum.z += A.kp.bg[2]; sum.w += 1.0f;
+            }
+            ++s;
+            if (s >= A.sp.spp) {
+                write_pixel(A, (uint32_t)px, sum);
+                px = -1;
+            } else {
+                const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
+                primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                tp = mk3(1, 1, 1);
+                I = mk3(0, 0, 0);
+                bounce = 0;
+                rhit = false;
+            }
+        }
+    }
+    if (A.ray_counts) {
+        // wave reduction, one atomic pair per wave
+        for (int off = 32; off >= 1; off >>= 1) {
+            n_primary += __shfl_xor(n_primary, off);
+            n_secondary += __shfl_xor(n_secondary, off);
+            if (COUNT) {
+                n_pairs += __shfl_xor(n_pairs, off);
+                n_tris += __shfl_xor(n_tris, off);
+            }
+        }
+        if (lane == 0) {
+            atomicAdd(&A.ray_counts[0], n_primary);
+            atomicAdd(&A.ray_counts[1], n_secondary);
+            if (COUNT) {
+                atomicAdd(&A.ray_counts[2], n_pairs);
+                atomicAdd(&A.ray_counts[3], n_tris);
+            }
+        }
+    }
+}
+
+struct ao_args {
+    bvh_view b;
+    vr_camera cam;
+    vr_sched_params sp;
+    uint32_t samples;
+    float radius;
+    float bg[3];
+    float* accum;
+    float* color;
+    uint32_t arg_order;
+};
+
+// AO kernel of src/examples/ao/main.cpp:192-260: primary closest hit, then
+// `samples` cosine-distributed any-hit rays of length `radius` from
+// P + dir * 1e-3; color = 1 - hits / samples.
+__global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
+{
+    __shared__ uint32_t stack[LDS_STACK * BLOCK];
+    uint32_t* st = stack + threadIdx.x;
+    const uint32_t w = A.sp.width, h = A.sp.height;
+    const uint32_t px = blockIdx.x * BLOCK + threadIdx.x;
+    if (px >= w * h) return;
+    const int x = (int)(px % w), y = (int)(px / w);
+    rng_t g;
+    rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)A.sp.frame_id));
+    float4 sum = make_float4(0, 0, 0, 0);
+    const float inc = 1.0f / (float)A.samples;
+    for (uint32_t s = 0; s < A.sp.spp; ++s) {
+        f3 o, d;
+        primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+        float4 c = make_float4(A.bg[0], A.bg[1], A.bg[2], 1.0f);
+        const hit_t hr = traverse<false, false, LDS_STACK, BLOCK>(A.b, o, d, 0.0f, FLT_MAX, st, nullptr);
+        if (hr.tri >= 0) {
+            const f3 pos = add3(o, muls3(d, hr.t));
+            const float4* tq = reinterpret_cast<const float4*>(A.b.tris + hr.tri);
+            f3 n = normalize3(cross3(ldtri3(tq[1]), ldtri3(tq[2])));
+            n = faceforward3(n, neg3(d), n);
+            f3 u, v;
+            onb(n, u, v);
+            float ao = 0.0f;
+            for (uint32_t i = 0; i < A.samples; ++i) {
+                float a1, a2;
+                rng_pair(g, A.arg_order, a1, a2);
+                const f3 sp = cosine_hemisphere(a1, a2);
+                const f3 dir = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
+                const f3 ao_o = add3(pos, muls3(dir, 1e-3f));
+                const hit_t ah = traverse<true, false, LDS_STACK, BLOCK>(A.b, ao_o, dir, 0.0f, A.radius, st, nullptr);
+                if (ah.tri >= 0) ao = ao + inc;
+            }
+            ao = 1.0f - ao;
+            c = make_float4(ao, ao, ao, 1.0f);
+        }
+        sum.x += c.x; sum.y += c.y; sum.z += c.z; sum.w += c.w;
+    }
+    const float spp = (float)A.sp.spp;
+    sum.x = sum.x / spp; sum.y = sum.y / spp; sum.z = sum.z / spp; sum.w = sum.w / spp;
+    float4* acc = reinterpret_cast<float4*>(A.accum) + px;
+    const float4 old = *acc;
+    const float sf = A.sp.sfactor, df = A.sp.dfactor;
+    const float4 r = make_float4(sum.x * sf + old.x * df, sum.y * sf + old.y * df, sum.z * sf + old.z * df,
+                                 sum.w * sf + old.w * df);
+    *acc = r;
+    reinterpret_cast<float4*>(A.color)[px] = r;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+template <bool COUNT>
+int launch_pt(const pt_args& A, hipStream_t st)
+{
+    static int grid_cache = 0;
+    if (grid_cache == 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        VR_HIP_CHECK(hipGetDevice(&dev));
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<COUNT>, BLOCK, 0));
+        VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        grid_cache = std::max(1, per_cu) * std::max(1, cus);
+    }
+    hipLaunchKernelGGL(k_pathtrace<COUNT>, dim3(grid_cache), dim3(BLOCK), 0, st, A);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+}  // namespace
+
+// argument evaluation order of the reference host build (see pathtrace docs)
+uint32_t g_arg_order = 1;
+
+int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+              const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
+              unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st)
+{
+    if ((size_t)sp.width * sp.height == 0) return VR_OK;
+    pt_args A;
+    A.b = bs.view();
+    A.mats = mats;
+    A.num_mats = (uint32_t)num_mats;
+    A.kp = kp;
+    A.cam = cam;
+    A.sp = sp;
+    A.accum = accum;
+    A.color = color;
+    A.ray_counts = ray_counts;
+    A.work = work;
+    A.arg_order = g_arg_order;
+    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, work);
+    return counted ? launch_pt<true>(A, st) : launch_pt<false>(A, st);
+}
+
+int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& sp, uint32_t samples, float radius,
+              const float bg[3], float* accum, float* color, hipStream_t st)
+{
+    const size_t npix = (size_t)sp.width * sp.height;
+    if (npix == 0) return VR_OK;
+    ao_args A;
+    A.b = bs.view();
+    A.cam = cam;
+    A.sp = sp;
+    A.samples = samples;
+    A.radius = radius;
+    A.bg[0] = bg[0]; A.bg[1] = bg[1]; A.bg[2] = bg[2];
+    A.accum = accum;
+    A.color = color;
+    A.arg_order = g_arg_order;
+    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_ao: BVH deeper than the traversal stack");
+    hipLaunchKernelGGL(k_ao, dim3(div_up(npix, BLOCK)), dim3(BLOCK), 0, st, A);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+}  // namespace vr

@@ -1,0 +1,317 @@
+// vr_internal.h -- device data layouts, exact-arithmetic helpers and the BVH
+// traversal shared by every kernel of the library.
+//
+// All HIP code of the library is compiled with -ffp-contract=off: each float
+// operation below rounds exactly as written, in the order the reference writes
+// it, so results are bit-identical to the reference's CPU arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+
+#include "../../include/visionaray_amd.h"
+
+namespace vr {
+
+// ---------------------------------------------------------------------------
+// error handling: every C entry point returns a status and records a message
+// ---------------------------------------------------------------------------
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define VR_HIP_CHECK(expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::vr::fail(VR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define VR_CHECK(cond, code, msg)                                                          \
+    do {                                                                                   \
+        if (!(cond)) return ::vr::fail((code), (msg));                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device layouts
+// ---------------------------------------------------------------------------
+
+// Traversal node: the two children of one inner node in one 64-byte record,
+// so an inner visit is a single cache-line fetch of four 16-byte loads.
+//   f[0..5]   child 0 box  lo.x lo.y lo.z hi.x hi.y hi.z
+//   f[6..11]  child 1 box
+//   e[0], e[1] child entries (see below); e[2] parent pair (-1 at the root
+//   pair), e[3] spare.
+// Child entry: bit 31 clear -> inner node, the index of its pair record;
+// bit 31 set -> leaf covering triangles [first, first + count) with
+// first = entry & 0x03FFFFFF and count = ((entry >> 26) & 31) + 1.
+struct alignas(64) pair_node {
+    float f[12];
+    uint32_t e[4];
+};
+static_assert(sizeof(pair_node) == 64, "pair_node is one 64-byte line");
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t LEAF_FIRST_MASK = 0x03FFFFFFu;
+constexpr uint32_t MAX_LEAF_FIRST = LEAF_FIRST_MASK;
+constexpr uint32_t MAX_LEAF_COUNT = 32;
+
+__host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count)
+{
+    return LEAF_BIT | ((count - 1u) << 26) | first;
+}
+
+// Triangle in traversal order, 48 bytes = three 16-byte loads:
+//   a = v1.xyz, prim_id    b = e1.xyz, geom_id    c = e2.xyz, original index
+struct alignas(16) tri_node {
+    float4 a;
+    float4 b;
+    float4 c;
+};
+static_assert(sizeof(tri_node) == 48, "tri_node layout");
+
+// Device view of a BVH, passed by value into kernels (the analogue of
+// index_bvh_ref_t, bvh.h:495-569).
+struct bvh_view {
+    const pair_node* pairs;
+    const tri_node* tris;
+    uint32_t num_prims;
+    uint32_t root;  // entry of the root: 0 (pair 0) or a leaf entry; unused when num_prims == 0
+    uint32_t num_pairs;
+};
+
+// Host-side state behind a vr_bvh handle.
+struct bvh_state {
+    int device = 0;
+    size_t num_prims = 0;
+    size_t num_pairs = 0;
+    uint32_t root = 0;
+    uint32_t depth = 0;
+    uint32_t max_leaf = 1;
+    bool is_lbvh = false;
+    pair_node* pairs = nullptr;      // num_pairs
+    tri_node* tris = nullptr;        // num_prims, traversal order
+    vr_triangle* prims = nullptr;    // num_prims, caller order (tree.primitives())
+    uint32_t* sorted_codes = nullptr;
+    int32_t* sorted_ids = nullptr;   // traversal position -> original index
+    int32_t* parents = nullptr;      // LBVH: parent pair of every node (2n-1)
+    uint32_t* flags = nullptr;       // bottom-up arrival counters (num_pairs)
+    uint32_t* heights = nullptr;     // bottom-up subtree heights (num_pairs)
+    vr_bvh_node* ref_nodes = nullptr;   // optional reference layout
+    uint32_t* indices = nullptr;        // tree.indices()
+    size_t num_ref_nodes = 0;
+    float* root_box = nullptr;          // 6 floats
+    void* scratch = nullptr;            // build scratch
+    size_t scratch_bytes = 0;
+
+    bvh_view view() const { return bvh_view{pairs, tris, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
+};
+
+// ---------------------------------------------------------------------------
+// exact scalar helpers (reference semantics)
+// ---------------------------------------------------------------------------
+
+// math/detail/math.h:116-128 ternary min/max
+__host__ __device__ inline float tmin_f(float x, float y) { return x < y ? x : y; }
+__host__ __device__ inline float tmax_f(float x, float y) { return x < y ? y : x; }
+
+struct f3 {
+    float x, y, z;
+};
+__host__ __device__ inline f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ inline f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline f3 mul3(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline f3 muls3(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline f3 smul3(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline f3 neg3(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__host__ __device__ inline f3 min3v(f3 a, f3 b) { return f3{tmin_f(a.x, b.x), tmin_f(a.y, b.y), tmin_f(a.z, b.z)}; }
+__host__ __device__ inline f3 max3v(f3 a, f3 b) { return f3{tmax_f(a.x, b.x), tmax_f(a.y, b.y), tmax_f(a.z, b.z)}; }
+// math/detail/vector3.inl:292-307
+__host__ __device__ inline f3 cross3(f3 u, f3 v)
+{
+    return f3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+__host__ __device__ inline float dot3(f3 u, f3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+// normalize = v * (1 / sqrt(dot(v, v))), both correctly rounded
+// (math/detail/vector3.inl:332-335, math/detail/math.h:596-599)
+__host__ __device__ inline f3 normalize3(f3 v) { return muls3(v, 1.0f / sqrtf(dot3(v, v))); }
+// math/detail/vector3.inl:432-447
+__host__ __device__ inline float min_el3(f3 u)
+{
+    float r = u.x;
+    if (u.y < r) r = u.y;
+    if (u.z < r) r = u.z;
+    return r;
+}
+__host__ __device__ inline float max_el3(f3 u)
+{
+    float r = u.x;
+    if (u.y > r) r = u.y;
+    if (u.z > r) r = u.z;
+    return r;
+}
+// math/detail/vector.inl:679-686
+__host__ __device__ inline f3 faceforward3(f3 n, f3 i, f3 nref) { return dot3(nref, i) < 0.0f ? neg3(n) : n; }
+
+// ---------------------------------------------------------------------------
+// ray / triangle: Moller-Trumbore, two-sided, inclusive edges
+// (math/intersect.h:176-229); returns hit and writes t, b1, b2
+// ---------------------------------------------------------------------------
+__device__ inline bool intersect_tri(f3 ori, f3 dir, f3 v1, f3 e1, f3 e2, float& t, float& b1, float& b2)
+{
+    f3 s1 = cross3(dir, e2);
+    float div = dot3(s1, e1);
+    if (!(div != 0.0f)) return false;
+    float inv_div = 1.0f / div;
+    f3 d = sub3(ori, v1);
+    b1 = dot3(d, s1) * inv_div;
+    if (!(b1 >= 0.0f && b1 <= 1.0f)) return false;
+    f3 s2 = cross3(d, e1);
+    b2 = dot3(dir, s2) * inv_div;
+    if (!(b2 >= 0.0f && b1 + b2 <= 1.0f)) return false;
+    t = dot3(e2, s2) * inv_div;
+    return true;
+}
+
+struct hit_t {
+    float t;
+    float u;
+    float v;
+    int32_t tri;  // traversal position of the hit triangle, -1 = miss
+};
+
+// ---------------------------------------------------------------------------
+// BVH traversal (detail/bvh/intersect.inl:166-356, device order).
+//
+// The root box is not tested.  At an inner node both child boxes are slab
+// tested with tnear = max(tmin, max_el(min(tlo, thi))) and tfar =
+// min(tmax, min_el(max(tlo, thi))); a child is entered when tfar >= tnear and
+// tnear < t_closest, and with both entered the nearer one (tnear0 < tnear1 ?
+// 0 : 1) goes first while the other is deferred.  Primitives are accepted
+// with is_closer (update_if.h:68-107): hit && t < t_closest && tmin <= t <=
+// tmax, strictly closer, so the first hit found wins a tie.  The reference
+// keeps deferred nodes in a 5-entry register stack plus a restart trail; the
+// full stack here visits the same nodes in the same order.
+//
+// The stack keeps its top LDS_DEPTH entries in LDS, one column per lane
+// (stack[k * STRIDE], so the 64 lanes of a wave touch 64 consecutive words);
+// deeper entries go to a per-lane private array that only rays on unusually
+// deep paths touch.  MAX_STACK entries must cover the tree depth (checked on
+// the host before launch).
+// ---------------------------------------------------------------------------
+
+struct trace_counters {
+    uint32_t pairs;
+    uint32_t tris;
+};
+
+constexpr int MAX_STACK = 128;
+
+template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
+__device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
+                                          uint32_t* __restrict__ stack, trace_counters* cnt)
+{
+    hit_t r;
+    r.t = FLT_MAX;
+    r.u = 0.0f;
+    r.v = 0.0f;
+    r.tri = -1;
+    if (b.num_prims == 0) return r;
+
+    const f3 inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
+                       dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
+    int sp = 0;
+    uint32_t cur = b.root;
+    uint32_t spill[MAX_STACK - LDS_DEPTH];
+    auto push = [&](uint32_t e) {
+        if (sp < LDS_DEPTH) stack[sp * STRIDE] = e;
+        else spill[sp - LDS_DEPTH] = e;
+        ++sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        --sp;
+        return sp < LDS_DEPTH ? stack[sp * STRIDE] : spill[sp - LDS_DEPTH];
+    };
+
+    // a ray visits each node at most once; a longer walk means a corrupt tree
+    // and ends the query instead of looping
+    uint32_t budget = 2u * b.num_prims + 2u;
+    for (;;) {
+        // descend through inner nodes
+        while (!(cur & LEAF_BIT)) {
+            if (--budget == 0 || cur >= b.num_pairs) return r;
+            const float4* p = reinterpret_cast<const float4*>(b.pairs + cur);
+            const float4 a0 = p[0], q1 = p[1], q2 = p[2];
+            const uint4 q3 = *reinterpret_cast<const uint4*>(p + 3);
+            if (COUNT) cnt->pairs++;
+            // child 0
+            float tn0, tf0, tn1, tf1;
+            {
+                f3 lo = mk3(a0.x, a0.y, a0.z), hi = mk3(a0.w, q1.x, q1.y);
+                f3 tlo = mul3(sub3(lo, ori), inv);
+                f3 thi = mul3(sub3(hi, ori), inv);
+                tn0 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
+                tf0 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+            }
+            {
+                f3 lo = mk3(q1.z, q1.w, q2.x), hi = mk3(q2.y, q2.z, q2.w);
+                f3 tlo = mul3(sub3(lo, ori), inv);
+                f3 thi = mul3(sub3(hi, ori), inv);
+                tn1 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
+                tf1 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+            }
+            const bool h0 = tf0 >= tn0 && tn0 < r.t;
+            const bool h1 = tf1 >= tn1 && tn1 < r.t;
+            if (h0 && h1) {
+                const bool near1 = !(tn0 < tn1);
+                push(near1 ? q3.x : q3.y);
+                cur = near1 ? q3.y : q3.x;
+            } else if (h0) {
+                cur = q3.x;
+            } else if (h1) {
+                cur = q3.y;
+            } else {
+                if (sp == 0) return r;
+                cur = pop();
+            }
+        }
+        // leaf: test its triangles in order
+        {
+            const uint32_t first = cur & LEAF_FIRST_MASK;
+            const uint32_t count = ((cur >> 26) & 31u) + 1u;
+            if (first + count > b.num_prims) return r;  // corrupt entry
+            for (uint32_t k = first; k < first + count; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
+                const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                if (COUNT) cnt->tris++;
+                float t, u, v;
+                if (intersect_tri(ori, dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tc.x, tc.y, tc.z), t, u, v)) {
+                    if (t < r.t && t >= tmin && t <= tmax) {
+                        r.t = t;
+                        r.u = u;
+                        r.v = v;
+                        r.tri = (int32_t)k;
+                        if (ANY_HIT) return r;
+                    }
+                }
+            }
+        }
+        if (sp == 0) return r;
+        cur = pop();
+    }
+}
+
+// LDS part of the traversal stack (entries per lane)
+constexpr int LDS_STACK = 16;
+
+// ---------------------------------------------------------------------------
+// shared host helpers
+// ---------------------------------------------------------------------------
+int current_device();
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace vr
