@@ -1,11 +1,211 @@
-// sah.cpp -- host binned SAH builder (binned_sah_builder, detail/bvh/sah.h:176-807).
+// sah.cpp -- host binned SAH builder (binned_sah_builder with object splits,
+// detail/bvh/sah.h:176-807, driven as build_top_down, detail/bvh/build_top_down.h:27-174).
+//
+// The tree is meant to be the reference's tree node for node, so the decisions
+// follow the same arithmetic: 16 bins along the largest centroid extent, the
+// split cost 1 + (A_l / A) 3 N_l + (A_r / A) 3 N_r over half surface areas,
+// ties resolved towards the rightmost bin boundary, a leaf when the best split
+// costs more than 3 N, the partition done by std::partition (so the reference
+// order within each range is reproduced), children appended at the end of the
+// node array and the right subtree numbered before the left.  The recursion is
+// an explicit stack here.
 #include "vr_internal.h"
+
+#include <algorithm>
+#include <array>
+#include <vector>
 
 namespace vr {
 
-int sah_build_host(const vr_triangle*, size_t, int, vr_bvh_node*, size_t*, uint32_t*)
+namespace {
+
+constexpr int NUM_BINS = 16;
+
+struct box {
+    f3 lo{FLT_MAX, FLT_MAX, FLT_MAX};
+    f3 hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void insert(f3 p)
+    {
+        lo = min3v(lo, p);
+        hi = max3v(hi, p);
+    }
+    void insert(const box& b)
+    {
+        lo = min3v(lo, b.lo);
+        hi = max3v(hi, b.hi);
+    }
+    f3 center() const { return muls3(add3(hi, lo), 0.5f); }
+    f3 size() const { return sub3(hi, lo); }
+};
+
+inline float axis_of(f3 v, int a) { return a == 0 ? v.x : a == 1 ? v.y : v.z; }
+
+// safe_half_surface_area (math/detail/aabb.inl:211-217)
+inline float safe_hsa(const box& b)
 {
-    return fail(VR_ERR_UNSUPPORTED, "vr_sah_build: not available in this build");
+    f3 s = b.size();
+    s = f3{tmax_f(0.0f, s.x), tmax_f(0.0f, s.y), tmax_f(0.0f, s.z)};
+    return s.x * s.y + s.y * s.z + s.z * s.x;
+}
+
+// max_index (math/detail/vector.inl:874-884): ties go to the later axis
+inline int max_index3(f3 u)
+{
+    int i = u.y < u.x ? 0 : 1;
+    return axis_of(u, 2) < axis_of(u, i) ? i : 2;
+}
+
+struct prim_ref {
+    box bounds;
+    int index;
+};
+
+struct bin {
+    box prim_bounds, cent_bounds;
+    int enter = 0, leave = 0;
+    void merge(const bin& r)
+    {
+        prim_bounds.insert(r.prim_bounds);
+        cent_bounds.insert(r.cent_bounds);
+        enter += r.enter;
+        leave += r.leave;
+    }
+};
+
+struct range_info {
+    box prim_bounds, cent_bounds;
+    int first, last;  // refs[first, last)
+    uint32_t node;
+};
+
+struct projection {
+    float k0, k1;
+    int axis;
+    projection(const box& b, int a) : k0(axis_of(b.lo, a)), k1((float)NUM_BINS / (axis_of(b.hi, a) - k0)), axis(a) {}
+    int project_unsafe(f3 v) const { return (int)(k1 * (axis_of(v, axis) - k0)); }
+    int project(f3 v) const { return std::min(std::max(project_unsafe(v), 0), NUM_BINS - 1); }
+};
+
+inline void set_node(vr_bvh_node& n, const box& b, uint32_t first, uint32_t count, uint8_t axis, uint8_t sign)
+{
+    n.lo[0] = b.lo.x; n.lo[1] = b.lo.y; n.lo[2] = b.lo.z;
+    n.hi[0] = b.hi.x; n.hi[1] = b.hi.y; n.hi[2] = b.hi.z;
+    n.first = first;
+    n.num_prims = (uint16_t)count;
+    n.axis = axis;
+    n.sign = sign;
+}
+
+}  // namespace
+
+int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                   uint32_t* indices)
+{
+    *num_nodes = 0;
+    if (n == 0) return VR_OK;
+    VR_CHECK(n < (size_t)1 << 31, VR_ERR_UNSUPPORTED, "vr_sah_build: too many primitives");
+    if (max_leaf_size <= 0) max_leaf_size = 4;  // build_top_down.h:153-156
+    VR_CHECK(max_leaf_size <= 65535, VR_ERR_INVALID_ARGUMENT, "vr_sah_build: leaf size exceeds bvh_node range");
+
+    std::vector<prim_ref> refs(n);
+    range_info root;
+    root.first = 0;
+    root.last = (int)n;
+    root.node = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const vr_triangle& t = prims[i];
+        const f3 v1 = mk3(t.v1[0], t.v1[1], t.v1[2]);
+        box b;
+        b.insert(v1);
+        b.insert(add3(v1, mk3(t.e1[0], t.e1[1], t.e1[2])));
+        b.insert(add3(v1, mk3(t.e2[0], t.e2[1], t.e2[2])));
+        refs[i].bounds = b;
+        refs[i].index = (int)i;
+        root.prim_bounds.insert(b);
+        root.cent_bounds.insert(b.center());
+    }
+
+    size_t node_count = 1, index_count = 0;
+    std::vector<range_info> stack{root};
+    while (!stack.empty()) {
+        const range_info leaf = stack.back();
+        stack.pop_back();
+        const int size = leaf.last - leaf.first;
+        bool do_split = size > max_leaf_size;
+        int axis = 0;
+        projection pr(leaf.cent_bounds, 0);
+        int best = -1;
+        bin L, R;
+        if (do_split) {
+            const f3 cs = leaf.cent_bounds.size();
+            axis = max_index3(cs);
+            if (!(axis_of(cs, axis) > 0.0f)) do_split = false;
+        }
+        if (do_split) {
+            pr = projection(leaf.cent_bounds, axis);
+            std::array<bin, NUM_BINS> bins;
+            for (int i = leaf.first; i < leaf.last; ++i) {
+                const f3 c = refs[i].bounds.center();
+                bin& b = bins[pr.project(c)];
+                b.prim_bounds.insert(refs[i].bounds);
+                b.cent_bounds.insert(c);
+                b.enter++;
+                b.leave++;
+            }
+            // sweeps (find_split, detail/bvh/sah.h:343-400)
+            const float hsa_parent = safe_hsa(leaf.prim_bounds);
+            std::array<bin, NUM_BINS> acc_l, acc_r;
+            acc_l[0] = bins[0];
+            for (int i = 1; i < NUM_BINS; ++i) {
+                acc_l[i] = acc_l[i - 1];
+                acc_l[i].merge(bins[i]);
+            }
+            acc_r[NUM_BINS - 1] = bins[NUM_BINS - 1];
+            float best_cost = FLT_MAX;
+            for (int i = NUM_BINS - 1; i > 0; --i) {
+                acc_r[i - 1] = acc_r[i];
+                acc_r[i - 1].merge(bins[i - 1]);
+                const bin& l = acc_l[i - 1];
+                const bin& r = acc_r[i];
+                const float cost = 1.0f + (safe_hsa(l.prim_bounds) / hsa_parent) * (3.0f * (float)l.enter) +
+                                   (safe_hsa(r.prim_bounds) / hsa_parent) * (3.0f * (float)r.leave);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best = i;
+                }
+            }
+            if (best < 0 || best_cost > 3.0f * (float)size) {
+                do_split = false;
+            } else {
+                L = acc_l[best - 1];
+                R = acc_r[best];
+            }
+        }
+        if (!do_split) {
+            // leaf: its references, in range order, go to the index list
+            VR_CHECK(size <= 65535, VR_ERR_UNSUPPORTED, "vr_sah_build: leaf larger than bvh_node allows");
+            box b = leaf.prim_bounds;
+            set_node(nodes[leaf.node], b, (uint32_t)index_count, (uint32_t)size, 0, 0);
+            for (int i = leaf.first; i < leaf.last; ++i) indices[index_count++] = (uint32_t)refs[i].index;
+            continue;
+        }
+        const int split_bin = best;
+        auto pivot = std::partition(refs.begin() + leaf.first, refs.begin() + leaf.last,
+                                    [&](const prim_ref& x) { return pr.project_unsafe(x.bounds.center()) < split_bin; });
+        const int mid = (int)(pivot - refs.begin());
+        const uint32_t first_child = (uint32_t)node_count;
+        VR_CHECK(node_count + 2 <= 2 * n - 1, VR_ERR_INVALID_ARGUMENT, "vr_sah_build: node array overflow");
+        const uint8_t sign = axis_of(L.prim_bounds.lo, axis) < axis_of(R.prim_bounds.lo, axis) ? 0 : 1;
+        set_node(nodes[leaf.node], leaf.prim_bounds, first_child, 0, (uint8_t)axis, sign);
+        node_count += 2;
+        range_info lc{L.prim_bounds, L.cent_bounds, leaf.first, mid, first_child};
+        range_info rc{R.prim_bounds, R.cent_bounds, mid, leaf.last, first_child + 1};
+        // the right subtree is built (and numbered) first
+        stack.push_back(lc);
+        stack.push_back(rc);
+    }
+    *num_nodes = node_count;
+    return VR_OK;
 }
 
 }  // namespace vr
