@@ -222,6 +222,25 @@ int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_
                          const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                          float* d_accum, float* d_color, uint64_t* d_stats, void* stream);
 
+/* ---- differentiation (no reference counterpart: the reference renders
+ * only; these two calls are the forward/backward pair of a differentiable
+ * frame) --------------------------------------------------------------- */
+/* L2 image loss sum over pixels and RGB of (I - T)^2 into *d_loss (FP64,
+ * accumulated) and, when d_dl_dimage is not NULL, its image gradient
+ * 2 (I - T) (RGBA32F, alpha 0).  Images are RGBA32F, num_pixels each. */
+int vr_image_l2_loss(const float* d_image, const float* d_target, size_t num_pixels, float* d_dl_dimage,
+                     double* d_loss, void* stream);
+
+/* Path-replay adjoint of vr_render_pt with the same arguments: replays every
+ * path of that frame (same seeds, hits recomputed) and accumulates into
+ * d_grad[6 * m + 0..2] d loss / d cd (matte albedo) and d_grad[6 * m + 3..5]
+ * d loss / d ce (emission) of material m, FP64, given d loss / d image
+ * (RGBA32F).  Russian-roulette probabilities are held constant.  At most 2048
+ * materials and 16 bounces.  d_ray_counts as in vr_render_pt. */
+int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                          const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
+                          const float* d_dl_dimage, double* d_grad, uint64_t* d_ray_counts, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,7 @@ def lib():
         l.vro_refit.argtypes = [P, SZ, P, P]
         l.vro_render_ao.argtypes = [P, P, P, SZ, P, P, P, P, C.c_int, C.c_int]
         l.vro_render_pt.argtypes = [P, P, P, SZ, P, P, P, P, P, C.c_int, C.c_int, P]
+        l.vro_render_pt_adjoint.argtypes = [P, P, P, SZ, P, P, P, P, P, P, C.c_int, C.c_int]
         _lib = l
     return _lib
 
@@ -202,6 +203,23 @@ def render_pt(nodes, tris, indices, mats, cam, params, rows=None):
     lib().vro_render_pt(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(cam), _p(params), _p(acc),
                         _p(col), y0, y1, _p(counts))
     return col, counts
+
+
+def render_pt_adjoint(nodes, tris, indices, mats, cam, params, dl_dimg, rows=None):
+    """FP64 adjoint: -> (grad_cd (nm, 3), grad_ce (nm, 3))"""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    mats = np.ascontiguousarray(mats, MAT_DTYPE)
+    cam = np.ascontiguousarray(cam, CAMERA_DTYPE)
+    dl = np.ascontiguousarray(dl_dimg, np.float32)
+    h = int(params["height"][0])
+    gcd = np.zeros((mats.size, 3), np.float64)
+    gce = np.zeros((mats.size, 3), np.float64)
+    y0, y1 = rows if rows else (0, h)
+    lib().vro_render_pt_adjoint(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(cam), _p(params), _p(dl),
+                                _p(gcd), _p(gce), y0, y1)
+    return gcd, gce
 
 
 # ---------------------------------------------------------------------------

@@ -724,6 +724,109 @@ static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t
     else { out[0] = p->bg[0]; out[1] = p->bg[1]; out[2] = p->bg[2]; out[3] = 1.0f; }
 }
 
+/* ---------------------------------------------------------------------------
+ * Adjoint of the path tracer (no reference counterpart: the reference has no
+ * differentiation).  FP64 accumulation of d(loss)/d(albedo cd) and
+ * d(loss)/d(emission ce) per material, replaying each sample's path with the
+ * forward's random stream.
+ *
+ * With no light list a path's estimate is I = T_K * E: E the emission
+ * ce * ls of the emitter that ends it (or the ambient term on escape) and
+ * T_K the throughput, the product of f_j = cd_{m_j} kd / pi * (cos / pdf)
+ * over the matte vertices j < K divided by the Russian-roulette
+ * probabilities of the vertices j >= 2.  The roulette probabilities are held
+ * constant (their decisions and values are not differentiated), which keeps
+ * the gradient estimate unbiased for the gradient of the expected image:
+ *   d I_c / d cd_{m,c} = I_c * #{j : m_j = m} / cd_{m,c}
+ *   d I_c / d ce_{e,c} = T_c * ls_e   for the ending emitter e.
+ * ------------------------------------------------------------------------- */
+static void pt_sample_adjoint(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                              const vro_material* mats, const vro_render_params* p, vro_ray ray, rng_t* g,
+                              const double w[3], double* grad_cd, double* grad_ce)
+{
+    float tp[3] = {1, 1, 1};
+    uint32_t mids[64];
+    int nm = 0;
+    for (uint32_t bounce = 0; bounce < p->num_bounces; ++bounce) {
+        hitrec hr = trace_one(nodes, tris, indices, num_prims, &ray, 0, NULL);
+        if (!hr.hit) {
+            double I[3];
+            for (int k = 0; k < 3; ++k) I[k] = (double)(p->ambient[k] * tp[k]);
+            for (int j = 0; j < nm; ++j) {
+                const vro_material* m = &mats[mids[j]];
+                for (int c = 0; c < 3; ++c) grad_cd[3 * mids[j] + c] += w[c] * I[c] / (double)m->cd[c];
+            }
+            return;
+        }
+        v3 dir = ld3(ray.dir), ori = ld3(ray.ori);
+        v3 view = neg3(dir);
+        v3 pos = add3(ori, muls(dir, hr.t));
+        const vro_tri* pr = &tris[indices[hr.list_index]];
+        v3 gn = normalize3(cross3(ld3(pr->e1), ld3(pr->e2)));
+        uint32_t mid = hr.geom_id;
+        const vro_material* m = &mats[mid];
+        if (m->type == 1) {
+            double I[3];
+            for (int k = 0; k < 3; ++k) I[k] = (double)((1.0f * tp[k]) * ((1.0f * m->ce[k]) * m->ls));
+            for (int c = 0; c < 3; ++c) grad_ce[3 * mid + c] += w[c] * (double)tp[c] * (double)m->ls;
+            for (int j = 0; j < nm; ++j) {
+                const vro_material* mj = &mats[mids[j]];
+                for (int c = 0; c < 3; ++c) grad_cd[3 * mids[j] + c] += w[c] * I[c] / (double)mj->cd[c];
+            }
+            /* the forward still draws its roulette number here */
+            if (bounce >= 2) (void)rng_next(g);
+            return;
+        }
+        v3 n = faceforward3(gn, view, gn);
+        v3 u, v, wv = n;
+        onb(wv, &u, &v);
+        float a, b;
+        rng_pair(g, p->arg_order, &a, &b);
+        v3 sp = cosine_hemisphere(a, b);
+        v3 refl = normalize3(add3(add3(smul(sp.x, u), smul(sp.y, v)), smul(sp.z, wv)));
+        float pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
+        float src[3];
+        for (int k = 0; k < 3; ++k) src[k] = 1.0f * ((m->cd[k] * m->kd) * 3.18309886183790691216444201928e-01f);
+        if (pdf <= 0.0f) {
+            if (bounce >= 2) (void)rng_next(g);
+            return;
+        }
+        if (nm < 64) mids[nm++] = mid;
+        float q = dot3(n, refl) / pdf;
+        for (int k = 0; k < 3; ++k) tp[k] = tp[k] * (src[k] * q);
+        if (bounce >= 2) {
+            float prob = tp[0];
+            if (tp[1] > prob) prob = tp[1];
+            if (tp[2] > prob) prob = tp[2];
+            int terminate = rng_next(g) > prob;
+            for (int k = 0; k < 3; ++k) tp[k] = tp[k] / prob;
+            if (terminate) return;
+        }
+        v3 o2 = add3(pos, muls(refl, p->epsilon));
+        ray = mkray(o2, refl, ray.tmin, ray.tmax);
+    }
+}
+
+void vro_render_pt_adjoint(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                           const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
+                           const float* dl_dimg, double* grad_cd, double* grad_ce, int y_begin, int y_end)
+{
+    int w = (int)p->width, h = (int)p->height;
+    for (int y = y_begin; y < y_end; ++y) {
+        for (int x = 0; x < w; ++x) {
+            size_t px = (size_t)y * w + x;
+            double wgt[3];
+            for (int k = 0; k < 3; ++k) wgt[k] = (double)dl_dimg[px * 4 + k] * (double)p->sfactor / (double)p->spp;
+            rng_t g;
+            rng_seed(&g, vro_make_random_seed(y * w + x, (int32_t)p->frame_id));
+            for (uint32_t s = 0; s < p->spp; ++s) {
+                vro_ray r = primary_ray(cam, &g, p->arg_order, x, y, w, h);
+                pt_sample_adjoint(nodes, tris, indices, num_prims, mats, p, r, &g, wgt, grad_cd, grad_ce);
+            }
+        }
+    }
+}
+
 /* sample_pixel_impl with jittered_blend (detail/sched_common.h:427-526) under
  * basic_sched::frame seeding (detail/basic_sched.inl:100-112). */
 void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,

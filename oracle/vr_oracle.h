@@ -140,6 +140,13 @@ void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* i
                    const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
                    float* accum, float* color, int y_begin, int y_end, uint64_t* ray_counts);
 
+/* Adjoint of render_pt (no reference counterpart): FP64 gradients of a loss
+ * with image gradient dl_dimg (RGBA per pixel) w.r.t. matte albedo cd and
+ * emission ce, 3 per material each, accumulated into grad_cd / grad_ce. */
+void vro_render_pt_adjoint(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                           const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
+                           const float* dl_dimg, double* grad_cd, double* grad_ce, int y_begin, int y_end);
+
 #ifdef __cplusplus
 }
 #endif

@@ -346,6 +346,16 @@ class ao_kernel:
     bg: tuple = (0.0, 0.0, 0.0)
 
 
+def image_l2_loss(image: DeviceArray, target: DeviceArray, dl_dimage: DeviceArray | None = None,
+                  stream=None) -> float:
+    """sum (I - T)^2 over pixels and RGB (FP64); writes 2 (I - T) into dl_dimage."""
+    npix = image.count // 4
+    loss = DeviceArray(1, np.float64, zero=True)
+    check(lib().vr_image_l2_loss(image.ptr, target.ptr, npix, dl_dimage.ptr if dl_dimage is not None else None,
+                                 loss.ptr, _stream(stream)))
+    return float(loss.to_host()[0])
+
+
 class hip_sched:
     """Scheduler launching one frame of a kernel (cuda_sched<R>::frame,
     detail/cuda_sched.inl:226-246): per-pixel seeds make_random_seed(y*w+x,
@@ -385,6 +395,17 @@ class hip_sched:
         else:
             raise TypeError(f"unsupported kernel {type(kernel).__name__}")
         self.frame_id += 1
+
+    def backward(self, kernel, sp: sched_params, dl_dimage: "DeviceArray", grad: "DeviceArray", stream=None,
+                 frame_id: int | None = None):
+        """Path-replay adjoint of the frame rendered with `frame_id` (default:
+        the last one): accumulates d loss / d (cd rgb, ce rgb) per material
+        into `grad` (6 x num_materials float64) given d loss / d image."""
+        s = self._sp(sp)
+        s["frame_id"] = self.frame_id - 1 if frame_id is None else frame_id
+        check(lib().vr_render_pt_backward(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,
+                                          ptr(kernel.params), ptr(sp.cam), ptr(s), dl_dimage.ptr, grad.ptr,
+                                          self.ray_counts.ptr, _stream(stream)))
 
     def rays(self) -> tuple[int, int]:
         """(primary, secondary) closest-hit queries accumulated so far."""

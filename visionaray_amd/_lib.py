@@ -59,6 +59,8 @@ SIGNATURES = {
     "vr_render_ao": (I, [P, P, P, U32, F, P, P, P, P]),
     "vr_render_pt": (I, [P, P, SZ, P, P, P, P, P, P, P]),
     "vr_render_pt_counted": (I, [P, P, SZ, P, P, P, P, P, P, P]),
+    "vr_image_l2_loss": (I, [P, P, SZ, P, P, P]),
+    "vr_render_pt_backward": (I, [P, P, SZ, P, P, P, P, P, P, P]),
 }
 
 # record layouts shared with the C header

@@ -24,6 +24,10 @@ int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& 
               const float bg[3], float* accum, float* color, hipStream_t st);
 int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
                    uint32_t* indices);
+int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                      const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
+                      unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
+int image_l2_loss(const float* img, const float* tgt, size_t npix, float* dl, double* loss, hipStream_t st);
 
 namespace {
 thread_local std::string g_error;
@@ -505,6 +509,31 @@ int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_
     if (rc) return rc;
     return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
                      reinterpret_cast<unsigned long long*>(d_stats), dev_state()->work, true, (hipStream_t)stream);
+}
+
+int vr_image_l2_loss(const float* d_image, const float* d_target, size_t num_pixels, float* d_dl_dimage,
+                     double* d_loss, void* stream)
+{
+    VR_CHECK(num_pixels == 0 || (d_image && d_target && d_loss), VR_ERR_INVALID_ARGUMENT,
+             "vr_image_l2_loss: null argument");
+    return image_l2_loss(d_image, d_target, num_pixels, d_dl_dimage, d_loss, (hipStream_t)stream);
+}
+
+int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                          const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
+                          const float* d_dl_dimage, double* d_grad, uint64_t* d_ray_counts, void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_dl_dimage && d_grad, VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_backward: null argument");
+    VR_CHECK(d_materials && num_materials > 0, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_backward: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_backward: spp must be >= 1");
+    VR_CHECK((uint64_t)sp->width * sp->height < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_backward: image too large");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_pt_adjoint(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_dl_dimage, d_grad,
+                             reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work,
+                             (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -120,6 +120,8 @@ struct pt_args {
     unsigned long long* ray_counts;
     uint32_t* work;
     uint32_t arg_order;
+    const float* dl;  // adjoint: d loss / d image, RGBA per pixel
+    double* grad;     // adjoint: 6 per material (d/d cd rgb, d/d ce rgb), accumulated
 };
 
 __device__ inline void write_pixel(const pt_args& A, uint32_t px, float4 c)
@@ -137,15 +139,38 @@ __device__ inline void write_pixel(const pt_args& A, uint32_t px, float4 c)
     reinterpret_cast<float4*>(A.color)[px] = r;
 }
 
-template <bool COUNT>
+enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2 };
+constexpr int ADJ_MAX_VERTICES = 16;
+
+// One kernel for the frame (PT_RENDER), the frame with traversal counters
+// (PT_COUNT) and the path-replay adjoint (PT_ADJOINT): all three draw the same
+// random numbers in the same order, so the adjoint replays exactly the paths
+// of the forward frame with the same seeds, recomputing every hit.
+//
+// Adjoint (no counterpart in the reference, which has no differentiation):
+// a path's estimate is I = T * E, E the ending emitter's ce * ls or the
+// ambient term on escape, T the product over its matte vertices of
+// cd kd / pi * cos / pdf divided by the roulette probabilities, which are
+// held constant (unbiased for the gradient of the expected image):
+//   d I_c / d cd_{m,c} = I_c * (vertices with material m) / cd_{m,c}
+//   d I_c / d ce_{e,c} = T_c * ls_e.
+// Per-block FP32 partials in LDS; FP64 only in the final reduction.
+template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
 {
+    constexpr bool COUNT = MODE == PT_COUNT;
+    constexpr bool ADJ = MODE == PT_ADJOINT;
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
+    extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
     const uint32_t w = A.sp.width, h = A.sp.height, npix = w * h;
     const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     const f3 amb = mk3(A.kp.ambient[0], A.kp.ambient[1], A.kp.ambient[2]);
     const float eps = A.kp.epsilon;
+    if (ADJ) {
+        for (uint32_t i = threadIdx.x; i < 6 * A.num_mats; i += BLOCK) s_grad[i] = 0.0f;
+        __syncthreads();
+    }
 
     int32_t px = -1;
     bool done = false;
@@ -158,6 +183,9 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
     unsigned long long n_primary = 0, n_secondary = 0;
     trace_counters tc{0, 0};
     unsigned long long n_pairs = 0, n_tris = 0;
+    f3 wgt = mk3(0, 0, 0);
+    uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
+    uint32_t nmid = 0;
 
     for (;;) {
         // regenerate: lanes without a pixel take the next ones, one atomic per wave
@@ -181,6 +209,13 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
                     I = mk3(0, 0, 0);
                     bounce = 0;
                     rhit = false;
+                    if (ADJ) {
+                        // the image is accum = mean(samples) * sfactor + ...
+                        const float4 dl = reinterpret_cast<const float4*>(A.dl)[my];
+                        const float k = A.sp.sfactor / (float)A.sp.spp;
+                        wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
+                        nmid = 0;
+                    }
                 } else {
                     done = true;
                 }
@@ -202,15 +237,24 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
         bool finished = false;
         if (hr.tri < 0) {
             // escaped: ambient light times throughput (pathtracing.inl:69-77)
-            I = add3(I, mul3(amb, tp));
+            const f3 e = mul3(amb, tp);
+            I = add3(I, e);
             finished = true;
+            if (ADJ) {
+                for (uint32_t j = 0; j < nmid; ++j) {
+                    const vr_material* mj = A.mats + mids[j];
+                    atomicAdd(&s_grad[6 * mids[j] + 0], wgt.x * e.x / mj->cd[0]);
+                    atomicAdd(&s_grad[6 * mids[j] + 1], wgt.y * e.y / mj->cd[1]);
+                    atomicAdd(&s_grad[6 * mids[j] + 2], wgt.z * e.z / mj->cd[2]);
+                }
+            }
         } else {
             if (bounce == 0) rhit = true;
             const f3 view = neg3(d);
             const f3 pos = add3(o, muls3(d, hr.t));
             const float4* tq = reinterpret_cast<const float4*>(A.b.tris + hr.tri);
-            const float4 tb = tq[1], tc = tq[2];
-            const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tc)));
+            const float4 tb = tq[1], tcq = tq[2];
+            const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
             uint32_t mid = __float_as_uint(tb.w);
             mid = mid < A.num_mats ? mid : A.num_mats - 1u;
             const vr_material* m = A.mats + mid;
@@ -242,8 +286,23 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
             bool active = true;
             if (emission) {
                 // no light list: MIS weight 1 (pathtracing.inl:134-146)
-                I = add3(I, mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src));
+                const f3 e = mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src);
+                I = add3(I, e);
                 active = false;
+                if (ADJ) {
+                    const float ls = m->ls;
+                    atomicAdd(&s_grad[6 * mid + 3], wgt.x * tp.x * ls);
+                    atomicAdd(&s_grad[6 * mid + 4], wgt.y * tp.y * ls);
+                    atomicAdd(&s_grad[6 * mid + 5], wgt.z * tp.z * ls);
+                    for (uint32_t j = 0; j < nmid; ++j) {
+                        const vr_material* mj = A.mats + mids[j];
+                        atomicAdd(&s_grad[6 * mids[j] + 0], wgt.x * e.x / mj->cd[0]);
+                        atomicAdd(&s_grad[6 * mids[j] + 1], wgt.y * e.y / mj->cd[1]);
+                        atomicAdd(&s_grad[6 * mids[j] + 2], wgt.z * e.z / mj->cd[2]);
+                    }
+                }
+            } else if (ADJ && !zero_pdf && nmid < ADJ_MAX_VERTICES) {
+                mids[nmid++] = mid;
             }
             if (zero_pdf) active = false;
             const f3 n = faceforward3(gn, view, gn);
@@ -269,14 +328,17 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
             }
         }
         if (finished) {
-            if (rhit) {
-                sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
-            } else {
-                sum.x += A.kp.bg[0]; sum.y += A.kp.bg[1]; sum.z += A.kp.bg[2]; sum.w += 1.0f;
+            if (!ADJ) {
+                if (rhit) {
+                    sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
+                } else {
+                    sum.x += A.kp.bg[0]; sum.y += A.kp.bg[1]; sum.z += A.kp.bg[2]; sum.w += 1.0f;
+                }
             }
             ++s;
+            if (ADJ) nmid = 0;
             if (s >= A.sp.spp) {
-                write_pixel(A, (uint32_t)px, sum);
+                if (!ADJ) write_pixel(A, (uint32_t)px, sum);
                 px = -1;
             } else {
                 const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
@@ -305,6 +367,14 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
                 atomicAdd(&A.ray_counts[2], n_pairs);
                 atomicAdd(&A.ray_counts[3], n_tris);
             }
+        }
+    }
+    if (ADJ) {
+        // FP32 block partials -> FP64 gradient
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 6 * A.num_mats; i += BLOCK) {
+            const float v = s_grad[i];
+            if (v != 0.0f) atomicAdd(&A.grad[i], (double)v);
         }
     }
 }
@@ -376,18 +446,20 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
-template <bool COUNT>
-int launch_pt(const pt_args& A, hipStream_t st)
+template <int MODE>
+int launch_pt(const pt_args& A, size_t dyn_lds, hipStream_t st)
 {
-    static int grid_cache = 0;
-    if (grid_cache == 0) {
+    static int grid_cache[3] = {0, 0, 0};
+    static size_t lds_cache[3] = {0, 0, 0};
+    if (grid_cache[MODE] == 0 || lds_cache[MODE] != dyn_lds) {
         int dev = 0, per_cu = 0, cus = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
-        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<COUNT>, BLOCK, 0));
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE>, BLOCK, dyn_lds));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid_cache = std::max(1, per_cu) * std::max(1, cus);
+        grid_cache[MODE] = std::max(1, per_cu) * std::max(1, cus);
+        lds_cache[MODE] = dyn_lds;
     }
-    hipLaunchKernelGGL(k_pathtrace<COUNT>, dim3(grid_cache), dim3(BLOCK), 0, st, A);
+    hipLaunchKernelGGL(k_pathtrace<MODE>, dim3(grid_cache[MODE]), dim3(BLOCK), dyn_lds, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
@@ -414,9 +486,74 @@ int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, con
     A.ray_counts = ray_counts;
     A.work = work;
     A.arg_order = g_arg_order;
+    A.dl = nullptr;
+    A.grad = nullptr;
     VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, work);
-    return counted ? launch_pt<true>(A, st) : launch_pt<false>(A, st);
+    return counted ? launch_pt<PT_COUNT>(A, 0, st) : launch_pt<PT_RENDER>(A, 0, st);
+}
+
+// largest material table the adjoint keeps in LDS partials
+constexpr size_t ADJ_MAX_MATERIALS = 2048;
+
+int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                      const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
+                      unsigned long long* ray_counts, uint32_t* work, hipStream_t st)
+{
+    if ((size_t)sp.width * sp.height == 0) return VR_OK;
+    VR_CHECK(num_mats <= ADJ_MAX_MATERIALS, VR_ERR_UNSUPPORTED, "render_pt_backward: more than 2048 materials");
+    VR_CHECK(kp.num_bounces <= (uint32_t)ADJ_MAX_VERTICES, VR_ERR_UNSUPPORTED,
+             "render_pt_backward: more than 16 bounces");
+    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_backward: BVH deeper than the stack");
+    pt_args A;
+    A.b = bs.view();
+    A.mats = mats;
+    A.num_mats = (uint32_t)num_mats;
+    A.kp = kp;
+    A.cam = cam;
+    A.sp = sp;
+    A.accum = nullptr;
+    A.color = nullptr;
+    A.ray_counts = ray_counts;
+    A.work = work;
+    A.arg_order = g_arg_order;
+    A.dl = dl;
+    A.grad = grad;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, work);
+    return launch_pt<PT_ADJOINT>(A, 6 * num_mats * sizeof(float), st);
+}
+
+// L2 image loss sum_px sum_rgb (I - T)^2 (FP64 reduction) and its image
+// gradient 2 (I - T) (alpha 0).
+__global__ __launch_bounds__(BLOCK) void k_l2_loss(const float4* __restrict__ img, const float4* __restrict__ tgt,
+                                                   size_t npix, float4* __restrict__ dl, double* __restrict__ loss)
+{
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < npix; i += (size_t)gridDim.x * BLOCK) {
+        const float4 a = img[i], b = tgt[i];
+        const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        if (dl) dl[i] = make_float4(2.0f * dx, 2.0f * dy, 2.0f * dz, 0.0f);
+        acc += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    __shared__ double s[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < BLOCK / 64; ++q) t += s[q];
+        atomicAdd(loss, t);
+    }
+}
+
+int image_l2_loss(const float* img, const float* tgt, size_t npix, float* dl, double* loss, hipStream_t st)
+{
+    if (npix == 0) return VR_OK;
+    const unsigned grid = std::max(1u, std::min(div_up(npix, BLOCK), 2048u));
+    hipLaunchKernelGGL(k_l2_loss, dim3(grid), dim3(BLOCK), 0, st, reinterpret_cast<const float4*>(img),
+                       reinterpret_cast<const float4*>(tgt), npix, reinterpret_cast<float4*>(dl), loss);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
 }
 
 int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& sp, uint32_t samples, float radius,
