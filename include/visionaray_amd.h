@@ -241,6 +241,41 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
                           const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                           const float* d_dl_dimage, double* d_grad, uint64_t* d_ray_counts, void* stream);
 
+/* ---- batched scenes (no reference counterpart: the reference launches one
+ * cuda_sched::frame per scene, detail/cuda_sched.inl:226-246) ------------- */
+/* One scene of a batch: what one reference frame() call renders. */
+typedef struct vr_scene {
+    vr_bvh bvh;
+    const vr_material* d_materials;
+    size_t num_materials;
+    vr_kernel_params kp;
+    vr_camera cam;
+    vr_sched_params sp;          /* frame_id: the scene's first frame */
+    float* d_accum;
+    float* d_color;
+    const float* d_dl_dimage;    /* for vr_batch_backward only; may be NULL */
+} vr_scene;
+
+typedef struct vr_batch_t* vr_batch;
+
+/* Prepare a batch: scene descriptors and the pixel prefix go to device
+ * memory once.  The scenes' BVHs and buffers must outlive the batch. */
+int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out);
+/* One frame of every scene in a single persistent launch (pixels of all
+ * scenes back to back; paths regenerate across scene boundaries).  Frame k
+ * of the batch (k = 0, 1, ...) seeds scene s with sp.frame_id + k.  Enqueues
+ * kernels only, so it can be captured in a HIP graph. */
+int vr_batch_render(vr_batch batch, uint64_t* d_ray_counts, void* stream);
+/* Adjoint of the batch's last frame: d_grad holds 6 slots per material, the
+ * scenes' material tables concatenated in scene order (at most 2048 in
+ * total); each scene's d_dl_dimage must be set. */
+int vr_batch_backward(vr_batch batch, double* d_grad, uint64_t* d_ray_counts, void* stream);
+/* Capture vr_batch_render on `stream` (not the NULL stream) into a graph
+ * owned by the batch; vr_batch_launch_graph replays it (one frame each). */
+int vr_batch_capture_graph(vr_batch batch, uint64_t* d_ray_counts, void* stream);
+int vr_batch_launch_graph(vr_batch batch, void* stream);
+int vr_batch_destroy(vr_batch batch);
+
 #ifdef __cplusplus
 }
 #endif

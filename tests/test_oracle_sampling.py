@@ -156,3 +156,51 @@ def test_pt_diffuse_plane_under_sky(O, albedo):
     p = O.render_params(8, 8, spp=4, num_bounces=2, epsilon=1e-3, ambient=sky)
     img, _ = O.render_pt(nodes, tris, idx, m, _cam(O, 8, 8), p)
     assert np.allclose(img[..., :3], sky * np.array(albedo, np.float32), rtol=2e-6)
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    """The FP64 adjoint against central differences with common random
+    numbers, at depth 2 where no Russian roulette makes the estimator
+    discontinuous: d/d albedo and d/d emission must agree (the derivative of a
+    product of albedos and one emission is exact; FD error is O(h^2))."""
+    import oracle as O
+    import visionaray_amd as V
+    rng = np.random.default_rng(0)
+    tris = S.config0_scene(level=2)
+    nm = 10
+    n_s = tris.size - 24
+    tris["geom_id"] = np.concatenate([np.arange(24) % 4, 4 + (np.arange(n_s) * (nm - 4) // n_s)])
+    mats = S.lambert_materials(nm, seed=3, lo=0.1, hi=0.9)
+    mats["type"][0] = 1
+    mats["ce"][0] = (1.2, 0.8, 1.0)
+    mats["ls"][0] = 1.0
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    w = h = 16
+    cam = V.pinhole_camera(*S.config0_camera(w, h)).astype(O.CAMERA_DTYPE)
+    p = O.render_params(w, h, spp=4, num_bounces=2, epsilon=1e-4, ambient=(0.3, 0.3, 0.3))
+    target = mats.copy()
+    target["cd"] = np.clip(mats["cd"] + rng.normal(0, 0.05, mats["cd"].shape), 0.01, 0.99)
+    T, _ = O.render_pt(nodes, tris, idx, target, cam, p)
+
+    def loss(m):
+        img, _ = O.render_pt(nodes, tris, idx, m, cam, p)
+        return float(np.sum((img[..., :3].astype(np.float64) - T[..., :3]) ** 2)), img
+
+    _, img = loss(mats)
+    dl = np.zeros_like(img)
+    dl[..., :3] = 2 * (img[..., :3] - T[..., :3])
+    gcd, gce = O.render_pt_adjoint(nodes, tris, idx, mats, cam, p, dl)
+    hstep = 1e-3
+    a, f = [], []
+    for m in range(nm):
+        key = "ce" if mats["type"][m] == 1 else "cd"
+        for c in range(3):
+            mp, mm = mats.copy(), mats.copy()
+            mp[key][m, c] += hstep
+            mm[key][m, c] -= hstep
+            f.append((loss(mp)[0] - loss(mm)[0]) / (2 * hstep))
+            a.append((gce if key == "ce" else gcd)[m, c])
+    a, f = np.array(a), np.array(f)
+    assert np.count_nonzero(a) > 10
+    assert float(a @ f / np.linalg.norm(a) / np.linalg.norm(f)) > 0.99999
+    assert np.allclose(a, f, rtol=1e-3, atol=1e-6 * np.abs(f).max())

@@ -346,6 +346,60 @@ class ao_kernel:
     bg: tuple = (0.0, 0.0, 0.0)
 
 
+class batch:
+    """Many scenes rendered by one persistent launch (vr_batch_*).  Each entry
+    is (pathtracing.kernel, sched_params[, d loss / d image DeviceArray]); the
+    scenes' BVHs, materials and render targets must stay alive."""
+
+    def __init__(self, entries, frame_id: int = 0):
+        _ensure()
+        self._keep = list(entries)
+        sc = np.zeros(len(self._keep), _lib.SCENE_DTYPE)
+        for i, e in enumerate(self._keep):
+            k, sp = e[0], e[1]
+            dl = e[2] if len(e) > 2 else None
+            sc[i]["bvh"] = k.bvh.handle.value or 0
+            sc[i]["materials"] = k.materials.ptr.value or 0
+            sc[i]["num_materials"] = k.materials.count
+            sc[i]["kp"] = k.params[0]
+            sc[i]["cam"] = sp.cam[0]
+            sc[i]["sp"]["width"], sc[i]["sp"]["height"] = sp.rt.width, sp.rt.height
+            sc[i]["sp"]["spp"], sc[i]["sp"]["frame_id"] = sp.sample_params.spp, frame_id
+            sc[i]["sp"]["sfactor"], sc[i]["sp"]["dfactor"] = sp.sample_params.sfactor, sp.sample_params.dfactor
+            sc[i]["accum"] = sp.rt.accum.ptr.value or 0
+            sc[i]["color"] = sp.rt.color.ptr.value or 0
+            sc[i]["dl"] = (dl.ptr.value or 0) if dl is not None else 0
+        self.num_materials = int(sum(int(e[0].materials.count) for e in self._keep))
+        self.handle = C.c_void_p()
+        check(lib().vr_batch_create(ptr(sc), sc.size, C.byref(self.handle)))
+        self.ray_counts = DeviceArray(2, np.uint64, zero=True)
+
+    def render(self, stream=None):
+        check(lib().vr_batch_render(self.handle, self.ray_counts.ptr, _stream(stream)))
+
+    def backward(self, grad: DeviceArray, stream=None):
+        check(lib().vr_batch_backward(self.handle, grad.ptr, self.ray_counts.ptr, _stream(stream)))
+
+    def capture_graph(self, stream: Stream):
+        check(lib().vr_batch_capture_graph(self.handle, self.ray_counts.ptr, _stream(stream)))
+
+    def launch_graph(self, stream: Stream):
+        check(lib().vr_batch_launch_graph(self.handle, _stream(stream)))
+
+    def rays(self) -> tuple[int, int]:
+        c = self.ray_counts.to_host()
+        return int(c[0]), int(c[1])
+
+    def reset_counters(self):
+        check(lib().vr_memset(self.ray_counts.ptr, 0, self.ray_counts.nbytes))
+
+    def __del__(self):
+        try:
+            lib().vr_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+
 def image_l2_loss(image: DeviceArray, target: DeviceArray, dl_dimage: DeviceArray | None = None,
                   stream=None) -> float:
     """sum (I - T)^2 over pixels and RGB (FP64); writes 2 (I - T) into dl_dimage."""

@@ -61,6 +61,12 @@ SIGNATURES = {
     "vr_render_pt_counted": (I, [P, P, SZ, P, P, P, P, P, P, P]),
     "vr_image_l2_loss": (I, [P, P, SZ, P, P, P]),
     "vr_render_pt_backward": (I, [P, P, SZ, P, P, P, P, P, P, P]),
+    "vr_batch_create": (I, [P, SZ, C.POINTER(P)]),
+    "vr_batch_render": (I, [P, P, P]),
+    "vr_batch_backward": (I, [P, P, P, P]),
+    "vr_batch_capture_graph": (I, [P, P, P]),
+    "vr_batch_launch_graph": (I, [P, P]),
+    "vr_batch_destroy": (I, [P]),
 }
 
 # record layouts shared with the C header
@@ -73,10 +79,16 @@ KERNEL_PARAMS_DTYPE = np.dtype([("num_bounces", "<u4"), ("epsilon", "<f4"), ("bg
                                 ("ambient", "<f4", (3,))])
 SCHED_PARAMS_DTYPE = np.dtype([("width", "<u4"), ("height", "<u4"), ("spp", "<u4"), ("frame_id", "<u4"),
                                ("sfactor", "<f4"), ("dfactor", "<f4")])
+SCENE_DTYPE = np.dtype([
+    ("bvh", "<u8"), ("materials", "<u8"), ("num_materials", "<u8"),
+    ("kp", KERNEL_PARAMS_DTYPE), ("cam", CAMERA_DTYPE), ("sp", SCHED_PARAMS_DTYPE),
+    ("accum", "<u8"), ("color", "<u8"), ("dl", "<u8"),
+])
 BVH_INFO_DTYPE = np.dtype([("num_prims", "<u8"), ("num_nodes", "<u8"), ("depth", "<u4"),
                            ("max_leaf_size", "<u4")])
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE_DTYPE.itemsize == 32
 assert CAMERA_DTYPE.itemsize == 48 and KERNEL_PARAMS_DTYPE.itemsize == 32 and SCHED_PARAMS_DTYPE.itemsize == 24
+assert SCENE_DTYPE.itemsize == 152
 
 
 class VRError(RuntimeError):

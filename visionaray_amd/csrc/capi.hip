@@ -28,6 +28,13 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
                       unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
 int image_l2_loss(const float* img, const float* tgt, size_t npix, float* dl, double* loss, hipStream_t st);
+int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num_scenes, uint32_t total_pixels,
+                    uint32_t* d_frame, bool advance_frame, unsigned long long* ray_counts, uint32_t* work,
+                    uint32_t grad_slots, double* grad, hipStream_t st);
+size_t scene_desc_size();
+void make_scene_desc(void* out, const bvh_state& bs, const vr_material* mats, uint32_t num_mats, uint32_t grad_base,
+                     const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp, float* accum,
+                     float* color, const float* dl);
 
 namespace {
 thread_local std::string g_error;
@@ -67,6 +74,20 @@ using namespace vr;
 
 struct vr_bvh_t {
     bvh_state s;
+};
+
+struct vr_batch_t {
+    int device = 0;
+    uint32_t num_scenes = 0;
+    uint32_t total_pixels = 0;
+    uint32_t total_materials = 0;
+    uint32_t max_depth = 0;
+    void* d_scenes = nullptr;      // scene descriptors
+    uint32_t* d_prefix = nullptr;  // num_scenes + 1
+    uint32_t* d_frame = nullptr;   // frame counter (starts at -1, advanced before each frame)
+    uint32_t* d_work = nullptr;    // the batch's own work counter (graph replays)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 #define VR_TRY_BEGIN try {
@@ -534,6 +555,118 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
     return render_pt_adjoint(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_dl_dimage, d_grad,
                              reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work,
                              (hipStream_t)stream);
+}
+
+int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(out && (num_scenes == 0 || h_scenes), VR_ERR_INVALID_ARGUMENT, "vr_batch_create: null argument");
+    VR_CHECK(num_scenes < (1u << 20), VR_ERR_INVALID_ARGUMENT, "vr_batch_create: too many scenes");
+    int rc = ensure_device();
+    if (rc) return rc;
+    const size_t dsz = scene_desc_size();
+    std::vector<unsigned char> descs(std::max<size_t>(num_scenes, 1) * dsz);
+    std::vector<uint32_t> prefix(num_scenes + 1, 0);
+    uint64_t total = 0, mats = 0;
+    uint32_t depth = 0;
+    for (size_t i = 0; i < num_scenes; ++i) {
+        const vr_scene& sc = h_scenes[i];
+        VR_CHECK(sc.bvh && sc.d_accum && sc.d_color, VR_ERR_INVALID_ARGUMENT, "vr_batch_create: scene without bvh/target");
+        VR_CHECK(sc.bvh->s.num_prims == 0 || (sc.d_materials && sc.num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+                 "vr_batch_create: scene without materials");
+        VR_CHECK(sc.sp.spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_batch_create: spp must be >= 1");
+        make_scene_desc(&descs[i * dsz], sc.bvh->s, sc.d_materials, (uint32_t)sc.num_materials, (uint32_t)mats,
+                        sc.kp, sc.cam, sc.sp, sc.d_accum, sc.d_color, sc.d_dl_dimage);
+        prefix[i] = (uint32_t)total;
+        total += (uint64_t)sc.sp.width * sc.sp.height;
+        mats += sc.num_materials;
+        depth = std::max(depth, sc.bvh->s.depth);
+    }
+    VR_CHECK(total < (1ull << 31), VR_ERR_INVALID_ARGUMENT, "vr_batch_create: more than 2^31 pixels");
+    VR_CHECK(depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "vr_batch_create: BVH deeper than the traversal stack");
+    prefix[num_scenes] = (uint32_t)total;
+    vr_batch b = new vr_batch_t;
+    b->device = current_device();
+    b->num_scenes = (uint32_t)num_scenes;
+    b->total_pixels = (uint32_t)total;
+    b->total_materials = (uint32_t)std::min<uint64_t>(mats, 0xFFFFFFFFu);
+    b->max_depth = depth;
+    const uint32_t start = 0xFFFFFFFFu;
+    if (hipMalloc(&b->d_scenes, descs.size()) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->d_prefix), prefix.size() * 4) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->d_frame), 256) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->d_work), 256) != hipSuccess ||
+        hipMemcpy(b->d_scenes, descs.data(), descs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_frame, &start, 4, hipMemcpyHostToDevice) != hipSuccess) {
+        vr_batch_destroy(b);
+        return fail(VR_ERR_OUT_OF_MEMORY, "vr_batch_create: device allocation or upload failed");
+    }
+    *out = b;
+    return VR_OK;
+    VR_TRY_END
+}
+
+int vr_batch_render(vr_batch batch, uint64_t* d_ray_counts, void* stream)
+{
+    VR_CHECK(batch, VR_ERR_INVALID_ARGUMENT, "vr_batch_render: null handle");
+    return render_pt_batch(batch->d_scenes, batch->d_prefix, batch->num_scenes, batch->total_pixels, batch->d_frame,
+                           true, reinterpret_cast<unsigned long long*>(d_ray_counts), batch->d_work, 0, nullptr,
+                           (hipStream_t)stream);
+}
+
+int vr_batch_backward(vr_batch batch, double* d_grad, uint64_t* d_ray_counts, void* stream)
+{
+    VR_CHECK(batch && d_grad, VR_ERR_INVALID_ARGUMENT, "vr_batch_backward: null argument");
+    return render_pt_batch(batch->d_scenes, batch->d_prefix, batch->num_scenes, batch->total_pixels, batch->d_frame,
+                           false, reinterpret_cast<unsigned long long*>(d_ray_counts), batch->d_work,
+                           batch->total_materials, d_grad, (hipStream_t)stream);
+}
+
+int vr_batch_capture_graph(vr_batch batch, uint64_t* d_ray_counts, void* stream)
+{
+    VR_CHECK(batch, VR_ERR_INVALID_ARGUMENT, "vr_batch_capture_graph: null handle");
+    VR_CHECK(stream != nullptr, VR_ERR_INVALID_ARGUMENT, "vr_batch_capture_graph: capture needs a created stream");
+    hipStream_t st = (hipStream_t)stream;
+    if (batch->exec) {
+        (void)hipGraphExecDestroy(batch->exec);
+        batch->exec = nullptr;
+    }
+    if (batch->graph) {
+        (void)hipGraphDestroy(batch->graph);
+        batch->graph = nullptr;
+    }
+    VR_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = vr_batch_render(batch, d_ray_counts, stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    VR_HIP_CHECK(e);
+    batch->graph = g;
+    VR_HIP_CHECK(hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0));
+    return VR_OK;
+}
+
+int vr_batch_launch_graph(vr_batch batch, void* stream)
+{
+    VR_CHECK(batch && batch->exec, VR_ERR_INVALID_ARGUMENT, "vr_batch_launch_graph: no captured graph");
+    VR_HIP_CHECK(hipGraphLaunch(batch->exec, (hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_batch_destroy(vr_batch batch)
+{
+    if (!batch) return VR_OK;
+    if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
+    if (batch->graph) (void)hipGraphDestroy(batch->graph);
+    void* ptrs[] = {batch->d_scenes, batch->d_prefix, batch->d_frame, batch->d_work};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete batch;
+    return VR_OK;
 }
 
 }  // extern "C"

@@ -11,6 +11,9 @@
 // while the wave's longest path finishes.
 #include "vr_internal.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace vr {
 
 namespace {
@@ -108,35 +111,48 @@ __device__ inline uint64_t lane_mask_lt_count(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-struct pt_args {
+// Everything one scene contributes to a frame (the kernel_params, camera,
+// sampler and render target of one reference frame() call).
+struct scene_desc {
     bvh_view b;
     const vr_material* mats;
     uint32_t num_mats;
+    uint32_t grad_base;  // adjoint: first gradient slot (material) of this scene
     vr_kernel_params kp;
     vr_camera cam;
     vr_sched_params sp;
     float* accum;
     float* color;
+    const float* dl;  // adjoint: d loss / d image, RGBA per pixel
+};
+
+struct pt_args {
+    scene_desc single;           // one scene (BATCH = false)
+    const scene_desc* scenes;    // BATCH: num_scenes descriptors
+    const uint32_t* prefix;      // BATCH: first global pixel of every scene, num_scenes + 1 entries
+    uint32_t num_scenes;
+    uint32_t total_pixels;
+    const uint32_t* frame_offset;  // optional device frame counter added to every frame_id
     unsigned long long* ray_counts;
     uint32_t* work;
     uint32_t arg_order;
-    const float* dl;  // adjoint: d loss / d image, RGBA per pixel
-    double* grad;     // adjoint: 6 per material (d/d cd rgb, d/d ce rgb), accumulated
+    uint32_t grad_slots;  // adjoint: materials covered by the LDS partials
+    double* grad;         // adjoint: 6 per material slot (d/d cd rgb, d/d ce rgb), accumulated
 };
 
-__device__ inline void write_pixel(const pt_args& A, uint32_t px, float4 c)
+__device__ inline void write_pixel(const scene_desc& S, uint32_t px, float4 c)
 {
     // sample average, pixel_access::blend into accum, accum -> color
     // (detail/sched_common.h:466-509, detail/pixel_access.h:1040-1131)
-    const float spp = (float)A.sp.spp;
+    const float spp = (float)S.sp.spp;
     c.x = c.x / spp; c.y = c.y / spp; c.z = c.z / spp; c.w = c.w / spp;
-    float4* acc = reinterpret_cast<float4*>(A.accum) + px;
+    float4* acc = reinterpret_cast<float4*>(S.accum) + px;
     const float4 old = *acc;
-    const float sf = A.sp.sfactor, df = A.sp.dfactor;
+    const float sf = S.sp.sfactor, df = S.sp.dfactor;
     const float4 r = make_float4(c.x * sf + old.x * df, c.y * sf + old.y * df, c.z * sf + old.z * df,
                                  c.w * sf + old.w * df);
     *acc = r;
-    reinterpret_cast<float4*>(A.color)[px] = r;
+    reinterpret_cast<float4*>(S.color)[px] = r;
 }
 
 enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2 };
@@ -155,7 +171,11 @@ constexpr int ADJ_MAX_VERTICES = 16;
 //   d I_c / d cd_{m,c} = I_c * (vertices with material m) / cd_{m,c}
 //   d I_c / d ce_{e,c} = T_c * ls_e.
 // Per-block FP32 partials in LDS; FP64 only in the final reduction.
-template <int MODE>
+//
+// BATCH: many scenes in one persistent launch.  The global pixel index runs
+// over all scenes back to back; a lane finds its scene by binary search over
+// the pixel prefix and reads that scene's descriptor.
+template <int MODE, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = MODE == PT_COUNT;
@@ -163,14 +183,15 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
-    const uint32_t w = A.sp.width, h = A.sp.height, npix = w * h;
+    const uint32_t npix = BATCH ? A.total_pixels : A.single.sp.width * A.single.sp.height;
     const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    const f3 amb = mk3(A.kp.ambient[0], A.kp.ambient[1], A.kp.ambient[2]);
-    const float eps = A.kp.epsilon;
+    const uint32_t frame_add = A.frame_offset ? *A.frame_offset : 0u;
     if (ADJ) {
-        for (uint32_t i = threadIdx.x; i < 6 * A.num_mats; i += BLOCK) s_grad[i] = 0.0f;
+        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLOCK) s_grad[i] = 0.0f;
         __syncthreads();
     }
+    uint32_t sid = 0;  // scene of this lane's pixel (BATCH)
+#define SCN (BATCH ? A.scenes[sid] : A.single)
 
     int32_t px = -1;
     bool done = false;
@@ -199,20 +220,33 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
             if (need) {
                 const uint32_t my = base + (uint32_t)lane_mask_lt_count(needm);
                 if (my < npix) {
-                    px = (int32_t)my;
-                    const int x = (int)(my % w), y = (int)(my / w);
-                    rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)A.sp.frame_id));
+                    uint32_t lpx = my;
+                    if (BATCH) {
+                        uint32_t lo = 0, hi = A.num_scenes;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (A.prefix[mid] <= my) lo = mid;
+                            else hi = mid;
+                        }
+                        sid = lo;
+                        lpx = my - A.prefix[lo];
+                    }
+                    const scene_desc& S = SCN;
+                    px = (int32_t)lpx;
+                    const uint32_t w = S.sp.width, h = S.sp.height;
+                    const int x = (int)(lpx % w), y = (int)(lpx / w);
+                    rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
                     s = 0;
                     sum = make_float4(0, 0, 0, 0);
-                    primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                    primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
                     tp = mk3(1, 1, 1);
                     I = mk3(0, 0, 0);
                     bounce = 0;
                     rhit = false;
                     if (ADJ) {
                         // the image is accum = mean(samples) * sfactor + ...
-                        const float4 dl = reinterpret_cast<const float4*>(A.dl)[my];
-                        const float k = A.sp.sfactor / (float)A.sp.spp;
+                        const float4 dl = reinterpret_cast<const float4*>(S.dl)[lpx];
+                        const float k = S.sp.sfactor / (float)S.sp.spp;
                         wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
                         nmid = 0;
                     }
@@ -224,7 +258,8 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
         if (__ballot(!done) == 0) break;
         if (done) continue;
 
-        const hit_t hr = traverse<false, COUNT, LDS_STACK, BLOCK>(A.b, o, d, 0.0f, FLT_MAX, st, &tc);
+        const scene_desc& S = SCN;
+        const hit_t hr = traverse<false, COUNT, LDS_STACK, BLOCK>(S.b, o, d, 0.0f, FLT_MAX, st, &tc);
         if (COUNT) {
             n_pairs += tc.pairs;
             n_tris += tc.tris;
@@ -237,27 +272,28 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
         bool finished = false;
         if (hr.tri < 0) {
             // escaped: ambient light times throughput (pathtracing.inl:69-77)
-            const f3 e = mul3(amb, tp);
+            const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), tp);
             I = add3(I, e);
             finished = true;
             if (ADJ) {
                 for (uint32_t j = 0; j < nmid; ++j) {
-                    const vr_material* mj = A.mats + mids[j];
-                    atomicAdd(&s_grad[6 * mids[j] + 0], wgt.x * e.x / mj->cd[0]);
-                    atomicAdd(&s_grad[6 * mids[j] + 1], wgt.y * e.y / mj->cd[1]);
-                    atomicAdd(&s_grad[6 * mids[j] + 2], wgt.z * e.z / mj->cd[2]);
+                    const vr_material* mj = S.mats + mids[j];
+                    const uint32_t slot = 6 * (S.grad_base + mids[j]);
+                    atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
+                    atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
+                    atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
                 }
             }
         } else {
             if (bounce == 0) rhit = true;
             const f3 view = neg3(d);
             const f3 pos = add3(o, muls3(d, hr.t));
-            const float4* tq = reinterpret_cast<const float4*>(A.b.tris + hr.tri);
+            const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
             const float4 tb = tq[1], tcq = tq[2];
             const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
             uint32_t mid = __float_as_uint(tb.w);
-            mid = mid < A.num_mats ? mid : A.num_mats - 1u;
-            const vr_material* m = A.mats + mid;
+            mid = mid < S.num_mats ? mid : S.num_mats - 1u;
+            const vr_material* m = S.mats + mid;
             f3 refl = mk3(0.0f, 0.0f, 0.0f);
             float pdf;
             f3 src;
@@ -291,14 +327,16 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
                 active = false;
                 if (ADJ) {
                     const float ls = m->ls;
-                    atomicAdd(&s_grad[6 * mid + 3], wgt.x * tp.x * ls);
-                    atomicAdd(&s_grad[6 * mid + 4], wgt.y * tp.y * ls);
-                    atomicAdd(&s_grad[6 * mid + 5], wgt.z * tp.z * ls);
+                    const uint32_t es = 6 * (S.grad_base + mid);
+                    atomicAdd(&s_grad[es + 3], wgt.x * tp.x * ls);
+                    atomicAdd(&s_grad[es + 4], wgt.y * tp.y * ls);
+                    atomicAdd(&s_grad[es + 5], wgt.z * tp.z * ls);
                     for (uint32_t j = 0; j < nmid; ++j) {
-                        const vr_material* mj = A.mats + mids[j];
-                        atomicAdd(&s_grad[6 * mids[j] + 0], wgt.x * e.x / mj->cd[0]);
-                        atomicAdd(&s_grad[6 * mids[j] + 1], wgt.y * e.y / mj->cd[1]);
-                        atomicAdd(&s_grad[6 * mids[j] + 2], wgt.z * e.z / mj->cd[2]);
+                        const vr_material* mj = S.mats + mids[j];
+                        const uint32_t slot = 6 * (S.grad_base + mids[j]);
+                        atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
+                        atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
+                        atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
                     }
                 }
             } else if (ADJ && !zero_pdf && nmid < ADJ_MAX_VERTICES) {
@@ -321,10 +359,10 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
             if (!active) {
                 finished = true;
             } else {
-                o = add3(pos, muls3(refl, eps));
+                o = add3(pos, muls3(refl, S.kp.epsilon));
                 d = refl;
                 ++bounce;
-                if (bounce >= A.kp.num_bounces) finished = true;
+                if (bounce >= S.kp.num_bounces) finished = true;
             }
         }
         if (finished) {
@@ -332,17 +370,18 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
                 if (rhit) {
                     sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
                 } else {
-                    sum.x += A.kp.bg[0]; sum.y += A.kp.bg[1]; sum.z += A.kp.bg[2]; sum.w += 1.0f;
+                    sum.x += S.kp.bg[0]; sum.y += S.kp.bg[1]; sum.z += S.kp.bg[2]; sum.w += 1.0f;
                 }
             }
             ++s;
             if (ADJ) nmid = 0;
-            if (s >= A.sp.spp) {
-                if (!ADJ) write_pixel(A, (uint32_t)px, sum);
+            if (s >= S.sp.spp) {
+                if (!ADJ) write_pixel(S, (uint32_t)px, sum);
                 px = -1;
             } else {
+                const uint32_t w = S.sp.width, h = S.sp.height;
                 const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
-                primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
                 tp = mk3(1, 1, 1);
                 I = mk3(0, 0, 0);
                 bounce = 0;
@@ -372,11 +411,12 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
     if (ADJ) {
         // FP32 block partials -> FP64 gradient
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < 6 * A.num_mats; i += BLOCK) {
+        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLOCK) {
             const float v = s_grad[i];
             if (v != 0.0f) atomicAdd(&A.grad[i], (double)v);
         }
     }
+#undef SCN
 }
 
 struct ao_args {
@@ -444,57 +484,80 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
     reinterpret_cast<float4*>(A.color)[px] = r;
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
-
-template <int MODE>
+template <int MODE, bool BATCH>
 int launch_pt(const pt_args& A, size_t dyn_lds, hipStream_t st)
 {
-    static int grid_cache[3] = {0, 0, 0};
-    static size_t lds_cache[3] = {0, 0, 0};
-    if (grid_cache[MODE] == 0 || lds_cache[MODE] != dyn_lds) {
+    constexpr int slot = MODE * 2 + (BATCH ? 1 : 0);
+    static int grid_cache[6] = {0, 0, 0, 0, 0, 0};
+    static size_t lds_cache[6] = {0, 0, 0, 0, 0, 0};
+    if (grid_cache[slot] == 0 || lds_cache[slot] != dyn_lds) {
         int dev = 0, per_cu = 0, cus = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
-        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE>, BLOCK, dyn_lds));
+        VR_HIP_CHECK(
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE, BATCH>, BLOCK, dyn_lds));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid_cache[MODE] = std::max(1, per_cu) * std::max(1, cus);
-        lds_cache[MODE] = dyn_lds;
+        grid_cache[slot] = std::max(1, per_cu) * std::max(1, cus);
+        lds_cache[slot] = dyn_lds;
     }
-    hipLaunchKernelGGL(k_pathtrace<MODE>, dim3(grid_cache[MODE]), dim3(BLOCK), dyn_lds, st, A);
+    hipLaunchKernelGGL((k_pathtrace<MODE, BATCH>), dim3(grid_cache[slot]), dim3(BLOCK), dyn_lds, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
 
+__global__ void k_frame_begin(uint32_t* work, uint32_t* frame, int advance)
+{
+    *work = 0u;
+    if (frame && advance) *frame += 1u;
+}
+
+pt_args single_args(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                    const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color, const float* dl,
+                    unsigned long long* ray_counts, uint32_t* work, double* grad, uint32_t arg_order)
+{
+    pt_args A{};
+    A.single.b = bs.view();
+    A.single.mats = mats;
+    A.single.num_mats = (uint32_t)num_mats;
+    A.single.grad_base = 0;
+    A.single.kp = kp;
+    A.single.cam = cam;
+    A.single.sp = sp;
+    A.single.accum = accum;
+    A.single.color = color;
+    A.single.dl = dl;
+    A.scenes = nullptr;
+    A.prefix = nullptr;
+    A.num_scenes = 1;
+    A.total_pixels = sp.width * sp.height;
+    A.frame_offset = nullptr;
+    A.ray_counts = ray_counts;
+    A.work = work;
+    A.arg_order = arg_order;
+    A.grad_slots = grad ? (uint32_t)num_mats : 0u;
+    A.grad = grad;
+    return A;
+}
+
 }  // namespace
 
-// argument evaluation order of the reference host build (see pathtrace docs)
+// argument evaluation order of the reference host build: GCC evaluates
+// f(gen.next(), gen.next()) right to left (1)
 uint32_t g_arg_order = 1;
+
+// largest material table the adjoint keeps in LDS partials
+constexpr size_t ADJ_MAX_MATERIALS = 2048;
 
 int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
               const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
               unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st)
 {
     if ((size_t)sp.width * sp.height == 0) return VR_OK;
-    pt_args A;
-    A.b = bs.view();
-    A.mats = mats;
-    A.num_mats = (uint32_t)num_mats;
-    A.kp = kp;
-    A.cam = cam;
-    A.sp = sp;
-    A.accum = accum;
-    A.color = color;
-    A.ray_counts = ray_counts;
-    A.work = work;
-    A.arg_order = g_arg_order;
-    A.dl = nullptr;
-    A.grad = nullptr;
     VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, work);
-    return counted ? launch_pt<PT_COUNT>(A, 0, st) : launch_pt<PT_RENDER>(A, 0, st);
+    const pt_args A =
+        single_args(bs, mats, num_mats, kp, cam, sp, accum, color, nullptr, ray_counts, work, nullptr, g_arg_order);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
+    return counted ? launch_pt<PT_COUNT, false>(A, 0, st) : launch_pt<PT_RENDER, false>(A, 0, st);
 }
-
-// largest material table the adjoint keeps in LDS partials
-constexpr size_t ADJ_MAX_MATERIALS = 2048;
 
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
@@ -505,22 +568,59 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
     VR_CHECK(kp.num_bounces <= (uint32_t)ADJ_MAX_VERTICES, VR_ERR_UNSUPPORTED,
              "render_pt_backward: more than 16 bounces");
     VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_backward: BVH deeper than the stack");
-    pt_args A;
-    A.b = bs.view();
-    A.mats = mats;
-    A.num_mats = (uint32_t)num_mats;
-    A.kp = kp;
-    A.cam = cam;
-    A.sp = sp;
-    A.accum = nullptr;
-    A.color = nullptr;
+    const pt_args A =
+        single_args(bs, mats, num_mats, kp, cam, sp, nullptr, nullptr, dl, ray_counts, work, grad, g_arg_order);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
+    return launch_pt<PT_ADJOINT, false>(A, 6 * num_mats * sizeof(float), st);
+}
+
+// Many scenes in one persistent launch.  d_scenes / d_prefix are device
+// arrays prepared by the batch handle; d_frame (optional) is a device frame
+// counter incremented before every frame, so a captured graph renders
+// successive frames.  Only kernel launches are enqueued (graph capturable).
+int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num_scenes, uint32_t total_pixels,
+                    uint32_t* d_frame, bool advance_frame, unsigned long long* ray_counts, uint32_t* work,
+                    uint32_t grad_slots, double* grad, hipStream_t st)
+{
+    if (total_pixels == 0) return VR_OK;
+    pt_args A{};
+    A.scenes = static_cast<const scene_desc*>(d_scenes);
+    A.prefix = d_prefix;
+    A.num_scenes = num_scenes;
+    A.total_pixels = total_pixels;
+    A.frame_offset = d_frame;
     A.ray_counts = ray_counts;
     A.work = work;
     A.arg_order = g_arg_order;
-    A.dl = dl;
+    A.grad_slots = grad ? grad_slots : 0u;
     A.grad = grad;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, work);
-    return launch_pt<PT_ADJOINT>(A, 6 * num_mats * sizeof(float), st);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, d_frame, advance_frame ? 1 : 0);
+    if (grad) {
+        VR_CHECK(grad_slots <= ADJ_MAX_MATERIALS, VR_ERR_UNSUPPORTED, "batch adjoint: more than 2048 materials");
+        return launch_pt<PT_ADJOINT, true>(A, 6 * (size_t)grad_slots * sizeof(float), st);
+    }
+    return launch_pt<PT_RENDER, true>(A, 0, st);
+}
+
+size_t scene_desc_size() { return sizeof(scene_desc); }
+
+// fill one scene descriptor (host side) for the batch handle
+void make_scene_desc(void* out, const bvh_state& bs, const vr_material* mats, uint32_t num_mats, uint32_t grad_base,
+                     const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp, float* accum,
+                     float* color, const float* dl)
+{
+    scene_desc d{};
+    d.b = bs.view();
+    d.mats = mats;
+    d.num_mats = num_mats;
+    d.grad_base = grad_base;
+    d.kp = kp;
+    d.cam = cam;
+    d.sp = sp;
+    d.accum = accum;
+    d.color = color;
+    d.dl = dl;
+    std::memcpy(out, &d, sizeof(d));
 }
 
 // L2 image loss sum_px sum_rgb (I - T)^2 (FP64 reduction) and its image
