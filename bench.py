@@ -53,6 +53,11 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--out", type=str, default="")
+    ap.add_argument("--config", type=str, default="c1", choices=["c1", "c2", "c3", "c4"],
+                    help="c1 (default): the headline workload; c2 batched scenes; c3 adjoint vs finite "
+                         "differences; c4 dynamic geometry (refit vs rebuild) with batched adjoint")
+    ap.add_argument("--scenes", type=int, default=0, help="c2/c4: number of scenes (default 64 / 32)")
+    ap.add_argument("--fd-params", type=int, default=0, help="c3: finite-difference parameters (0 = all)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -63,6 +68,15 @@ def main():
 
     import visionaray_amd as V
     from visionaray_amd import scenes as S
+
+    if args.config != "c1":
+        from visionaray_amd import workloads
+        line = {"c2": workloads.bench_batched, "c3": workloads.bench_adjoint, "c4": workloads.bench_dynamic}[
+            args.config](args, V, S, log)
+        print(json.dumps(line), flush=True)
+        if args.out:
+            Path(args.out).write_text(json.dumps(line) + "\n")
+        return
 
     t0 = time.time()
     tris = S.config1_scene(cells=args.cells, spheres=args.spheres)

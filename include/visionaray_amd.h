@@ -241,6 +241,28 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
                           const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                           const float* d_dl_dimage, double* d_grad, uint64_t* d_ray_counts, void* stream);
 
+/* ---- launch shape and autotuning (no reference counterpart: cuda_sched
+ * takes a fixed 16x16 block, detail/cuda_sched.h:43) -------------------- */
+/* Launch shape of the path-tracing frame kernel: threads per block (128 or
+ * 256), traversal-stack entries per lane kept in LDS (8, 16 or 32; deeper
+ * entries spill to private memory), and the pixel-queue tile (pixels are
+ * handed out in tile x tile squares, 1 = scanlines; 1..64). */
+typedef struct vr_launch_shape {
+    uint32_t block_size;
+    uint32_t stack_lds;
+    uint32_t tile;
+} vr_launch_shape;
+
+int vr_set_launch_shape(const vr_launch_shape* shape);
+int vr_get_launch_shape(vr_launch_shape* shape);
+/* Time every compiled shape with tiles {1, 8, 16} on this frame (arguments
+ * as vr_render_pt; one warm-up and `reps` timed frames each), keep the
+ * fastest as the current shape and return it with its mean frame time.
+ * Synchronizes the stream. */
+int vr_autotune_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                   const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color, int reps,
+                   vr_launch_shape* best, float* best_ms, void* stream);
+
 /* ---- batched scenes (no reference counterpart: the reference launches one
  * cuda_sched::frame per scene, detail/cuda_sched.inl:226-246) ------------- */
 /* One scene of a batch: what one reference frame() call renders. */

@@ -230,3 +230,44 @@ def test_adjoint_matches_finite_differences_without_roulette(V):
     cos = float(a @ f / np.linalg.norm(a) / np.linalg.norm(f))
     print(f"[adjoint] vs central FD (depth 2): cosine {cos:.7f}")
     assert cos > 0.9999
+
+
+@pytest.mark.parametrize("shape", [(128, 8, 1), (256, 32, 8), (128, 16, 16), (256, 16, 1)])
+def test_launch_shapes_give_identical_frames(V, shape):
+    """Block size, LDS stack depth and pixel tile change only the schedule:
+    the image is bit-identical for every compiled shape."""
+    tris = S.config0_scene(level=3)
+    mats = S.lambert_materials(1, seed=3)
+    tris["geom_id"] = 0
+    bvh = V.lbvh_builder().build(tris)
+    w, h = 37, 23
+    cam = V.pinhole_camera(*S.config0_camera(w, h))
+    kp = V.make_kernel_params(num_bounces=5, epsilon=1e-4, ambient_color=(0.6, 0.7, 0.9, 1.0))
+    V.set_launch_shape(256, 16, 1)
+    want, _, _ = gpu_frame(V, bvh, mats, kp, cam, w, h, 3)
+    V.set_launch_shape(*shape)
+    try:
+        got, (npri, _), _ = gpu_frame(V, bvh, mats, kp, cam, w, h, 3)
+    finally:
+        V.set_launch_shape(256, 16, 1)
+    assert np.array_equal(got, want)
+    assert npri == w * h * 3
+
+
+def test_autotune_picks_a_valid_shape(V):
+    tris = S.config0_scene(level=3)
+    mats = S.lambert_materials(1, seed=3)
+    tris["geom_id"] = 0
+    bvh = V.lbvh_builder().build(tris)
+    cam = V.pinhole_camera(*S.config0_camera(64, 64))
+    kp = V.make_kernel_params(num_bounces=3, epsilon=1e-4)
+    rt = V.render_target(64, 64)
+    kern = V.pathtracing.kernel(bvh, V.DeviceArray.from_host(mats), kp)
+    sp = V.make_sched_params(V.jittered_blend_type(spp=1), cam, rt)
+    try:
+        shape, ms = V.autotune(kern, sp, reps=1)
+        assert shape["block_size"] in (128, 256) and shape["stack_lds"] in (8, 16, 32) and shape["tile"] in (1, 8, 16)
+        assert 0 < ms < 1e4
+        assert V.get_launch_shape() == shape
+    finally:
+        V.set_launch_shape(256, 16, 1)

@@ -346,6 +346,35 @@ class ao_kernel:
     bg: tuple = (0.0, 0.0, 0.0)
 
 
+LAUNCH_SHAPE_DTYPE = np.dtype([("block_size", "<u4"), ("stack_lds", "<u4"), ("tile", "<u4")])
+
+
+def set_launch_shape(block_size: int = 256, stack_lds: int = 16, tile: int = 1):
+    """Launch shape of the path-tracing frame kernel (vr_set_launch_shape)."""
+    a = np.array([(block_size, stack_lds, tile)], LAUNCH_SHAPE_DTYPE)
+    check(lib().vr_set_launch_shape(ptr(a)))
+
+
+def get_launch_shape() -> dict:
+    a = np.zeros(1, LAUNCH_SHAPE_DTYPE)
+    check(lib().vr_get_launch_shape(ptr(a)))
+    return {k: int(a[k][0]) for k in a.dtype.names}
+
+
+def autotune(kernel, sp, reps: int = 2, stream=None) -> tuple[dict, float]:
+    """vr_autotune_pt: time every launch shape on this frame and keep the
+    fastest; returns (shape, mean frame ms)."""
+    s = np.zeros(1, SCHED_PARAMS_DTYPE)
+    s["width"], s["height"], s["spp"] = sp.rt.width, sp.rt.height, sp.sample_params.spp
+    s["sfactor"], s["dfactor"] = sp.sample_params.sfactor, sp.sample_params.dfactor
+    best = np.zeros(1, LAUNCH_SHAPE_DTYPE)
+    ms = C.c_float()
+    check(lib().vr_autotune_pt(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count, ptr(kernel.params),
+                               ptr(sp.cam), ptr(s), sp.rt.accum.ptr, sp.rt.color.ptr, int(reps), ptr(best),
+                               C.byref(ms), _stream(stream)))
+    return {k: int(best[k][0]) for k in best.dtype.names}, float(ms.value)
+
+
 class batch:
     """Many scenes rendered by one persistent launch (vr_batch_*).  Each entry
     is (pathtracing.kernel, sched_params[, d loss / d image DeviceArray]); the

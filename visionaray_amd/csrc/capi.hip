@@ -30,7 +30,14 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
 int image_l2_loss(const float* img, const float* tgt, size_t npix, float* dl, double* loss, hipStream_t st);
 int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num_scenes, uint32_t total_pixels,
                     uint32_t* d_frame, bool advance_frame, unsigned long long* ray_counts, uint32_t* work,
-                    uint32_t grad_slots, double* grad, hipStream_t st);
+                    uint32_t grad_slots, double* grad, const vr_launch_shape& shape, hipStream_t st);
+int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                     const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
+                     unsigned long long* ray_counts, uint32_t* work, bool counted, const vr_launch_shape& shape,
+                     hipStream_t st);
+uint32_t padded_pixel_count(uint32_t w, uint32_t h, uint32_t tile);
+extern vr_launch_shape g_shape;
+bool valid_shape(const vr_launch_shape& s);
 size_t scene_desc_size();
 void make_scene_desc(void* out, const bvh_state& bs, const vr_material* mats, uint32_t num_mats, uint32_t grad_base,
                      const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp, float* accum,
@@ -82,6 +89,7 @@ struct vr_batch_t {
     uint32_t total_pixels = 0;
     uint32_t total_materials = 0;
     uint32_t max_depth = 0;
+    vr_launch_shape shape{256, 16, 1};  // fixed at creation (the pixel prefix depends on the tile)
     void* d_scenes = nullptr;      // scene descriptors
     uint32_t* d_prefix = nullptr;  // num_scenes + 1
     uint32_t* d_frame = nullptr;   // frame counter (starts at -1, advanced before each frame)
@@ -578,7 +586,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
         make_scene_desc(&descs[i * dsz], sc.bvh->s, sc.d_materials, (uint32_t)sc.num_materials, (uint32_t)mats,
                         sc.kp, sc.cam, sc.sp, sc.d_accum, sc.d_color, sc.d_dl_dimage);
         prefix[i] = (uint32_t)total;
-        total += (uint64_t)sc.sp.width * sc.sp.height;
+        total += padded_pixel_count(sc.sp.width, sc.sp.height, g_shape.tile);
         mats += sc.num_materials;
         depth = std::max(depth, sc.bvh->s.depth);
     }
@@ -591,6 +599,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
     b->total_pixels = (uint32_t)total;
     b->total_materials = (uint32_t)std::min<uint64_t>(mats, 0xFFFFFFFFu);
     b->max_depth = depth;
+    b->shape = g_shape;
     const uint32_t start = 0xFFFFFFFFu;
     if (hipMalloc(&b->d_scenes, descs.size()) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&b->d_prefix), prefix.size() * 4) != hipSuccess ||
@@ -612,7 +621,7 @@ int vr_batch_render(vr_batch batch, uint64_t* d_ray_counts, void* stream)
     VR_CHECK(batch, VR_ERR_INVALID_ARGUMENT, "vr_batch_render: null handle");
     return render_pt_batch(batch->d_scenes, batch->d_prefix, batch->num_scenes, batch->total_pixels, batch->d_frame,
                            true, reinterpret_cast<unsigned long long*>(d_ray_counts), batch->d_work, 0, nullptr,
-                           (hipStream_t)stream);
+                           batch->shape, (hipStream_t)stream);
 }
 
 int vr_batch_backward(vr_batch batch, double* d_grad, uint64_t* d_ray_counts, void* stream)
@@ -620,7 +629,72 @@ int vr_batch_backward(vr_batch batch, double* d_grad, uint64_t* d_ray_counts, vo
     VR_CHECK(batch && d_grad, VR_ERR_INVALID_ARGUMENT, "vr_batch_backward: null argument");
     return render_pt_batch(batch->d_scenes, batch->d_prefix, batch->num_scenes, batch->total_pixels, batch->d_frame,
                            false, reinterpret_cast<unsigned long long*>(d_ray_counts), batch->d_work,
-                           batch->total_materials, d_grad, (hipStream_t)stream);
+                           batch->total_materials, d_grad, batch->shape, (hipStream_t)stream);
+}
+
+int vr_set_launch_shape(const vr_launch_shape* shape)
+{
+    VR_CHECK(shape, VR_ERR_INVALID_ARGUMENT, "vr_set_launch_shape: null argument");
+    VR_CHECK(valid_shape(*shape), VR_ERR_INVALID_ARGUMENT,
+             "vr_set_launch_shape: block_size 128|256, stack_lds 8|16|32, tile 1..64");
+    g_shape = *shape;
+    return VR_OK;
+}
+
+int vr_get_launch_shape(vr_launch_shape* shape)
+{
+    VR_CHECK(shape, VR_ERR_INVALID_ARGUMENT, "vr_get_launch_shape: null argument");
+    *shape = g_shape;
+    return VR_OK;
+}
+
+int vr_autotune_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                   const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color, int reps,
+                   vr_launch_shape* best, float* best_ms, void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_accum && d_color && best, VR_ERR_INVALID_ARGUMENT,
+             "vr_autotune_pt: null argument");
+    VR_CHECK(bvh->s.num_prims == 0 || (d_materials && num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_autotune_pt: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_autotune_pt: spp must be >= 1");
+    int rc = ensure_device();
+    if (rc) return rc;
+    reps = std::max(1, reps);
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    VR_HIP_CHECK(hipEventCreate(&e0));
+    VR_HIP_CHECK(hipEventCreate(&e1));
+    float best_t = 1e30f;
+    vr_launch_shape best_s = g_shape;
+    const uint32_t blocks[] = {128, 256}, stacks[] = {8, 16, 32}, tiles[] = {1, 8, 16};
+    for (uint32_t b : blocks)
+        for (uint32_t d : stacks)
+            for (uint32_t t : tiles) {
+                const vr_launch_shape shp{b, d, t};
+                rc = render_pt_shaped(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color, nullptr,
+                                      dev_state()->work, false, shp, st);
+                if (rc) break;
+                VR_HIP_CHECK(hipEventRecord(e0, st));
+                for (int r = 0; r < reps && rc == 0; ++r)
+                    rc = render_pt_shaped(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
+                                          nullptr, dev_state()->work, false, shp, st);
+                VR_HIP_CHECK(hipEventRecord(e1, st));
+                VR_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                VR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                ms /= (float)reps;
+                if (rc == 0 && ms < best_t) {
+                    best_t = ms;
+                    best_s = shp;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    g_shape = best_s;
+    *best = best_s;
+    if (best_ms) *best_ms = best_t;
+    return VR_OK;
 }
 
 int vr_batch_capture_graph(vr_batch batch, uint64_t* d_ray_counts, void* stream)

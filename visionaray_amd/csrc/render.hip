@@ -136,6 +136,7 @@ struct pt_args {
     unsigned long long* ray_counts;
     uint32_t* work;
     uint32_t arg_order;
+    uint32_t tile;        // pixel queue order (tile x tile squares; <= 1: scanlines)
     uint32_t grad_slots;  // adjoint: materials covered by the LDS partials
     double* grad;         // adjoint: 6 per material slot (d/d cd rgb, d/d ce rgb), accumulated
 };
@@ -153,6 +154,28 @@ __device__ inline void write_pixel(const scene_desc& S, uint32_t px, float4 c)
                                  c.w * sf + old.w * df);
     *acc = r;
     reinterpret_cast<float4*>(S.color)[px] = r;
+}
+
+// Pixel order of the work queue: tile x tile squares row by row (tile <= 1:
+// scanlines).  The queue runs over whole tiles; indices in the padding of a
+// partial tile map to no pixel.
+__host__ __device__ inline uint32_t padded_pixels(uint32_t w, uint32_t h, uint32_t tile)
+{
+    if (tile <= 1) return w * h;
+    return ((w + tile - 1) / tile) * ((h + tile - 1) / tile) * tile * tile;
+}
+__device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t tile, int& x, int& y)
+{
+    if (tile <= 1) {
+        x = (int)(idx % w);
+        y = (int)(idx / w);
+        return true;
+    }
+    const uint32_t tiles_x = (w + tile - 1) / tile, tt = tile * tile;
+    const uint32_t t = idx / tt, o = idx % tt;
+    x = (int)((t % tiles_x) * tile + o % tile);
+    y = (int)((t / tiles_x) * tile + o / tile);
+    return (uint32_t)x < w && (uint32_t)y < h;
 }
 
 enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2 };
@@ -175,19 +198,22 @@ constexpr int ADJ_MAX_VERTICES = 16;
 // BATCH: many scenes in one persistent launch.  The global pixel index runs
 // over all scenes back to back; a lane finds its scene by binary search over
 // the pixel prefix and reads that scene's descriptor.
-template <int MODE, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
+//
+// BLK (threads per block) and LDSD (LDS stack entries per lane) are the
+// autotuned launch shape; A.tile orders pixels in tile x tile squares.
+template <int MODE, bool BATCH, int BLK, int LDSD>
+__global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = MODE == PT_COUNT;
     constexpr bool ADJ = MODE == PT_ADJOINT;
-    __shared__ uint32_t stack[LDS_STACK * BLOCK];
+    __shared__ uint32_t stack[LDSD * BLK];
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
-    const uint32_t npix = BATCH ? A.total_pixels : A.single.sp.width * A.single.sp.height;
+    const uint32_t npix = BATCH ? A.total_pixels : padded_pixels(A.single.sp.width, A.single.sp.height, A.tile);
     const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     const uint32_t frame_add = A.frame_offset ? *A.frame_offset : 0u;
     if (ADJ) {
-        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLOCK) s_grad[i] = 0.0f;
+        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLK) s_grad[i] = 0.0f;
         __syncthreads();
     }
     uint32_t sid = 0;  // scene of this lane's pixel (BATCH)
@@ -232,23 +258,27 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
                         lpx = my - A.prefix[lo];
                     }
                     const scene_desc& S = SCN;
-                    px = (int32_t)lpx;
                     const uint32_t w = S.sp.width, h = S.sp.height;
-                    const int x = (int)(lpx % w), y = (int)(lpx / w);
-                    rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
-                    s = 0;
-                    sum = make_float4(0, 0, 0, 0);
-                    primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
-                    tp = mk3(1, 1, 1);
-                    I = mk3(0, 0, 0);
-                    bounce = 0;
-                    rhit = false;
-                    if (ADJ) {
-                        // the image is accum = mean(samples) * sfactor + ...
-                        const float4 dl = reinterpret_cast<const float4*>(S.dl)[lpx];
-                        const float k = S.sp.sfactor / (float)S.sp.spp;
-                        wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
-                        nmid = 0;
+                    int x, y;
+                    // pixels of the padding of a partial tile are skipped (px stays -1)
+                    if (tile_pixel(lpx, w, h, A.tile, x, y)) {
+                        lpx = (uint32_t)y * w + (uint32_t)x;
+                        px = (int32_t)lpx;
+                        rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
+                        s = 0;
+                        sum = make_float4(0, 0, 0, 0);
+                        primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                        tp = mk3(1, 1, 1);
+                        I = mk3(0, 0, 0);
+                        bounce = 0;
+                        rhit = false;
+                        if (ADJ) {
+                            // the image is accum = mean(samples) * sfactor + ...
+                            const float4 dl = reinterpret_cast<const float4*>(S.dl)[lpx];
+                            const float k = S.sp.sfactor / (float)S.sp.spp;
+                            wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
+                            nmid = 0;
+                        }
                     }
                 } else {
                     done = true;
@@ -256,10 +286,10 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
             }
         }
         if (__ballot(!done) == 0) break;
-        if (done) continue;
+        if (done || px < 0) continue;
 
         const scene_desc& S = SCN;
-        const hit_t hr = traverse<false, COUNT, LDS_STACK, BLOCK>(S.b, o, d, 0.0f, FLT_MAX, st, &tc);
+        const hit_t hr = traverse<false, COUNT, LDSD, BLK>(S.b, o, d, 0.0f, FLT_MAX, st, &tc);
         if (COUNT) {
             n_pairs += tc.pairs;
             n_tris += tc.tris;
@@ -411,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) void k_pathtrace(pt_args A)
     if (ADJ) {
         // FP32 block partials -> FP64 gradient
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLOCK) {
+        for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLK) {
             const float v = s_grad[i];
             if (v != 0.0f) atomicAdd(&A.grad[i], (double)v);
         }
@@ -484,24 +514,41 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
     reinterpret_cast<float4*>(A.color)[px] = r;
 }
 
-template <int MODE, bool BATCH>
-int launch_pt(const pt_args& A, size_t dyn_lds, hipStream_t st)
+// One compiled launch shape: a persistent grid of exactly the blocks the
+// device holds at once (occupancy x CUs), computed once per shape.
+template <int MODE, bool BATCH, int BLK, int LDSD>
+int launch_variant(const pt_args& A, size_t dyn_lds, hipStream_t st)
 {
-    constexpr int slot = MODE * 2 + (BATCH ? 1 : 0);
-    static int grid_cache[6] = {0, 0, 0, 0, 0, 0};
-    static size_t lds_cache[6] = {0, 0, 0, 0, 0, 0};
-    if (grid_cache[slot] == 0 || lds_cache[slot] != dyn_lds) {
+    static int grid = 0;
+    static size_t grid_lds = (size_t)-1;
+    if (grid == 0 || grid_lds != dyn_lds) {
         int dev = 0, per_cu = 0, cus = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
-        VR_HIP_CHECK(
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE, BATCH>, BLOCK, dyn_lds));
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE, BATCH, BLK, LDSD>, BLK,
+                                                                 dyn_lds));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid_cache[slot] = std::max(1, per_cu) * std::max(1, cus);
-        lds_cache[slot] = dyn_lds;
+        grid = std::max(1, per_cu) * std::max(1, cus);
+        grid_lds = dyn_lds;
     }
-    hipLaunchKernelGGL((k_pathtrace<MODE, BATCH>), dim3(grid_cache[slot]), dim3(BLOCK), dyn_lds, st, A);
+    hipLaunchKernelGGL((k_pathtrace<MODE, BATCH, BLK, LDSD>), dim3(grid), dim3(BLK), dyn_lds, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
+}
+
+// The frame kernel is compiled for 2 block sizes x 3 LDS stack depths; the
+// counting and adjoint variants use the default shape.
+template <int MODE, bool BATCH>
+int launch_pt(const pt_args& A, size_t dyn_lds, const vr_launch_shape& sh, hipStream_t st)
+{
+    if (MODE != PT_RENDER) return launch_variant<MODE, BATCH, 256, 16>(A, dyn_lds, st);
+    switch (sh.block_size * 100 + sh.stack_lds) {
+    case 12808: return launch_variant<MODE, BATCH, 128, 8>(A, dyn_lds, st);
+    case 12816: return launch_variant<MODE, BATCH, 128, 16>(A, dyn_lds, st);
+    case 12832: return launch_variant<MODE, BATCH, 128, 32>(A, dyn_lds, st);
+    case 25608: return launch_variant<MODE, BATCH, 256, 8>(A, dyn_lds, st);
+    case 25632: return launch_variant<MODE, BATCH, 256, 32>(A, dyn_lds, st);
+    default: return launch_variant<MODE, BATCH, 256, 16>(A, dyn_lds, st);
+    }
 }
 
 __global__ void k_frame_begin(uint32_t* work, uint32_t* frame, int advance)
@@ -544,8 +591,23 @@ pt_args single_args(const bvh_state& bs, const vr_material* mats, size_t num_mat
 // f(gen.next(), gen.next()) right to left (1)
 uint32_t g_arg_order = 1;
 
+// current launch shape of the frame kernel (vr_set_launch_shape / autotuner)
+vr_launch_shape g_shape = {256, 16, 1};
+
+bool valid_shape(const vr_launch_shape& s)
+{
+    const bool b = s.block_size == 128 || s.block_size == 256;
+    const bool d = s.stack_lds == 8 || s.stack_lds == 16 || s.stack_lds == 32;
+    return b && d && s.tile >= 1 && s.tile <= 64;
+}
+
 // largest material table the adjoint keeps in LDS partials
 constexpr size_t ADJ_MAX_MATERIALS = 2048;
+
+int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                     const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
+                     unsigned long long* ray_counts, uint32_t* work, bool counted, const vr_launch_shape& shape,
+                     hipStream_t st);
 
 int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
               const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
@@ -553,10 +615,22 @@ int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, con
 {
     if ((size_t)sp.width * sp.height == 0) return VR_OK;
     VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
-    const pt_args A =
+    return render_pt_shaped(bs, mats, num_mats, kp, cam, sp, accum, color, ray_counts, work, counted, g_shape, st);
+}
+
+int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
+                     const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
+                     unsigned long long* ray_counts, uint32_t* work, bool counted, const vr_launch_shape& shape,
+                     hipStream_t st)
+{
+    if ((size_t)sp.width * sp.height == 0) return VR_OK;
+    VR_CHECK((uint64_t)padded_pixels(sp.width, sp.height, shape.tile) < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "render_pt: image too large");
+    pt_args A =
         single_args(bs, mats, num_mats, kp, cam, sp, accum, color, nullptr, ray_counts, work, nullptr, g_arg_order);
+    A.tile = shape.tile;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
-    return counted ? launch_pt<PT_COUNT, false>(A, 0, st) : launch_pt<PT_RENDER, false>(A, 0, st);
+    return counted ? launch_pt<PT_COUNT, false>(A, 0, shape, st) : launch_pt<PT_RENDER, false>(A, 0, shape, st);
 }
 
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
@@ -568,10 +642,11 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
     VR_CHECK(kp.num_bounces <= (uint32_t)ADJ_MAX_VERTICES, VR_ERR_UNSUPPORTED,
              "render_pt_backward: more than 16 bounces");
     VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_backward: BVH deeper than the stack");
-    const pt_args A =
+    pt_args A =
         single_args(bs, mats, num_mats, kp, cam, sp, nullptr, nullptr, dl, ray_counts, work, grad, g_arg_order);
+    A.tile = g_shape.tile;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
-    return launch_pt<PT_ADJOINT, false>(A, 6 * num_mats * sizeof(float), st);
+    return launch_pt<PT_ADJOINT, false>(A, 6 * num_mats * sizeof(float), g_shape, st);
 }
 
 // Many scenes in one persistent launch.  d_scenes / d_prefix are device
@@ -580,10 +655,11 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
 // successive frames.  Only kernel launches are enqueued (graph capturable).
 int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num_scenes, uint32_t total_pixels,
                     uint32_t* d_frame, bool advance_frame, unsigned long long* ray_counts, uint32_t* work,
-                    uint32_t grad_slots, double* grad, hipStream_t st)
+                    uint32_t grad_slots, double* grad, const vr_launch_shape& shape, hipStream_t st)
 {
     if (total_pixels == 0) return VR_OK;
     pt_args A{};
+    A.tile = shape.tile;
     A.scenes = static_cast<const scene_desc*>(d_scenes);
     A.prefix = d_prefix;
     A.num_scenes = num_scenes;
@@ -597,10 +673,12 @@ int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, d_frame, advance_frame ? 1 : 0);
     if (grad) {
         VR_CHECK(grad_slots <= ADJ_MAX_MATERIALS, VR_ERR_UNSUPPORTED, "batch adjoint: more than 2048 materials");
-        return launch_pt<PT_ADJOINT, true>(A, 6 * (size_t)grad_slots * sizeof(float), st);
+        return launch_pt<PT_ADJOINT, true>(A, 6 * (size_t)grad_slots * sizeof(float), shape, st);
     }
-    return launch_pt<PT_RENDER, true>(A, 0, st);
+    return launch_pt<PT_RENDER, true>(A, 0, shape, st);
 }
+
+uint32_t padded_pixel_count(uint32_t w, uint32_t h, uint32_t tile) { return padded_pixels(w, h, tile); }
 
 size_t scene_desc_size() { return sizeof(scene_desc); }
 

@@ -275,3 +275,114 @@ def config1_camera():
 def config0_camera(width: int = 512, height: int = 512):
     """Camera inside the room looking at the icosphere."""
     return (0.2, 0.3, 0.95), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), np.float32(45.0 * np.pi / 180.0), width / height
+
+
+RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+
+
+def config2_scenes(count: int = 64, nmin: int = 1_000, nmax: int = 2_000_000, seed: int = 4,
+                   resolutions=RESOLUTIONS):
+    """`count` independent scenes of seeded random-triangle clusters, sizes
+    log-uniform in [nmin, nmax], each with its own camera and resolution.
+    Yields dicts with tris, mats (8 Lambertian materials, one emissive),
+    camera (eye, center, up, fovy, aspect) and (width, height)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count)).astype(np.int64)
+    for i, n in enumerate(sizes):
+        tris = random_clusters(int(n), seed=seed * 1000 + i, clusters=int(rng.integers(4, 32)), sigma=0.3)
+        tris["geom_id"] = rng.integers(0, 8, size=tris.size)
+        mats = lambert_materials(8, seed=seed * 1000 + i, emissive_fraction=0.125)
+        w, h = resolutions[int(rng.integers(0, len(resolutions)))]
+        v = tris["v1"]
+        c = np.median(v, axis=0)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = c + d * float(rng.uniform(2.5, 4.0))
+        yield dict(tris=tris, mats=mats, camera=(tuple(eye), tuple(c), (0.0, 1.0, 0.0), np.float32(0.8), w / h),
+                   size=(w, h))
+
+
+def config3_scene(level: int = 8, num_materials: int = 256, num_emissive: int = 8, seed: int = 0):
+    """Room plus an icosphere of level `level` (1,310,720 triangles at 8) with
+    `num_materials` materials: albedo U(0.1, 0.9) (seed 5); the last
+    `num_emissive` materials are emissive, ce U(0.5, 1.5), ls = 1.  Room quads
+    take materials 0..11 except the two ceiling quads, which take the first two
+    emissive materials; sphere triangles take a material per latitude/longitude
+    cell, six cells emissive.  Returns (tris, mats)."""
+    rng = np.random.default_rng(seed)
+    room_t = room(1.0)
+    sph = icosphere(level, 0.3, jitter=0.002, rng=rng)
+    nm, ne = num_materials, num_emissive
+    first_e = nm - ne
+    # room: 12 quads, triangles q and q + 12 belong to quad q
+    q = np.arange(24) % 12
+    ceiling = []
+    for k in range(12):
+        ys = np.concatenate([room_t["v1"][[k, k + 12], 1]])
+        if np.all(ys > 0.999):
+            ceiling.append(k)
+    room_m = q.copy()
+    for j, k in enumerate(ceiling[:2]):
+        room_m[q == k] = first_e + j
+    room_t["geom_id"] = room_m
+    # sphere: latitude/longitude cells over the centroid direction
+    c = sph["v1"] + (sph["e1"] + sph["e2"]) / 3.0
+    theta = np.arccos(np.clip(c[:, 1] / np.linalg.norm(c, axis=1), -1, 1))
+    phi = np.arctan2(c[:, 2], c[:, 0]) + np.pi
+    ncell = first_e - 12
+    nlat = max(1, int(round(np.sqrt(ncell / 2))))
+    nlon = ncell // nlat
+    cell = np.minimum((theta / np.pi * nlat).astype(np.int64), nlat - 1) * nlon + \
+        np.minimum((phi / (2 * np.pi) * nlon).astype(np.int64), nlon - 1)
+    sph_m = 12 + cell
+    emissive_cells = rng.choice(nlat * nlon, size=ne - 2, replace=False)
+    for j, ec in enumerate(emissive_cells):
+        sph_m[cell == ec] = first_e + 2 + j
+    sph["geom_id"] = sph_m
+    tris = concat(room_t, sph)
+    mats = lambert_materials(nm, seed=5, lo=0.1, hi=0.9)
+    erng = np.random.default_rng(6)
+    mats["type"][first_e:] = 1
+    mats["ce"][first_e:] = erng.uniform(0.5, 1.5, size=(ne, 3))
+    mats["ls"][first_e:] = 1.0
+    return tris, mats
+
+
+def displace(tris: np.ndarray, frame: int, amplitude: float = 0.01, seed: int = 0) -> np.ndarray:
+    """Per-frame sinusoidal vertex displacement (dynamic geometry): every
+    corner moves by amplitude * sin(k . p + frame * w + phase) along a seeded
+    direction, so shared corners move together."""
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=3) * 8.0
+    direction = rng.normal(size=3)
+    direction /= np.linalg.norm(direction)
+    wfreq = float(rng.uniform(0.5, 1.5))
+    out = tris.copy()
+    v1 = tris["v1"].astype(np.float64)
+    v2 = v1 + tris["e1"]
+    v3 = v1 + tris["e2"]
+
+    def move(p):
+        return p + amplitude * np.sin(p @ k + frame * wfreq)[:, None] * direction[None, :]
+
+    a, b, c = move(v1).astype(np.float32), move(v2).astype(np.float32), move(v3).astype(np.float32)
+    out["v1"] = a
+    out["e1"] = b - a
+    out["e2"] = c - a
+    return out
+
+
+def config4_scenes(count: int = 32, nmin: int = 50_000, nmax: int = 1_000_000, seed: int = 7):
+    """`count` scenes of random-triangle clusters with 50k-1M triangles and 64
+    materials each (one in eight emissive), camera per scene, 256 x 256."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count)).astype(np.int64)
+    for i, n in enumerate(sizes):
+        tris = random_clusters(int(n), seed=seed * 1000 + i, clusters=int(rng.integers(4, 16)), sigma=0.3)
+        tris["geom_id"] = rng.integers(0, 64, size=tris.size)
+        mats = lambert_materials(64, seed=seed * 1000 + i, lo=0.1, hi=0.9, emissive_fraction=0.125, emission=2.0)
+        c = np.median(tris["v1"], axis=0)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        yield dict(tris=tris, mats=mats, camera=(tuple(c + d * 3.0), tuple(c), (0.0, 1.0, 0.0), np.float32(0.8), 1.0),
+                   size=(256, 256))
