@@ -188,6 +188,14 @@ int vr_bvh_destroy(vr_bvh bvh);
 int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes,
                  size_t* num_nodes, uint32_t* h_indices);
 
+/* Host LBVH build in the reference's CPU layout (lbvh_builder::build on an
+ * index_bvh, detail/bvh/lbvh.h:500-656 driven by build_top_down): Morton
+ * order, splits at the highest differing code bit, equal codes split at the
+ * midpoint, leaves of up to max_leaf_size (default 4).  Arrays as
+ * vr_sah_build. */
+int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes,
+                       size_t* num_nodes, uint32_t* h_indices);
+
 /* ---- traversal -------------------------------------------------------- */
 /* closest_hit / any_hit over one BVH (detail/traverse_linear.inl:149-214 ->
  * intersect<ClosestHit|AnyHit>, detail/bvh/intersect.inl:41-357). */

@@ -387,6 +387,16 @@ def ref_sah(tris):
         t.close()
 
 
+def ref_lbvh_cpu(tris):
+    """The reference's CPU lbvh_builder tree (leaf size 4)."""
+    t = _Tmp()
+    try:
+        _run(["lbvhcpu", t.put("tris.bin", tris), tris.size, t.path("nodes.bin"), t.path("idx.bin")])
+        return np.fromfile(t.path("nodes.bin"), NODE_DTYPE), np.fromfile(t.path("idx.bin"), np.uint32)
+    finally:
+        t.close()
+
+
 def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, binary=None, timeout=600):
     """Time the reference CPU path (builder + tiled_sched frames)."""
     t = _Tmp()

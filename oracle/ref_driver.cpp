@@ -430,6 +430,16 @@ int main(int argc, char** argv)
         save_rt(a[9], rt);
         return 0;
     }
+    if (mode == "lbvhcpu" && na == 4) {
+        size_t n = std::strtoull(a[1], nullptr, 10);
+        auto tris = load<tri_t>(a[0], n);
+        lbvh_builder b;
+        auto tree = b.build(index_bvh<tri_t>{}, tris.data(), n, 4);
+        save(a[2], tree.nodes().data(), tree.nodes().size());
+        save(a[3], tree.indices().data(), tree.indices().size());
+        std::printf("%zu %zu\n", tree.nodes().size(), tree.indices().size());
+        return 0;
+    }
     if (mode == "sah" && na == 4) {
         size_t n = std::strtoull(a[1], nullptr, 10);
         auto tris = load<tri_t>(a[0], n);

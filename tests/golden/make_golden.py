@@ -42,6 +42,7 @@ def main():
     nodes, idx, _, _ = O.lbvh_build(tris)
     rays = rays_for(4096, seed=1)
     sah_nodes, sah_idx = O.ref_sah(tris)
+    cpu_nodes, cpu_idx = O.ref_lbvh_cpu(tris)
     np.savez_compressed(
         OUT / "c0_small.npz", tris=tris, ref_sorted_codes=sc, ref_sorted_ids=si, ref_karras=karras,
         lbvh_nodes=nodes, lbvh_indices=idx, rays=rays,
@@ -49,7 +50,8 @@ def main():
         ref_any=O.ref_trace(nodes, tris, idx, rays, True),
         ref_brute=O.ref_brute(tris, rays, False),
         sah_nodes=sah_nodes, sah_indices=sah_idx,
-        ref_sah_closest=O.ref_trace(sah_nodes, tris, sah_idx, rays, False))
+        ref_sah_closest=O.ref_trace(sah_nodes, tris, sah_idx, rays, False),
+        cpu_lbvh_nodes=cpu_nodes, cpu_lbvh_indices=cpu_idx)
 
     # seeds and random streams (make_random_seed.h, random_generator.h)
     rng = np.random.default_rng(2)

@@ -83,3 +83,15 @@ def test_sah_build_matches_reference_tree():
     inner = want_n["num_prims"] == 0
     assert np.array_equal(nodes["axis"][inner], want_n["axis"][inner])
     assert np.array_equal(nodes["sign"][inner], want_n["sign"][inner])
+
+
+def test_host_lbvh_matches_reference_cpu_tree():
+    """vr_lbvh_build_host reproduces the reference's CPU lbvh_builder tree
+    (top-down Morton splits, leaf size 4) node for node."""
+    import visionaray_amd as V
+    g = load_golden("c0_small")
+    nodes, idx = V.lbvh_build_host(g["tris"], 4)
+    want_n, want_i = g["cpu_lbvh_nodes"], g["cpu_lbvh_indices"]
+    assert nodes.size == want_n.size and np.array_equal(idx, want_i)
+    for k in ("lo", "hi", "first", "num_prims"):
+        assert np.array_equal(nodes[k], want_n[k]), k

@@ -245,6 +245,18 @@ def sah_build(prims: np.ndarray, max_leaf_size: int = 4):
     return nodes[: nn.value].copy(), idx[:n].copy()
 
 
+def lbvh_build_host(prims: np.ndarray, max_leaf_size: int = 4):
+    """Host LBVH in the reference's CPU layout (lbvh_builder on an index_bvh,
+    detail/bvh/lbvh.h:500-656): returns (nodes, indices)."""
+    prims = np.ascontiguousarray(prims, TRI_DTYPE)
+    n = prims.size
+    nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
+    idx = np.zeros(max(n, 1), np.uint32)
+    nn = C.c_size_t()
+    check(lib().vr_lbvh_build_host(ptr(prims), n, max_leaf_size, ptr(nodes), C.byref(nn), ptr(idx)))
+    return nodes[: nn.value].copy(), idx[:n].copy()
+
+
 def trace(bvh: BVH, rays, any_hit_: bool = False, counted: bool = False, stream=None):
     """Trace rays (numpy RAY_DTYPE or DeviceArray); returns host hits (and
     per-ray [pairs, triangles] test counts when counted)."""

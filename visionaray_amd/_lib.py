@@ -54,6 +54,7 @@ SIGNATURES = {
     "vr_bvh_indices": (I, [P, P, P]),
     "vr_bvh_destroy": (I, [P]),
     "vr_sah_build": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
+    "vr_lbvh_build_host": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_trace": (I, [P, I, P, SZ, P, P]),
     "vr_trace_counted": (I, [P, I, P, SZ, P, P, P]),
     "vr_render_ao": (I, [P, P, P, U32, F, P, P, P, P]),

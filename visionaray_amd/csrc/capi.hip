@@ -24,6 +24,8 @@ int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& 
               const float bg[3], float* accum, float* color, hipStream_t st);
 int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
                    uint32_t* indices);
+int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                    uint32_t* indices);
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
                       unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
@@ -480,6 +482,16 @@ int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh
     VR_CHECK(num_nodes && (n == 0 || (h_prims && h_nodes && h_indices)), VR_ERR_INVALID_ARGUMENT,
              "vr_sah_build: null argument");
     return sah_build_host(h_prims, n, max_leaf_size, h_nodes, num_nodes, h_indices);
+    VR_TRY_END
+}
+
+int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes,
+                       size_t* num_nodes, uint32_t* h_indices)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(num_nodes && (n == 0 || (h_prims && h_nodes && h_indices)), VR_ERR_INVALID_ARGUMENT,
+             "vr_lbvh_build_host: null argument");
+    return lbvh_build_host(h_prims, n, max_leaf_size, h_nodes, num_nodes, h_indices);
     VR_TRY_END
 }
 

@@ -1,7 +1,14 @@
-// sah.cpp -- host binned SAH builder (binned_sah_builder with object splits,
-// detail/bvh/sah.h:176-807, driven as build_top_down, detail/bvh/build_top_down.h:27-174).
+// host_build.cpp -- the reference's two host builders, driven top down like
+// build_top_down (detail/bvh/build_top_down.h:27-174): children appended at
+// the end of the node array, the right subtree numbered before the left, leaf
+// indices appended in visit order.  The recursion is an explicit stack here.
 //
-// The tree is meant to be the reference's tree node for node, so the decisions
+//  * binned SAH (binned_sah_builder with object splits, detail/bvh/sah.h:176-807)
+//  * CPU LBVH   (lbvh_builder::init/find_split/split, detail/bvh/lbvh.h:500-656):
+//    Morton-sorted references split at the highest differing code bit, equal
+//    codes at the midpoint, leaves of up to max_leaf_size references.
+//
+// SAH: the tree is meant to be the reference's tree node for node, so the decisions
 // follow the same arithmetic: 16 bins along the largest centroid extent, the
 // split cost 1 + (A_l / A) 3 N_l + (A_r / A) 3 N_r over half surface areas,
 // ties resolved towards the rightmost bin boundary, a leaf when the best split
@@ -203,6 +210,114 @@ int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh
         // the right subtree is built (and numbered) first
         stack.push_back(lc);
         stack.push_back(rc);
+    }
+    *num_nodes = node_count;
+    return VR_OK;
+}
+
+namespace {
+
+// morton_encode3D (morton.h:33-47)
+inline uint32_t spread10(uint32_t n)
+{
+    n &= 0x000003FFu;
+    n = (n ^ (n << 16)) & 0xFF0000FFu;
+    n = (n ^ (n << 8)) & 0x0300F00Fu;
+    n = (n ^ (n << 4)) & 0x030C30C3u;
+    n = (n ^ (n << 2)) & 0x09249249u;
+    return n;
+}
+
+struct code_ref {
+    int id;
+    uint32_t code;
+};
+
+inline int clz32(uint32_t v) { return v == 0 ? 32 : __builtin_clz(v); }
+
+// lbvh_builder::find_split (detail/bvh/lbvh.h:500-532): the last reference
+// sharing the first reference's code prefix beyond the range's common prefix
+int lbvh_find_split(const std::vector<code_ref>& r, int first, int last)
+{
+    const uint32_t cf = r[first].code, cl = r[last - 1].code;
+    if (cf == cl) return (first + last) / 2;
+    const int common = clz32(cf ^ cl);
+    int result = first, step = last - first;
+    do {
+        step = (step + 1) / 2;
+        const int next = result + step;
+        if (next < last) {
+            const uint32_t c = r[next].code;
+            if (cf == c || clz32(cf ^ c) > common) result = next;
+        }
+    } while (step > 1);
+    return result;
+}
+
+}  // namespace
+
+int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                    uint32_t* indices)
+{
+    *num_nodes = 0;
+    if (n == 0) return VR_OK;
+    VR_CHECK(n < (size_t)1 << 31, VR_ERR_UNSUPPORTED, "vr_lbvh_build_host: too many primitives");
+    if (max_leaf_size <= 0) max_leaf_size = 4;
+    VR_CHECK(max_leaf_size <= 65535, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build_host: leaf size exceeds bvh_node range");
+    // init (detail/bvh/lbvh.h:545-599): primitive boxes, scene and centroid
+    // boxes, Morton codes, std::stable_sort by code
+    std::vector<box> pb(n);
+    box scene, cent;
+    std::vector<f3> cs(n);
+    for (size_t i = 0; i < n; ++i) {
+        const vr_triangle& t = prims[i];
+        const f3 v1 = mk3(t.v1[0], t.v1[1], t.v1[2]);
+        box b;
+        b.insert(v1);
+        b.insert(add3(v1, mk3(t.e1[0], t.e1[1], t.e1[2])));
+        b.insert(add3(v1, mk3(t.e2[0], t.e2[1], t.e2[2])));
+        pb[i] = b;
+        scene.insert(b);
+        cs[i] = b.center();
+        cent.insert(cs[i]);
+    }
+    const f3 center = cent.center(), size = cent.size();
+    std::vector<code_ref> refs(n);
+    for (size_t i = 0; i < n; ++i) {
+        f3 c = sub3(cs[i], center);
+        c = add3(c, muls3(size, 0.5f));
+        c = f3{c.x / size.x, c.y / size.y, c.z / size.z};
+        c = min3v(max3v(muls3(c, 1024.0f), mk3(0.0f, 0.0f, 0.0f)), mk3(1023.0f, 1023.0f, 1023.0f));
+        refs[i].id = (int)i;
+        refs[i].code = spread10((uint32_t)c.x) | (spread10((uint32_t)c.y) << 1) | (spread10((uint32_t)c.z) << 2);
+    }
+    std::stable_sort(refs.begin(), refs.end(), [](const code_ref& a, const code_ref& b) { return a.code < b.code; });
+
+    struct item {
+        box bounds;
+        int first, last;
+        uint32_t node;
+    };
+    size_t node_count = 1, index_count = 0;
+    std::vector<item> stack{{scene, 0, (int)n, 0u}};
+    while (!stack.empty()) {
+        const item it = stack.back();
+        stack.pop_back();
+        if (it.last - it.first <= max_leaf_size) {
+            set_node(nodes[it.node], it.bounds, (uint32_t)index_count, (uint32_t)(it.last - it.first), 0, 0);
+            for (int i = it.first; i < it.last; ++i) indices[index_count++] = (uint32_t)refs[i].id;
+            continue;
+        }
+        const int split = lbvh_find_split(refs, it.first, it.last);
+        box lb, rb;
+        for (int i = it.first; i != split + 1; ++i) lb.insert(pb[refs[i].id]);
+        for (int i = split + 1; i != it.last; ++i) rb.insert(pb[refs[i].id]);
+        const uint32_t first_child = (uint32_t)node_count;
+        VR_CHECK(node_count + 2 <= 2 * n - 1, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build_host: node array overflow");
+        set_node(nodes[it.node], it.bounds, first_child, 0, 0, 0);
+        node_count += 2;
+        stack.push_back({lb, it.first, split + 1, first_child});
+        stack.push_back({rb, split + 1, it.last, first_child + 1});
     }
     *num_nodes = node_count;
     return VR_OK;
