@@ -58,6 +58,10 @@ def main():
                          "differences; c4 dynamic geometry (refit vs rebuild) with batched adjoint")
     ap.add_argument("--scenes", type=int, default=0, help="c2/c4: number of scenes (default 64 / 32)")
     ap.add_argument("--fd-params", type=int, default=0, help="c3: finite-difference parameters (0 = all)")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="for hardware-counter runs: one build, one frame, and one device copy of "
+                         "--copy-mb MiB (the calibration kernel of known traffic); prints a short JSON line")
+    ap.add_argument("--copy-mb", type=int, default=1024)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -87,6 +91,10 @@ def main():
     stream = V.Stream()
     d_tris = V.DeviceArray.from_host(tris)
     d_mats = V.DeviceArray.from_host(mats)
+
+    if args.profile_pass:
+        profile_pass(args, V, S, tris, d_tris, d_mats, stream)
+        return
 
     # ---- LBVH build timing (same scene) ----
     builder = V.lbvh_builder()
@@ -209,6 +217,33 @@ def main():
     print(line, flush=True)
     if args.out:
         Path(args.out).write_text(line + "\n")
+
+
+def profile_pass(args, V, S, tris, d_tris, d_mats, stream):
+    """One LBVH build, one path-traced frame, one copy kernel of known size:
+    the dispatches a hardware-counter run reads (FETCH_SIZE / WRITE_SIZE of the
+    copy calibrate the counters against bytes moved)."""
+    bvh = V.lbvh_builder().build(d_tris, keep_nodes=False, stream=stream)
+    eye, center, up, fovy, _ = S.config1_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, args.width / args.height)
+    kp = V.make_kernel_params(num_bounces=args.depth, epsilon=1e-4, bg_color=(0.0, 0.0, 0.0, 1.0),
+                              ambient_color=(0.6, 0.7, 0.9, 1.0))
+    rt = V.render_target(args.width, args.height)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=args.spp), cam, rt)
+    stats = V.DeviceArray(4, np.uint64, zero=True)
+    sched.frame(V.pathtracing.kernel(bvh, d_mats, kp), sp, stream=stream, stats=stats)
+    sched.frame(V.pathtracing.kernel(bvh, d_mats, kp), sp, stream=stream)
+    nbytes = args.copy_mb * (1 << 20)
+    src = V.DeviceArray(nbytes // 4, np.float32, zero=True)
+    dst = V.DeviceArray(nbytes // 4, np.float32)
+    V.check(V.lib().vr_device_copy(dst.ptr, src.ptr, nbytes, stream.handle))
+    stream.synchronize()
+    st = stats.to_host().astype(np.float64)
+    rays = st[0] + st[1]
+    alg = rays * (RAY_IN + HIT_OUT) + st[2] * PAIR_BYTES + st[3] * TRI_BYTES
+    print(json.dumps({"profile_pass": True, "copy_bytes": nbytes, "pathtrace_algorithmic_bytes": int(alg),
+                      "rays": int(rays), "node_pairs": int(st[2]), "triangles": int(st[3])}), flush=True)
 
 
 def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):

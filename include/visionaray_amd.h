@@ -143,6 +143,10 @@ int vr_event_create(void** event);
 int vr_event_destroy(void* event);
 int vr_event_record(void* event, void* stream);
 int vr_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Device-to-device copy by a kernel (16-byte vectors; bytes must be a
+ * multiple of 16): graph-capturable, and a kernel of known HBM traffic for
+ * calibrating hardware counters. */
+int vr_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
 
 /* ---- camera ------------------------------------------------------------- */
 /* pinhole_camera::perspective + look_at + begin_frame

@@ -181,6 +181,22 @@ struct binned_sah_builder {
     }
 };
 
+// The reference's CPU lbvh_builder tree (top-down, leaf size 4) built on the
+// host and uploaded (lbvh_builder::build on an index_bvh, detail/bvh/lbvh.h:534-656).
+struct host_lbvh_builder {
+    hip_index_bvh build(const hip_index_bvh& /* tag */, const basic_triangle* h_prims, size_t num_prims,
+                        int max_leaf_size = -1)
+    {
+        std::vector<bvh_node> nodes(num_prims ? 2 * num_prims - 1 : 1);
+        std::vector<uint32_t> idx(num_prims ? num_prims : 1);
+        size_t nn = 0;
+        check(vr_lbvh_build_host(h_prims, num_prims, max_leaf_size, nodes.data(), &nn, idx.data()));
+        vr_bvh h = nullptr;
+        check(vr_bvh_from_nodes(h_prims, num_prims, nodes.data(), nn, idx.data(), &h));
+        return hip_index_bvh(h);
+    }
+};
+
 // bvh_refitter::refit (detail/bvh/refit.h:28-114) for LBVHs on the device.
 struct bvh_refitter {
     void refit(hip_index_bvh& tree, const basic_triangle* d_prims, void* stream = nullptr)
@@ -328,6 +344,31 @@ public:
     }
     unsigned frame_id() const { return frame_id_; }
 
+    // Path-replay adjoint of the last frame (no reference counterpart):
+    // d_grad[6 m + 0..2] += d loss / d cd, d_grad[6 m + 3..5] += d loss / d ce.
+    void backward(const pathtracing::kernel<kernel_params>& k, sched_params sp, const float* d_dl_dimage,
+                  double* d_grad, void* stream = nullptr)
+    {
+        sp.cam.begin_frame();
+        vr_sched_params s = params(sp);
+        s.frame_id = frame_id_ ? frame_id_ - 1 : 0;
+        check(vr_render_pt_backward(k.params.bvh->handle(), k.params.materials, k.params.num_materials, &k.params.p,
+                                    &sp.cam.basis(), &s, d_dl_dimage, d_grad, nullptr, stream));
+    }
+
+    // Pick the fastest launch shape for this frame (vr_autotune_pt).
+    vr_launch_shape autotune(const pathtracing::kernel<kernel_params>& k, sched_params sp, int reps = 2,
+                             void* stream = nullptr)
+    {
+        sp.cam.begin_frame();
+        const vr_sched_params s = params(sp);
+        vr_launch_shape best{};
+        float ms = 0.0f;
+        check(vr_autotune_pt(k.params.bvh->handle(), k.params.materials, k.params.num_materials, &k.params.p,
+                             &sp.cam.basis(), &s, sp.rt.accum, sp.rt.color, reps, &best, &ms, stream));
+        return best;
+    }
+
 private:
     vr_sched_params params(const sched_params& sp) const
     {
@@ -341,6 +382,29 @@ private:
         return s;
     }
     unsigned frame_id_ = 0;
+};
+
+// Many scenes in one persistent launch (vr_batch_*), optionally replayed as a
+// captured HIP graph; no reference counterpart (one cuda_sched::frame per scene).
+class scene_batch {
+public:
+    scene_batch(const std::vector<vr_scene>& scenes) { check(vr_batch_create(scenes.data(), scenes.size(), &b_)); }
+    scene_batch(const scene_batch&) = delete;
+    scene_batch& operator=(const scene_batch&) = delete;
+    ~scene_batch() { vr_batch_destroy(b_); }
+    void render(uint64_t* d_ray_counts = nullptr, void* stream = nullptr)
+    {
+        check(vr_batch_render(b_, d_ray_counts, stream));
+    }
+    void backward(double* d_grad, void* stream = nullptr) { check(vr_batch_backward(b_, d_grad, nullptr, stream)); }
+    void capture(void* stream, uint64_t* d_ray_counts = nullptr)
+    {
+        check(vr_batch_capture_graph(b_, d_ray_counts, stream));
+    }
+    void replay(void* stream) { check(vr_batch_launch_graph(b_, stream)); }
+
+private:
+    vr_batch b_ = nullptr;
 };
 
 }  // namespace visionaray_amd

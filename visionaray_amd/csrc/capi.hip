@@ -26,6 +26,7 @@ int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh
                    uint32_t* indices);
 int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
                     uint32_t* indices);
+int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
                       unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
@@ -236,6 +237,12 @@ int vr_event_elapsed_ms(void* start, void* stop, float* ms)
     VR_HIP_CHECK(hipEventSynchronize((hipEvent_t)stop));
     VR_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
     return VR_OK;
+}
+
+int vr_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream)
+{
+    VR_CHECK(bytes == 0 || (d_dst && d_src), VR_ERR_INVALID_ARGUMENT, "vr_device_copy: null pointer");
+    return device_copy(d_dst, d_src, bytes, (hipStream_t)stream);
 }
 
 // pinhole_camera::look_at + perspective + begin_frame

@@ -462,6 +462,11 @@ __global__ void k_iota_u32(uint32_t* p, size_t n)
     if (i < n) p[i] = (uint32_t)i;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -648,6 +653,18 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     hipLaunchKernelGGL(k_bottom_up, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.prims, n, bs.sorted_ids,
                        s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, bs.ref_nodes, bs.root_box,
                        reinterpret_cast<uint32_t*>(bs.root_box + 6));
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    VR_CHECK(bytes % 16 == 0, VR_ERR_INVALID_ARGUMENT, "vr_device_copy: bytes must be a multiple of 16");
+    const size_t n = bytes / 16;
+    if (n == 0) return VR_OK;
+    const unsigned grid = std::max(1u, std::min(div_up(n, BLOCK), 65536u));
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(BLOCK), 0, st, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
