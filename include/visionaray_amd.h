@@ -253,16 +253,35 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
                           const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                           const float* d_dl_dimage, double* d_grad, uint64_t* d_ray_counts, void* stream);
 
+/* vr_render_pt with the Russian-roulette probabilities taken from the path
+ * throughput under d_rr_materials (same count and layout) while shading uses
+ * d_materials.  With both tables equal it renders exactly vr_render_pt's
+ * frame.  Holding d_rr_materials at a parameter point p0 gives the estimator
+ * whose derivative at p0 vr_render_pt_backward returns: central differences
+ * of it with common random numbers keep every roulette decision, so they
+ * check the adjoint without the noise of decisions that flip. */
+int vr_render_pt_frozen_rr(vr_bvh bvh, const vr_material* d_materials, const vr_material* d_rr_materials,
+                           size_t num_materials, const vr_kernel_params* kp, const vr_camera* cam,
+                           const vr_sched_params* sp, float* d_accum, float* d_color, uint64_t* d_ray_counts,
+                           void* stream);
+
 /* ---- launch shape and autotuning (no reference counterpart: cuda_sched
  * takes a fixed 16x16 block, detail/cuda_sched.h:43) -------------------- */
 /* Launch shape of the path-tracing frame kernel: threads per block (128 or
  * 256), traversal-stack entries per lane kept in LDS (8, 16 or 32; deeper
- * entries spill to private memory), and the pixel-queue tile (pixels are
- * handed out in tile x tile squares, 1 = scanlines; 1..64). */
+ * entries spill to private memory), the pixel-queue tile (pixels are
+ * handed out in tile x tile squares, 1 = scanlines; 1..64) and the schedule:
+ * VR_SCHEDULE_PERSISTENT (a resident grid whose lanes take the next pixel
+ * when theirs is done) or VR_SCHEDULE_PER_PIXEL (one thread per pixel over
+ * the whole image, as cuda_sched launches it, detail/cuda_sched.inl:27-70;
+ * stack_lds 16 only).  Batches always run persistent. */
+#define VR_SCHEDULE_PERSISTENT 0u
+#define VR_SCHEDULE_PER_PIXEL 1u
 typedef struct vr_launch_shape {
     uint32_t block_size;
     uint32_t stack_lds;
     uint32_t tile;
+    uint32_t schedule;
 } vr_launch_shape;
 
 int vr_set_launch_shape(const vr_launch_shape* shape);

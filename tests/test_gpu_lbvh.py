@@ -14,6 +14,8 @@ def _scenes():
         "c0_level3": S.config0_scene(level=3),
         "c0_level5": S.config0_scene(level=5),
         "clusters_50k": S.random_clusters(50_000, seed=5),
+        # more than 65536 leaves: every pass of the bottom-up box computation
+        "clusters_150k": S.random_clusters(150_000, seed=7),
         "heightfield_64": S.heightfield(64, seed=1),
     }
     # degenerate centroid box on one axis (flat heightfield: every y equal)
@@ -75,7 +77,7 @@ def test_lbvh_matches_reference_sort_golden(V):
 
 
 def test_lbvh_deterministic(V):
-    tris = SCENES["clusters_50k"]
+    tris = SCENES["clusters_150k"]
     a = V.lbvh_builder().build(tris).nodes()
     b = V.lbvh_builder().build(tris).nodes()
     assert a.tobytes() == b.tobytes()
@@ -91,8 +93,9 @@ def test_empty_scene(V):
     assert np.all(hits["prim_id"] == -1)
 
 
-def test_refit_matches_oracle(V, O):
-    tris = SCENES["c0_level5"].copy()
+@pytest.mark.parametrize("name", ["c0_level5", "clusters_150k"])
+def test_refit_matches_oracle(V, O, name):
+    tris = SCENES[name].copy()
     bvh = V.lbvh_builder().build(tris)
     nodes, idx, _, _ = O.lbvh_build(tris)
     moved = tris.copy()

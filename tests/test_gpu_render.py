@@ -31,7 +31,10 @@ def gpu_frame(V, bvh, mats, kp, cam, w, h, spp, frame_id=0, counted=False):
     sched.frame(V.pathtracing.kernel(bvh, d_mats, kp), V.make_sched_params(V.jittered_blend_type(spp=spp), cam, rt),
                 stats=stats)
     img = rt.color_host()
-    return (img, sched.rays(), stats.to_host() if counted else None)
+    if counted:  # a counted frame writes its ray counts into `stats`
+        st = stats.to_host()
+        return img, (int(st[0]), int(st[1])), st
+    return img, sched.rays(), None
 
 
 def stats_compare(got, want, sigma, tag):
@@ -232,10 +235,68 @@ def test_adjoint_matches_finite_differences_without_roulette(V):
     assert cos > 0.9999
 
 
-@pytest.mark.parametrize("shape", [(128, 8, 1), (256, 32, 8), (128, 16, 16), (256, 16, 1)])
+def _frozen_loss(V, bvh, mats, rr_mats, d_tgt, kp, cam, w, h, spp):
+    rt = V.render_target(w, h)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=spp), cam, rt)
+    sched.frame_frozen_rr(V.pathtracing.kernel(bvh, V.DeviceArray.from_host(mats), kp), rr_mats, sp)
+    return V.image_l2_loss(rt.color, d_tgt, None), rt.color_host()
+
+
+def test_frozen_roulette_frame_is_the_frame(V):
+    """With the roulette table equal to the shading table the frozen-roulette
+    frame is bit-identical to vr_render_pt's."""
+    tris, mats, _ = adjoint_scene()
+    w = h = 40
+    eye, center, up, fovy, _ = S.config0_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, 1.0)
+    kp = V.make_kernel_params(num_bounces=6, epsilon=1e-4, ambient_color=(0.3, 0.3, 0.3, 1.0))
+    bvh = V.lbvh_builder().build(tris)
+    want, _, _ = gpu_frame(V, bvh, mats, kp, cam, w, h, 4)
+    d_tgt = V.DeviceArray.from_host(np.zeros(w * h * 4, np.float32))
+    _, got = _frozen_loss(V, bvh, mats, V.DeviceArray.from_host(mats), d_tgt, kp, cam, w, h, 4)
+    assert np.array_equal(got, want)
+
+
+def test_adjoint_matches_finite_differences_with_roulette(V):
+    """Depth 5 (Russian roulette from the third vertex on): central
+    differences of the frozen-roulette estimator (every roulette decision kept
+    at common random numbers) must agree with the adjoint."""
+    tris, mats, target = adjoint_scene()
+    w = h = 32
+    eye, center, up, fovy, _ = S.config0_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, 1.0)
+    kp = V.make_kernel_params(num_bounces=5, epsilon=1e-4, ambient_color=(0.3, 0.3, 0.3, 1.0))
+    bvh = V.lbvh_builder().build(tris)
+    tgt_img, _, _ = gpu_frame(V, bvh, target, kp, cam, w, h, 8)
+    d_tgt = V.DeviceArray.from_host(tgt_img.reshape(-1))
+    _, g = gpu_loss_grad(V, bvh, mats, d_tgt, kp, cam, w, h, 8)
+    d_rr = V.DeviceArray.from_host(mats)
+    hstep = 1e-3
+    a, f = [], []
+    for m in range(mats.size):
+        emissive = mats["type"][m] == 1
+        key, off = ("ce", 3) if emissive else ("cd", 0)
+        for c in range(3):
+            mp, mm = mats.copy(), mats.copy()
+            mp[key][m, c] += hstep
+            mm[key][m, c] -= hstep
+            lp, _ = _frozen_loss(V, bvh, mp, d_rr, d_tgt, kp, cam, w, h, 8)
+            lm, _ = _frozen_loss(V, bvh, mm, d_rr, d_tgt, kp, cam, w, h, 8)
+            f.append((lp - lm) / (2 * hstep))
+            a.append(g[m, off + c])
+    a, f = np.array(a), np.array(f)
+    cos = float(a @ f / np.linalg.norm(a) / np.linalg.norm(f))
+    print(f"[adjoint] vs central FD of the frozen-roulette estimator (depth 5): cosine {cos:.7f}")
+    assert cos > 0.999
+
+
+@pytest.mark.parametrize("shape", [(128, 8, 1), (256, 32, 8), (128, 16, 16), (256, 16, 1), (256, 16, 16, 1),
+                                   (128, 16, 1, 1)])
 def test_launch_shapes_give_identical_frames(V, shape):
-    """Block size, LDS stack depth and pixel tile change only the schedule:
-    the image is bit-identical for every compiled shape."""
+    """Block size, LDS stack depth, pixel tile and schedule (persistent or one
+    thread per pixel) change only the schedule: the image is bit-identical for
+    every compiled shape."""
     tris = S.config0_scene(level=3)
     mats = S.lambert_materials(1, seed=3)
     tris["geom_id"] = 0

@@ -358,12 +358,15 @@ class ao_kernel:
     bg: tuple = (0.0, 0.0, 0.0)
 
 
-LAUNCH_SHAPE_DTYPE = np.dtype([("block_size", "<u4"), ("stack_lds", "<u4"), ("tile", "<u4")])
+LAUNCH_SHAPE_DTYPE = np.dtype([("block_size", "<u4"), ("stack_lds", "<u4"), ("tile", "<u4"), ("schedule", "<u4")])
+SCHEDULE_PERSISTENT = 0
+SCHEDULE_PER_PIXEL = 1
 
 
-def set_launch_shape(block_size: int = 256, stack_lds: int = 16, tile: int = 1):
-    """Launch shape of the path-tracing frame kernel (vr_set_launch_shape)."""
-    a = np.array([(block_size, stack_lds, tile)], LAUNCH_SHAPE_DTYPE)
+def set_launch_shape(block_size: int = 256, stack_lds: int = 16, tile: int = 1, schedule: int = SCHEDULE_PERSISTENT):
+    """Launch shape of the path-tracing frame kernel (vr_set_launch_shape);
+    schedule SCHEDULE_PER_PIXEL launches one thread per pixel as cuda_sched does."""
+    a = np.array([(block_size, stack_lds, tile, schedule)], LAUNCH_SHAPE_DTYPE)
     check(lib().vr_set_launch_shape(ptr(a)))
 
 
@@ -489,6 +492,16 @@ class hip_sched:
                                      ptr(bg), sp.rt.accum.ptr, sp.rt.color.ptr, _stream(stream)))
         else:
             raise TypeError(f"unsupported kernel {type(kernel).__name__}")
+        self.frame_id += 1
+
+    def frame_frozen_rr(self, kernel, rr_materials: "DeviceArray", sp: sched_params, stream=None):
+        """vr_render_pt_frozen_rr: a frame whose roulette probabilities follow
+        the throughput under `rr_materials` (the estimator the adjoint
+        differentiates); frame_id is incremented like frame()."""
+        s = self._sp(sp)
+        check(lib().vr_render_pt_frozen_rr(kernel.bvh.handle, kernel.materials.ptr, rr_materials.ptr,
+                                           kernel.materials.count, ptr(kernel.params), ptr(sp.cam), ptr(s),
+                                           sp.rt.accum.ptr, sp.rt.color.ptr, self.ray_counts.ptr, _stream(stream)))
         self.frame_id += 1
 
     def backward(self, kernel, sp: sched_params, dl_dimage: "DeviceArray", grad: "DeviceArray", stream=None,

@@ -63,6 +63,7 @@ SIGNATURES = {
     "vr_render_pt_counted": (I, [P, P, SZ, P, P, P, P, P, P, P]),
     "vr_image_l2_loss": (I, [P, P, SZ, P, P, P]),
     "vr_render_pt_backward": (I, [P, P, SZ, P, P, P, P, P, P, P]),
+    "vr_render_pt_frozen_rr": (I, [P, P, P, SZ, P, P, P, P, P, P, P]),
     "vr_set_launch_shape": (I, [P]),
     "vr_get_launch_shape": (I, [P]),
     "vr_autotune_pt": (I, [P, P, SZ, P, P, P, P, P, I, P, P, P]),

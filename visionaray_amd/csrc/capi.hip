@@ -31,6 +31,9 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
                       unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
 int image_l2_loss(const float* img, const float* tgt, size_t npix, float* dl, double* loss, hipStream_t st);
+int render_pt_frozen_rr(const bvh_state& bs, const vr_material* mats, const vr_material* rr_mats, size_t num_mats,
+                        const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp, float* accum,
+                        float* color, unsigned long long* ray_counts, uint32_t* work, hipStream_t st);
 int render_pt_batch(const void* d_scenes, const uint32_t* d_prefix, uint32_t num_scenes, uint32_t total_pixels,
                     uint32_t* d_frame, bool advance_frame, unsigned long long* ray_counts, uint32_t* work,
                     uint32_t grad_slots, double* grad, const vr_launch_shape& shape, hipStream_t st);
@@ -92,7 +95,7 @@ struct vr_batch_t {
     uint32_t total_pixels = 0;
     uint32_t total_materials = 0;
     uint32_t max_depth = 0;
-    vr_launch_shape shape{256, 16, 1};  // fixed at creation (the pixel prefix depends on the tile)
+    vr_launch_shape shape{256, 16, 1, VR_SCHEDULE_PERSISTENT};  // fixed at creation (the pixel prefix depends on the tile)
     void* d_scenes = nullptr;      // scene descriptors
     uint32_t* d_prefix = nullptr;  // num_scenes + 1
     uint32_t* d_frame = nullptr;   // frame counter (starts at -1, advanced before each frame)
@@ -584,6 +587,23 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
                              (hipStream_t)stream);
 }
 
+int vr_render_pt_frozen_rr(vr_bvh bvh, const vr_material* d_materials, const vr_material* d_rr_materials,
+                           size_t num_materials, const vr_kernel_params* kp, const vr_camera* cam,
+                           const vr_sched_params* sp, float* d_accum, float* d_color, uint64_t* d_ray_counts,
+                           void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_accum && d_color, VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_frozen_rr: null argument");
+    VR_CHECK(bvh->s.num_prims == 0 || (d_materials && d_rr_materials && num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_frozen_rr: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_frozen_rr: spp must be >= 1");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_pt_frozen_rr(bvh->s, d_materials, d_rr_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
+                               reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work,
+                               (hipStream_t)stream);
+}
+
 int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
 {
     VR_TRY_BEGIN
@@ -619,6 +639,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
     b->total_materials = (uint32_t)std::min<uint64_t>(mats, 0xFFFFFFFFu);
     b->max_depth = depth;
     b->shape = g_shape;
+    b->shape.schedule = VR_SCHEDULE_PERSISTENT;  // a batch is one persistent launch
     const uint32_t start = 0xFFFFFFFFu;
     if (hipMalloc(&b->d_scenes, descs.size()) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&b->d_prefix), prefix.size() * 4) != hipSuccess ||
@@ -655,7 +676,8 @@ int vr_set_launch_shape(const vr_launch_shape* shape)
 {
     VR_CHECK(shape, VR_ERR_INVALID_ARGUMENT, "vr_set_launch_shape: null argument");
     VR_CHECK(valid_shape(*shape), VR_ERR_INVALID_ARGUMENT,
-             "vr_set_launch_shape: block_size 128|256, stack_lds 8|16|32, tile 1..64");
+             "vr_set_launch_shape: block_size 128|256, stack_lds 8|16|32, tile 1..64, schedule persistent|per-pixel "
+             "(per-pixel with stack_lds 16)");
     g_shape = *shape;
     return VR_OK;
 }
@@ -689,7 +711,7 @@ int vr_autotune_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materi
     for (uint32_t b : blocks)
         for (uint32_t d : stacks)
             for (uint32_t t : tiles) {
-                const vr_launch_shape shp{b, d, t};
+                const vr_launch_shape shp{b, d, t, VR_SCHEDULE_PERSISTENT};
                 rc = render_pt_shaped(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color, nullptr,
                                       dev_state()->work, false, shp, st);
                 if (rc) break;

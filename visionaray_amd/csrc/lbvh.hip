@@ -17,6 +17,9 @@
 //      atomics per level per leaf) -- the boxes are the same exact min/max
 //      unions -- fused with the collapse to the reference bvh_node layout
 //      (lbvh.h:374-451) and the copy of triangles into traversal order.
+//      Nodes inside a 1024-leaf chunk meet in LDS, nodes inside a 65536-leaf
+//      block in one workgroup, the rest in a single workgroup: no
+//      device-scope fence is needed.
 #include "vr_internal.h"
 
 #include <algorithm>
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ c
         pairs[rc].e[2] = ((uint32_t)i << 1) | 1u;
     }
     if (i == 0) P.e[2] = 0xFFFFFFFFu;
-    P.e[3] = 0;
+    P.e[3] = (uint32_t)j;  // the range is [min(i, j), max(i, j)]
 }
 
 __device__ inline void store_box(float* f, f3 lo, f3 hi)
@@ -383,19 +386,63 @@ __device__ inline void store_ref_node(vr_bvh_node* o, f3 lo, f3 hi, uint32_t fir
     q[1] = make_uint4(__float_as_uint(hi.y), __float_as_uint(hi.z), first, num);
 }
 
-// Leaves in sorted order: copy the triangle into traversal order, store its
-// box in the parent's pair record and climb.  The first thread to arrive at a
-// node stops; the second sees both child boxes and continues with their
-// union, so every node box is written exactly once.
-__global__ __launch_bounds__(BLOCK) void k_bottom_up(const vr_triangle* __restrict__ prims, size_t n,
-                                                     const int32_t* __restrict__ sorted_ids,
-                                                     const uint32_t* __restrict__ leaf_parent,
-                                                     pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
-                                                     uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
-                                                     vr_bvh_node* __restrict__ ref_nodes,
-                                                     float* __restrict__ root_box, uint32_t* __restrict__ depth_out)
+// Bottom-up boxes.  The second thread to reach a node sees both child boxes,
+// stores them in the node's traversal record and climbs with their union, so
+// every box is computed exactly once.
+//
+// A Karras node covers a leaf range that has its own index as one end, so a
+// node whose range lies inside an aligned block of leaves has its index in
+// that block too.  Three passes, each finishing the nodes inside blocks of a
+// growing size with workgroup-scope ordering only (on gfx950 a device-scope
+// fence writes back the XCD's L2, far too slow to pay per node):
+//   1. k_bottom_up_local: one workgroup per CHUNK leaves, counters and child
+//      boxes in LDS;
+//   2. k_bottom_up_mid: one workgroup per SUPER leaves, in global memory;
+//   3. k_bottom_up_top: one workgroup for the rest.
+// A thread stops at the first node whose range leaves its pass's block: it
+// stores its box in that node's global record and counts the arrival; the
+// second arrival appends the node to the next pass's ready list, bucketed by
+// the block of the node's index.  Kernel boundaries make every earlier write
+// visible to the next pass.
+constexpr int CHUNK = 1024;
+constexpr int TOP_BLOCK = 1024;
+constexpr uint32_t SUPER = 64u * CHUNK;
+
+__device__ inline void publish_child(pair_node* pairs, uint32_t* heights, vr_bvh_node* ref_nodes, uint32_t p,
+                                     uint32_t side, f3 lo, f3 hi, uint32_t h, uint32_t ref_first,
+                                     uint32_t ref_num)
 {
-    const size_t j = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    store_box(&pairs[p].f[side * 6], lo, hi);
+    heights[2 * p + side] = h;
+    if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
+}
+
+__device__ inline void finish_root(float* root_box, vr_bvh_node* ref_nodes, uint32_t* depth_out, f3 lo, f3 hi,
+                                   uint32_t h)
+{
+    store_box(root_box, lo, hi);
+    if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, 1u, 0u);
+    *depth_out = h;
+}
+
+__global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __restrict__ prims, size_t n,
+                                                           const int32_t* __restrict__ sorted_ids,
+                                                           const uint32_t* __restrict__ leaf_parent,
+                                                           pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
+                                                           uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
+                                                           vr_bvh_node* __restrict__ ref_nodes,
+                                                           float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
+                                                           uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count)
+{
+    __shared__ float s_box[CHUNK * 12];
+    __shared__ uint32_t s_h[CHUNK * 2];
+    __shared__ uint32_t s_flag[CHUNK];
+    const uint32_t tid = threadIdx.x;
+    const size_t c0 = (size_t)blockIdx.x * CHUNK;
+    const size_t c1 = min(n, c0 + (size_t)CHUNK);
+    s_flag[tid] = 0;
+    __syncthreads();
+    const size_t j = c0 + tid;
     if (j >= n) return;
     const int32_t id = sorted_ids[j];
     const tri_in t = load_tri(prims, (size_t)id);
@@ -412,42 +459,122 @@ __global__ __launch_bounds__(BLOCK) void k_bottom_up(const vr_triangle* __restri
         *depth_out = 1;
         return;
     }
-    uint32_t link = leaf_parent[j];
     const uint32_t npairs = (uint32_t)(n - 1);
-    if ((link >> 1) >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
-    uint32_t p = link >> 1, side = link & 1u;
-    store_box(&pairs[p].f[side * 6], lo, hi);
-    heights[2 * p + side] = 1;
-    if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, (uint32_t)id, 1u);
+    uint32_t link = leaf_parent[j];
+    uint32_t h = 1, ref_first = (uint32_t)id, ref_num = 1;
     // a Karras tree has at most 65 levels; the bound only guards against a
     // corrupt hierarchy
     for (int level = 0; level < 128; ++level) {
-        __threadfence();
-        const uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0) return;
-        __threadfence();
-        const volatile float* f = pairs[p].f;
-        const f3 lo0 = mk3(f[0], f[1], f[2]), hi0 = mk3(f[3], f[4], f[5]);
-        const f3 lo1 = mk3(f[6], f[7], f[8]), hi1 = mk3(f[9], f[10], f[11]);
-        lo = min3v(lo0, lo1);
-        hi = max3v(hi0, hi1);
-        const volatile uint32_t* hv = heights;
-        const uint32_t h = max(hv[2 * p], hv[2 * p + 1]) + 1u;
+        const uint32_t p = link >> 1, side = link & 1u;
+        if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
+        const uint32_t other = pairs[p].e[3];
+        if (!(p >= c0 && p < c1 && other >= c0 && other < c1)) {
+            publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
+            if (atomicAdd(&flags[p], 1u) == 1u) {
+                const uint32_t b = p / SUPER;
+                ready[(size_t)b * SUPER + atomicAdd(&bucket_count[b], 1u)] = p;
+            }
+            return;
+        }
+        const uint32_t k = p - (uint32_t)c0;
+        if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
+        store_box(&s_box[k * 12 + side * 6], lo, hi);
+        s_h[2 * k + side] = h;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&s_flag[k], 1u) == 0u) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        float b[12];
+        for (int q = 0; q < 12; ++q) b[q] = s_box[k * 12 + q];
+        float4* dst = reinterpret_cast<float4*>(pairs[p].f);
+        dst[0] = make_float4(b[0], b[1], b[2], b[3]);
+        dst[1] = make_float4(b[4], b[5], b[6], b[7]);
+        dst[2] = make_float4(b[8], b[9], b[10], b[11]);
+        lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
+        hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
+        h = max(s_h[2 * k], s_h[2 * k + 1]) + 1u;
+        ref_first = 1u + 2u * p;
+        ref_num = 0;
         if (p == 0) {
-            store_box(root_box, lo, hi);
-            if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, 1u, 0u);
-            *depth_out = h;
+            finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
             return;
         }
         link = pairs[p].e[2];
-        const uint32_t pp = link >> 1;
+    }
+}
+
+// Climb from node p, whose children are both published, until an arrival
+// is first at its node, the root is done, or (when the block [c0, c1) is not
+// the whole tree) the next node leaves the block: that node's arrival is
+// counted for the next pass and, as the second, appended to its ready list.
+__device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size_t c1, bool whole,
+                                    pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
+                                    uint32_t* __restrict__ heights, vr_bvh_node* __restrict__ ref_nodes,
+                                    float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
+                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count)
+{
+    for (int level = 0; level < 128 && p < npairs; ++level) {
+        const float4* src = reinterpret_cast<const float4*>(pairs[p].f);
+        const float4 q0 = src[0], q1 = src[1], q2 = src[2];
+        const f3 lo = min3v(mk3(q0.x, q0.y, q0.z), mk3(q1.z, q1.w, q2.x));
+        const f3 hi = max3v(mk3(q0.w, q1.x, q1.y), mk3(q2.y, q2.z, q2.w));
+        const uint32_t h = max(heights[2 * p], heights[2 * p + 1]) + 1u;
+        if (p == 0) {
+            finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
+            return;
+        }
+        const uint32_t link = pairs[p].e[2];
+        const uint32_t pp = link >> 1, side = link & 1u;
         if (pp >= npairs) return;
-        side = link & 1u;
-        store_box(&pairs[pp].f[side * 6], lo, hi);
-        heights[2 * pp + side] = h;
-        if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * pp + side], lo, hi, 1u + 2u * p, 0u);
+        publish_child(pairs, heights, ref_nodes, pp, side, lo, hi, h, 1u + 2u * p, 0u);
+        if (!whole) {
+            const uint32_t other = pairs[pp].e[3];
+            if (!(pp >= c0 && pp < c1 && other >= c0 && other < c1)) {
+                if (atomicAdd(&flags[pp], 1u) == 1u) next[atomicAdd(next_count, 1u)] = pp;
+                return;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (__hip_atomic_fetch_add(&flags[pp], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         p = pp;
     }
+}
+
+// pass 2: workgroup b finishes the nodes inside leaves [b * SUPER, (b + 1) * SUPER)
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, pair_node* __restrict__ pairs,
+                                                             uint32_t* __restrict__ flags,
+                                                             uint32_t* __restrict__ heights,
+                                                             vr_bvh_node* __restrict__ ref_nodes,
+                                                             float* __restrict__ root_box,
+                                                             uint32_t* __restrict__ depth_out,
+                                                             const uint32_t* __restrict__ ready,
+                                                             const uint32_t* __restrict__ bucket_count,
+                                                             uint32_t* __restrict__ top, uint32_t* __restrict__ top_count)
+{
+    const uint32_t npairs = (uint32_t)(n - 1);
+    const size_t c0 = (size_t)blockIdx.x * SUPER;
+    const size_t c1 = min(n, c0 + (size_t)SUPER);
+    const uint32_t count = min(bucket_count[blockIdx.x], SUPER);
+    for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
+        climb_global(ready[c0 + i], npairs, c0, c1, false, pairs, flags, heights, ref_nodes, root_box, depth_out,
+                     top, top_count);
+}
+
+// pass 3: one workgroup finishes the nodes that span SUPER blocks
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, pair_node* __restrict__ pairs,
+                                                             uint32_t* __restrict__ flags,
+                                                             uint32_t* __restrict__ heights,
+                                                             vr_bvh_node* __restrict__ ref_nodes,
+                                                             float* __restrict__ root_box,
+                                                             uint32_t* __restrict__ depth_out,
+                                                             const uint32_t* __restrict__ top,
+                                                             const uint32_t* __restrict__ top_count)
+{
+    const uint32_t npairs = (uint32_t)(n - 1);
+    const uint32_t count = min(*top_count, npairs);
+    for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
+        climb_global(top[i], npairs, 0, n, true, pairs, flags, heights, ref_nodes, root_box, depth_out, nullptr,
+                     nullptr);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
@@ -486,6 +613,8 @@ struct build_scratch {
     uint32_t* sums2;
     uint32_t* cb;     // 6 words
     uint32_t* leaf_parent;
+    uint32_t* counts;  // bottom-up ready-list counters: one per SUPER block, then the top list's
+                       // (the ready lists reuse keys0 and vals0 after the sort)
     size_t bytes;
 };
 
@@ -513,6 +642,7 @@ build_scratch plan_scratch(size_t n, char* base)
     s.sums2 = take(1);
     s.cb = take(8);
     s.leaf_parent = take(n);
+    s.counts = take((n + SUPER - 1) / SUPER + 1);
     s.bytes = off;
     return s;
 }
@@ -546,6 +676,23 @@ int launch_morton(const vr_triangle* prims, size_t n, uint32_t* cb, uint32_t* co
     hipLaunchKernelGGL(k_morton, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, prims, n, cb, codes);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
+}
+
+void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* ref_nodes, hipStream_t st)
+{
+    const size_t n = bs.num_prims;
+    const size_t nb = (n + SUPER - 1) / SUPER;
+    uint32_t* depth = reinterpret_cast<uint32_t*>(bs.root_box + 6);
+    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
+    hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, bs.prims, n, bs.sorted_ids,
+                       s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
+                       s.keys0, s.counts);
+    if (n > (size_t)CHUNK)
+        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags,
+                           bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb);
+    if (n > (size_t)SUPER)
+        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags, bs.heights,
+                           ref_nodes, bs.root_box, depth, s.vals0, s.counts + nb);
 }
 
 }  // namespace
@@ -623,10 +770,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags, npairs, 0u);
     }
     // 5: boxes, traversal-order triangles, reference layout
-    hipLaunchKernelGGL(k_bottom_up, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.prims, n,
-                       bs.sorted_ids, s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris,
-                       keep_nodes ? bs.ref_nodes : nullptr, bs.root_box,
-                       reinterpret_cast<uint32_t*>(bs.root_box + 6));
+    launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, st);
     hipLaunchKernelGGL(k_iota_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.indices, n);
     VR_HIP_CHECK(hipGetLastError());
 
@@ -642,17 +786,12 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     if (n == 0) return VR_OK;
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
     VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
-    if (n > 1) {
-        // leaf parents are recomputed from the unchanged hierarchy
-        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, s.leaf_parent, n, 0xFFFFFFFFu);
-        hipLaunchKernelGGL(k_karras, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
-                           bs.pairs, s.leaf_parent);
+    // the hierarchy (child entries, parent links, ranges, leaf parents) is
+    // unchanged since the build; only the boxes are recomputed
+    if (n > 1)
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags,
                            bs.num_pairs, 0u);
-    }
-    hipLaunchKernelGGL(k_bottom_up, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.prims, n, bs.sorted_ids,
-                       s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, bs.ref_nodes, bs.root_box,
-                       reinterpret_cast<uint32_t*>(bs.root_box + 6));
+    launch_bottom_up(bs, s, bs.ref_nodes, st);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }

@@ -139,6 +139,7 @@ struct pt_args {
     uint32_t tile;        // pixel queue order (tile x tile squares; <= 1: scanlines)
     uint32_t grad_slots;  // adjoint: materials covered by the LDS partials
     double* grad;         // adjoint: 6 per material slot (d/d cd rgb, d/d ce rgb), accumulated
+    const vr_material* rr_mats;  // PT_FROZEN_RR: materials whose throughput sets the roulette
 };
 
 __device__ inline void write_pixel(const scene_desc& S, uint32_t px, float4 c)
@@ -178,7 +179,7 @@ __device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t
     return (uint32_t)x < w && (uint32_t)y < h;
 }
 
-enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2 };
+enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_PLAIN = 3, PT_FROZEN_RR = 4 };
 constexpr int ADJ_MAX_VERTICES = 16;
 
 // One kernel for the frame (PT_RENDER), the frame with traversal counters
@@ -195,6 +196,14 @@ constexpr int ADJ_MAX_VERTICES = 16;
 //   d I_c / d ce_{e,c} = T_c * ls_e.
 // Per-block FP32 partials in LDS; FP64 only in the final reduction.
 //
+// PT_PLAIN renders the frame as the reference's cuda_sched launches it: one
+// thread per pixel over the whole image and no work queue (a lane whose pixel
+// is done idles).  PT_FROZEN_RR renders the frame with the roulette
+// probabilities taken from the throughput under A.rr_mats instead of the
+// shading materials: with rr_mats = mats it is the frame itself, and for
+// mats near rr_mats it is the estimator the adjoint differentiates (central
+// differences of it at common random numbers keep every roulette decision).
+//
 // BATCH: many scenes in one persistent launch.  The global pixel index runs
 // over all scenes back to back; a lane finds its scene by binary search over
 // the pixel prefix and reads that scene's descriptor.
@@ -206,6 +215,8 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = MODE == PT_COUNT;
     constexpr bool ADJ = MODE == PT_ADJOINT;
+    constexpr bool PLAIN = MODE == PT_PLAIN;
+    constexpr bool FROZEN = MODE == PT_FROZEN_RR;
     __shared__ uint32_t stack[LDSD * BLK];
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
@@ -225,6 +236,8 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
     uint32_t s = 0, bounce = 0;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
     f3 tp = mk3(1, 1, 1), I = mk3(0, 0, 0);
+    f3 tp0 = mk3(1, 1, 1);  // FROZEN: throughput under A.rr_mats
+    bool fetched = false;   // PLAIN: the lane took its one pixel
     bool rhit = false;
     float4 sum = make_float4(0, 0, 0, 0);
     unsigned long long n_primary = 0, n_secondary = 0;
@@ -239,12 +252,20 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
         const bool need = !done && px < 0;
         const uint64_t needm = __ballot(need);
         if (needm) {
-            const int leader = __ffsll((unsigned long long)needm) - 1;
             uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(needm));
-            base = __shfl(base, leader);
+            if (!PLAIN) {
+                const int leader = __ffsll((unsigned long long)needm) - 1;
+                if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(needm));
+                base = __shfl(base, leader);
+            }
             if (need) {
-                const uint32_t my = base + (uint32_t)lane_mask_lt_count(needm);
+                uint32_t my;
+                if (PLAIN) {
+                    my = fetched ? npix : blockIdx.x * (uint32_t)BLK + threadIdx.x;
+                    fetched = true;
+                } else {
+                    my = base + (uint32_t)lane_mask_lt_count(needm);
+                }
                 if (my < npix) {
                     uint32_t lpx = my;
                     if (BATCH) {
@@ -269,6 +290,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                         sum = make_float4(0, 0, 0, 0);
                         primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
                         tp = mk3(1, 1, 1);
+                        tp0 = mk3(1, 1, 1);
                         I = mk3(0, 0, 0);
                         bounce = 0;
                         rhit = false;
@@ -326,7 +348,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
             const vr_material* m = S.mats + mid;
             f3 refl = mk3(0.0f, 0.0f, 0.0f);
             float pdf;
-            f3 src;
+            f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
             const bool emission = m->type == 1;
             if (emission) {
                 // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
@@ -347,6 +369,13 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                 src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
                           1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
                           1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
+                if (FROZEN) {
+                    const vr_material* m0 = A.rr_mats + mid;
+                    const float kd0 = m0->kd;
+                    src0 = mk3(1.0f * ((m0->cd[0] * kd0) * 3.18309886183790691216444201928e-01f),
+                               1.0f * ((m0->cd[1] * kd0) * 3.18309886183790691216444201928e-01f),
+                               1.0f * ((m0->cd[2] * kd0) * 3.18309886183790691216444201928e-01f));
+                }
             }
             const bool zero_pdf = pdf <= 0.0f;
             bool active = true;
@@ -377,14 +406,20 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
             const float q = dot3(n, refl) / pdf;
             tp = mk3(tp.x * (src.x * q), tp.y * (src.y * q), tp.z * (src.z * q));
             if (zero_pdf) tp = mk3(0.0f, 0.0f, 0.0f);
+            if (FROZEN) {
+                tp0 = emission ? tp : mk3(tp0.x * (src0.x * q), tp0.y * (src0.y * q), tp0.z * (src0.z * q));
+                if (zero_pdf) tp0 = mk3(0.0f, 0.0f, 0.0f);
+            }
             if (bounce >= 2) {
                 // Russian roulette (pathtracing.inl:201-213)
-                float prob = tp.x;
-                if (tp.y > prob) prob = tp.y;
-                if (tp.z > prob) prob = tp.z;
+                const f3 rt = FROZEN ? tp0 : tp;
+                float prob = rt.x;
+                if (rt.y > prob) prob = rt.y;
+                if (rt.z > prob) prob = rt.z;
                 const bool terminate = rng_next(g) > prob;
                 if (terminate) active = false;
                 tp = mk3(tp.x / prob, tp.y / prob, tp.z / prob);
+                if (FROZEN) tp0 = mk3(tp0.x / prob, tp0.y / prob, tp0.z / prob);
             }
             if (!active) {
                 finished = true;
@@ -413,6 +448,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                 const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
                 primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
                 tp = mk3(1, 1, 1);
+                tp0 = mk3(1, 1, 1);
                 I = mk3(0, 0, 0);
                 bounce = 0;
                 rhit = false;
@@ -535,12 +571,24 @@ int launch_variant(const pt_args& A, size_t dyn_lds, hipStream_t st)
     return VR_OK;
 }
 
+// One thread per pixel (the reference's cuda_sched): a grid over the queue.
+template <int BLK>
+int launch_plain(const pt_args& A, hipStream_t st)
+{
+    const uint32_t npix = padded_pixels(A.single.sp.width, A.single.sp.height, A.tile);
+    hipLaunchKernelGGL((k_pathtrace<PT_PLAIN, false, BLK, LDS_STACK>), dim3(div_up(npix, BLK)), dim3(BLK), 0, st, A);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
 // The frame kernel is compiled for 2 block sizes x 3 LDS stack depths; the
-// counting and adjoint variants use the default shape.
+// counting, adjoint and frozen-roulette variants use the default shape.
 template <int MODE, bool BATCH>
 int launch_pt(const pt_args& A, size_t dyn_lds, const vr_launch_shape& sh, hipStream_t st)
 {
     if (MODE != PT_RENDER) return launch_variant<MODE, BATCH, 256, 16>(A, dyn_lds, st);
+    if (!BATCH && sh.schedule == VR_SCHEDULE_PER_PIXEL)
+        return sh.block_size == 128 ? launch_plain<128>(A, st) : launch_plain<256>(A, st);
     switch (sh.block_size * 100 + sh.stack_lds) {
     case 12808: return launch_variant<MODE, BATCH, 128, 8>(A, dyn_lds, st);
     case 12816: return launch_variant<MODE, BATCH, 128, 16>(A, dyn_lds, st);
@@ -592,13 +640,14 @@ pt_args single_args(const bvh_state& bs, const vr_material* mats, size_t num_mat
 uint32_t g_arg_order = 1;
 
 // current launch shape of the frame kernel (vr_set_launch_shape / autotuner)
-vr_launch_shape g_shape = {256, 16, 1};
+vr_launch_shape g_shape = {256, 16, 1, VR_SCHEDULE_PERSISTENT};
 
 bool valid_shape(const vr_launch_shape& s)
 {
     const bool b = s.block_size == 128 || s.block_size == 256;
     const bool d = s.stack_lds == 8 || s.stack_lds == 16 || s.stack_lds == 32;
-    return b && d && s.tile >= 1 && s.tile <= 64;
+    const bool q = s.schedule == VR_SCHEDULE_PERSISTENT || (s.schedule == VR_SCHEDULE_PER_PIXEL && s.stack_lds == 16);
+    return b && d && q && s.tile >= 1 && s.tile <= 64;
 }
 
 // largest material table the adjoint keeps in LDS partials
@@ -631,6 +680,22 @@ int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_ma
     A.tile = shape.tile;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
     return counted ? launch_pt<PT_COUNT, false>(A, 0, shape, st) : launch_pt<PT_RENDER, false>(A, 0, shape, st);
+}
+
+int render_pt_frozen_rr(const bvh_state& bs, const vr_material* mats, const vr_material* rr_mats, size_t num_mats,
+                        const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp, float* accum,
+                        float* color, unsigned long long* ray_counts, uint32_t* work, hipStream_t st)
+{
+    if ((size_t)sp.width * sp.height == 0) return VR_OK;
+    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_frozen_rr: BVH deeper than the stack");
+    VR_CHECK((uint64_t)padded_pixels(sp.width, sp.height, g_shape.tile) < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "render_pt_frozen_rr: image too large");
+    pt_args A =
+        single_args(bs, mats, num_mats, kp, cam, sp, accum, color, nullptr, ray_counts, work, nullptr, g_arg_order);
+    A.tile = g_shape.tile;
+    A.rr_mats = rr_mats;
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
+    return launch_pt<PT_FROZEN_RR, false>(A, 0, g_shape, st);
 }
 
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,

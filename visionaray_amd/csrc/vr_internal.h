@@ -44,7 +44,8 @@ int fail(int code, const std::string& msg);
 //   f[0..5]   child 0 box  lo.x lo.y lo.z hi.x hi.y hi.z
 //   f[6..11]  child 1 box
 //   e[0], e[1] child entries (see below); e[2] parent pair (-1 at the root
-//   pair), e[3] spare.
+//   pair); e[3] for an LBVH the far end j of the pair's leaf range
+//   [min(index, j), max(index, j)], unused otherwise.
 // Child entry: bit 31 clear -> inner node, the index of its pair record;
 // bit 31 set -> leaf covering triangles [first, first + count) with
 // first = entry & 0x03FFFFFF and count = ((entry >> 26) & 31) + 1.

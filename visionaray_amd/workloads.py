@@ -38,24 +38,40 @@ def bench_batched(args, V, S, log):
     log(f"c2: {count} scenes, {total_tris} triangles, prepared in {time.time() - t0:.1f}s")
     steps = max(1, args.steps)
 
-    # one plain launch per scene (cuda_sched::frame per scene)
-    sched = V.hip_sched()
-    for k, sp in scenes:
-        sched.frame(k, sp, stream=st)
-    st.synchronize()
-    sched.reset_counters()
-    e0, e1 = _events(V)
-    e0.record(st)
-    for _ in range(steps):
+    def per_scene(shape):
+        """one frame() launch per scene under a launch shape; (ms, rays) per step"""
+        V.set_launch_shape(**shape)
+        sched = V.hip_sched()
         for k, sp in scenes:
             sched.frame(k, sp, stream=st)
-    e1.record(st)
-    plain_ms = e0.elapsed_ms(e1) / steps
-    p0, s0 = sched.rays()
-    plain_rays = (p0 + s0) / steps
+        st.synchronize()
+        sched.reset_counters()
+        e0, e1 = _events(V)
+        e0.record(st)
+        for _ in range(steps):
+            for k, sp in scenes:
+                sched.frame(k, sp, stream=st)
+        e1.record(st)
+        ms = e0.elapsed_ms(e1) / steps
+        p0, s0 = sched.rays()
+        return ms, (p0 + s0) / steps
 
-    # one persistent launch for all scenes, captured in a graph
-    b = V.batch(scenes)
+    # plain launches: one thread per pixel in 16 x 16 pixel blocks, one
+        # launch per scene (cuda_sched::frame per scene)
+        plain_ms, plain_rays = per_scene(dict(block_size=256, stack_lds=16, tile=16,
+                                              schedule=V.SCHEDULE_PER_PIXEL))
+        # the same launches on the persistent scheduler
+        pers_ms, pers_rays = per_scene(dict(block_size=256, stack_lds=16, tile=1))
+        # autotune the persistent shape on the scene with the most pixels x triangles
+        big = max(range(len(scenes)),
+                  key=lambda i: scenes[i][1].rt.width * scenes[i][1].rt.height * scenes[i][0].bvh.info()["num_prims"])
+        shape, tuned_ms = V.autotune(scenes[big][0], scenes[big][1], reps=2, stream=st)
+        log(f"c2: autotuned shape {shape} ({tuned_ms:.2f} ms on scene {big})")
+
+        # one persistent launch for all scenes, captured in a graph
+        b = V.batch(scenes)
+    finally:
+        pass
     b.render(st)
     b.capture_graph(st)
     b.launch_graph(st)
@@ -69,21 +85,27 @@ def bench_batched(args, V, S, log):
     batch_ms = e0.elapsed_ms(e1) / steps
     p1, s1 = b.rays()
     batch_rays = (p1 + s1) / steps
+    V.set_launch_shape(256, 16, 1)
     plain_rate = plain_rays / (plain_ms * 1e-3) / 1e6
+    pers_rate = pers_rays / (pers_ms * 1e-3) / 1e6
     batch_rate = batch_rays / (batch_ms * 1e-3) / 1e6
-    log(f"c2: per-scene launches {plain_ms:.2f} ms ({plain_rate:.0f} Mrays/s), batched graph {batch_ms:.2f} ms "
-        f"({batch_rate:.0f} Mrays/s), x{batch_rate / plain_rate:.2f}")
+    log(f"c2: per-scene plain launches {plain_ms:.2f} ms ({plain_rate:.0f} Mrays/s), per-scene persistent "
+        f"{pers_ms:.2f} ms ({pers_rate:.0f} Mrays/s), batched graph {batch_ms:.2f} ms ({batch_rate:.0f} Mrays/s): "
+        f"x{batch_rate / plain_rate:.2f} vs plain, x{batch_rate / pers_rate:.2f} vs persistent per scene")
     return {
-        "metric": "batched aggregate Mrays/s vs per-scene launches",
+        "metric": "batched aggregate Mrays/s vs per-scene plain launches",
         "value": round(batch_rate, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": round(batch_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"C2 batched: {count} scenes, {total_tris} triangles (1k-2M log-uniform), "
                                "resolutions from {256^2, 512^2, 1024^2, 1920x1080}, 4 spp, depth 4",
-                   "scenes": count, "triangles": int(total_tris)},
-        "per_scene_launches": {"ms": round(plain_ms, 4), "mrays_per_s": round(plain_rate, 3)},
+                   "scenes": count, "triangles": int(total_tris), "batch_shape": shape},
+        "per_scene_plain_launches": {"ms": round(plain_ms, 4), "mrays_per_s": round(plain_rate, 3),
+                                     "schedule": "one thread per pixel, 16x16 pixel blocks (cuda_sched)"},
+        "per_scene_persistent_launches": {"ms": round(pers_ms, 4), "mrays_per_s": round(pers_rate, 3)},
         "batched_graph": {"ms": round(batch_ms, 4), "mrays_per_s": round(batch_rate, 3)},
         "speedup": round(batch_rate / plain_rate, 3),
+        "speedup_vs_persistent_per_scene": round(batch_rate / pers_rate, 3),
         "rays_per_frame": int(batch_rays),
     }
 
@@ -122,9 +144,14 @@ def bench_adjoint(args, V, S, log):
     loss = V.DeviceArray(1, np.float64, zero=True)
     lib = V.lib()
 
-    def forward_loss(kernel):
+    d_rr = V.DeviceArray.from_host(mats)  # the roulette table held at the unperturbed point
+
+    def forward_loss(kernel, frozen=False):
         sched.frame_id = 0
-        sched.frame(kernel, sp, stream=st)
+        if frozen:
+            sched.frame_frozen_rr(kernel, d_rr, sp, stream=st)
+        else:
+            sched.frame(kernel, sp, stream=st)
         V.check(lib.vr_memset(loss.ptr, 0, 8))
         V.check(lib.vr_image_l2_loss(rt.color.ptr, target.ptr, width * height, dl.ptr, loss.ptr, st.handle))
 
@@ -149,37 +176,57 @@ def bench_adjoint(args, V, S, log):
     ga = np.concatenate([g[:, 0:3].ravel(), g[mats["type"] == 1, 3:6].ravel()])
     log(f"c3: loss {L0:.6f}; adjoint (forward + loss + backward) {adjoint_ms:.2f} ms, forward {forward_ms:.2f} ms")
 
-    # central differences with common random numbers, h = 1e-3
+    # central differences with common random numbers, h = 1e-3.  The check
+    # differentiates the frozen-roulette estimator (vr_render_pt_frozen_rr:
+    # roulette probabilities from the throughput at the unperturbed albedos,
+    # so every roulette decision is common to both sides); its expectation is
+    # the image's for any albedos, and its derivative is what the adjoint
+    # computes.  A subset is also differenced with the plain frame, whose
+    # roulette decisions flip under the perturbation (noise ~ 1/sqrt(h)).
     hstep = 1e-3
     params = [(m, "cd", c) for m in range(mats.size) for c in range(3)] + \
              [(m, "ce", c) for m in np.nonzero(mats["type"] == 1)[0] for c in range(3)]
     nfd = len(params) if not args.fd_params else min(args.fd_params, len(params))
-    fd = np.zeros(len(params))
     d_pert = V.DeviceArray(mats.size, mats.dtype)
     kpert = V.pathtracing.kernel(bvh, d_pert, kp)
-    t0 = time.perf_counter()
-    e0, e1 = _events(V)
-    e0.record(st)
-    for i, (m, key, c) in enumerate(params[:nfd]):
+
+    def central(i, frozen):
+        m, key, c = params[i]
         vals = []
         for sgn in (1.0, -1.0):
             mm = mats.copy()
             mm[key][m, c] += np.float32(sgn * hstep)
             d_pert.upload(mm)
-            forward_loss(kpert)
+            forward_loss(kpert, frozen)
             vals.append(float(loss.to_host()[0]))
-        fd[i] = (vals[0] - vals[1]) / (2 * hstep)
+        return (vals[0] - vals[1]) / (2 * hstep)
+
+    fd = np.zeros(len(params))
+    t0 = time.perf_counter()
+    e0, e1 = _events(V)
+    e0.record(st)
+    for i in range(nfd):
+        fd[i] = central(i, True)
         if i % 100 == 0:
             log(f"c3: finite differences {i}/{nfd}")
     e1.record(st)
     fd_ms = e0.elapsed_ms(e1)
     fd_wall = time.perf_counter() - t0
     fd_ms_all = fd_ms * len(params) / nfd
-    sel = slice(0, nfd)
-    cos = float(ga[sel] @ fd[sel] / (np.linalg.norm(ga[sel]) * np.linalg.norm(fd[sel]) + 1e-300))
+
+    def cosine(x, y):
+        return float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y) + 1e-300))
+
+    cos = cosine(ga[:nfd], fd[:nfd])
     speedup = fd_ms_all / adjoint_ms
     log(f"c3: FD {nfd} params in {fd_ms:.0f} ms (wall {fd_wall:.1f}s) -> all {fd_ms_all:.0f} ms; "
-        f"speedup x{speedup:.1f}, cosine {cos:.4f}")
+        f"speedup x{speedup:.1f}, cosine {cos:.6f} (frozen-roulette FD)")
+    sub = np.arange(0, nfd, max(1, nfd // 96))
+    fd_plain = np.array([central(int(i), False) for i in sub])
+    cos_plain = cosine(ga[sub], fd_plain)
+    cos_sub = cosine(ga[sub], fd[sub])
+    log(f"c3: on {sub.size} params: cosine {cos_plain:.4f} vs FD of the plain frame (roulette decisions flip), "
+        f"{cos_sub:.6f} vs frozen-roulette FD")
 
     # FP64 CPU adjoint on a band of rows (the image gradient masked to it)
     fp64 = None
@@ -223,7 +270,12 @@ def bench_adjoint(args, V, S, log):
                                f"{width}x{height}, {spp} spp, depth {depth}, L2 loss to perturbed-albedo target",
                    "params": len(params), "fd_params_timed": nfd, "h": hstep},
         "adjoint_ms": round(adjoint_ms, 4), "forward_ms": round(forward_ms, 4),
-        "fd_ms_all_params": round(fd_ms_all, 2), "cosine_vs_fd": round(cos, 6), "loss": L0,
+        "fd_ms_all_params": round(fd_ms_all, 2), "cosine_vs_fd": round(cos, 6),
+        "fd_estimator": "central, h=1e-3, common random numbers and common roulette decisions "
+                        "(vr_render_pt_frozen_rr at the unperturbed albedos)",
+        "plain_frame_fd_subset": {"params": int(sub.size), "cosine": round(cos_plain, 6),
+                                  "cosine_frozen_same_subset": round(cos_sub, 6)},
+        "loss": L0,
         "fp64_cpu_adjoint": fp64,
     }
 
