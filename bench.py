@@ -30,7 +30,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "incoherent Mrays/s @4M tris; LBVH build ms; adjoint vs FD speedup"
 HBM_PEAK_GBS = 8000.0
-RAY_IN, HIT_OUT, PAIR_BYTES, TRI_BYTES = 32, 16, 64, 48
+# algorithmic bytes of one query: ray in, hit out, one 128-byte line per
+# traversal-node visit, 48 bytes per triangle test
+RAY_IN, HIT_OUT, NODE_BYTES, TRI_BYTES = 32, 16, 128, 48
 
 
 def log(msg: str):
@@ -156,9 +158,9 @@ def main():
     stream.synchronize()
     st = stats.to_host().astype(np.float64)
     rays_frame = st[0] + st[1]
-    alg_bytes = rays_frame * (RAY_IN + HIT_OUT) + st[2] * PAIR_BYTES + st[3] * TRI_BYTES
+    alg_bytes = rays_frame * (RAY_IN + HIT_OUT) + st[2] * NODE_BYTES + st[3] * TRI_BYTES
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    log(f"per ray: {st[2] / rays_frame:.2f} node pairs, {st[3] / rays_frame:.2f} triangles; "
+    log(f"per ray: {st[2] / rays_frame:.2f} node visits, {st[3] / rays_frame:.2f} triangles; "
         f"{alg_bytes / 1e9:.2f} GB algorithmic per frame -> {achieved:.0f} GB/s")
 
     traffic = None
@@ -209,7 +211,7 @@ def main():
             "kernel": "k_pathtrace",
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": int(alg_bytes),
-            "per_ray": {"node_pairs": round(st[2] / rays_frame, 3), "triangles": round(st[3] / rays_frame, 3)},
+            "per_ray": {"nodes": round(st[2] / rays_frame, 3), "triangles": round(st[3] / rays_frame, 3)},
         },
         "cpu_baseline": cpu,
     }
@@ -241,9 +243,9 @@ def profile_pass(args, V, S, tris, d_tris, d_mats, stream):
     stream.synchronize()
     st = stats.to_host().astype(np.float64)
     rays = st[0] + st[1]
-    alg = rays * (RAY_IN + HIT_OUT) + st[2] * PAIR_BYTES + st[3] * TRI_BYTES
+    alg = rays * (RAY_IN + HIT_OUT) + st[2] * NODE_BYTES + st[3] * TRI_BYTES
     print(json.dumps({"profile_pass": True, "copy_bytes": nbytes, "pathtrace_algorithmic_bytes": int(alg),
-                      "rays": int(rays), "node_pairs": int(st[2]), "triangles": int(st[3])}), flush=True)
+                      "rays": int(rays), "nodes": int(st[2]), "triangles": int(st[3])}), flush=True)
 
 
 def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):

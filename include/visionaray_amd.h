@@ -205,8 +205,8 @@ int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, 
  * intersect<ClosestHit|AnyHit>, detail/bvh/intersect.inl:41-357). */
 int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
 
-/* Same, also counting per ray the node-pair and triangle tests
- * (d_counts[2 * i] = pairs, d_counts[2 * i + 1] = triangles); for the
+/* Same, also counting per ray the traversal-node visits and triangle tests
+ * (d_counts[2 * i] = nodes, d_counts[2 * i + 1] = triangles); for the
  * algorithmic-bytes model. */
 int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits,
                      uint32_t* d_counts, void* stream);
@@ -229,7 +229,7 @@ int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_material
 
 /* vr_render_pt with per-frame traversal statistics for the algorithmic-bytes
  * model: d_stats (4 x uint64, accumulated) = primary queries, secondary
- * queries, node-pair tests, triangle tests.  Same image as vr_render_pt. */
+ * queries, traversal-node visits, triangle tests.  Same image as vr_render_pt. */
 int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
                          const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                          float* d_accum, float* d_color, uint64_t* d_stats, void* stream);

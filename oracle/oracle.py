@@ -60,6 +60,7 @@ def lib():
         l.vro_determine_range.argtypes = [P, C.c_int, C.c_int, P, P, P]
         l.vro_trace.argtypes = [P, P, P, SZ, P, SZ, C.c_int, P]
         l.vro_trace_brute.argtypes = [P, SZ, P, SZ, C.c_int, P]
+        l.vro_intersect_tri.argtypes = [P, P, P, P, P, P]
         l.vro_rng_stream.argtypes = [C.c_uint32, SZ, P]
         l.vro_refit.argtypes = [P, SZ, P, P]
         l.vro_render_ao.argtypes = [P, P, P, SZ, P, P, P, P, C.c_int, C.c_int]
@@ -151,6 +152,52 @@ def trace_brute(tris, rays, any_hit=False) -> np.ndarray:
     hits = np.zeros(rays.size, HIT_DTYPE)
     lib().vro_trace_brute(_p(tris), tris.size, _p(rays), rays.size, int(any_hit), _p(hits))
     return hits
+
+
+def intersect(tris, rays, which) -> np.ndarray:
+    """Ray i against triangle tris[which[i]] alone (math/intersect.h:176-229);
+    prim_id is which[i] on a hit, -1 otherwise."""
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = np.zeros(rays.size, HIT_DTYPE)
+    out["prim_id"] = -1
+    hit, t, u, v = C.c_int(), C.c_float(), C.c_float(), C.c_float()
+    for i, k in enumerate(np.asarray(which)):
+        lib().vro_intersect_tri(C.c_void_p(rays.ctypes.data + i * RAY_DTYPE.itemsize),
+                                C.c_void_p(tris.ctypes.data + int(k) * TRI_DTYPE.itemsize),
+                                C.byref(hit), C.byref(t), C.byref(u), C.byref(v))
+        if hit.value:
+            out[i] = (int(k), t.value, u.value, v.value)
+    return out
+
+
+def check_hits(got, want, tris, rays, any_hit=False) -> int:
+    """Compare hit records of another traversal with the oracle's under the
+    closest-hit contract: the same rays hit, at bit-identical t; where the
+    reported triangle differs, it must be hit at exactly that t (a tie between
+    triangles, resolved by visit order).  Any-hit: the same rays hit, each at
+    a real intersection inside [tmin, tmax].  Returns the number of ties;
+    raises AssertionError otherwise."""
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    index = {int(p): i for i, p in enumerate(tris["prim_id"])}
+    gh, wh = got["prim_id"] >= 0, want["prim_id"] >= 0
+    assert np.array_equal(gh, wh), f"hit/miss differs on {int(np.sum(gh != wh))} rays"
+    if any_hit:
+        sel = np.nonzero(gh)[0]
+    else:
+        assert np.array_equal(got["t"][gh], want["t"][gh]), "closest t differs"
+        same = gh & (got["prim_id"] == want["prim_id"])
+        assert np.array_equal(got["u"][same], want["u"][same]) and np.array_equal(got["v"][same], want["v"][same])
+        sel = np.nonzero(gh & ~same)[0]
+    if sel.size == 0:
+        return 0
+    which = np.array([index[int(p)] for p in got["prim_id"][sel]])
+    re = intersect(tris, rays[sel], which)
+    ok = (re["prim_id"] >= 0) & (re["t"] == got["t"][sel]) & (re["u"] == got["u"][sel]) & (re["v"] == got["v"][sel])
+    ok &= (re["t"] >= rays["tmin"][sel]) & (re["t"] <= rays["tmax"][sel])
+    assert np.all(ok), f"{int(np.sum(~ok))} reported hits are not intersections at the reported t"
+    return 0 if any_hit else int(sel.size)
 
 
 def make_random_seed(v0: int, v1: int) -> int:

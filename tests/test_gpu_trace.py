@@ -1,6 +1,8 @@
 """GPU closest-hit / any-hit traversal vs the oracle and the reference's
-golden hits: prim ids, t, u, v bit-exact on the same tree
-(detail/bvh/intersect.inl:166-356, math/intersect.h:176-229)."""
+golden hits on the same tree (detail/bvh/intersect.inl:166-356,
+math/intersect.h:176-229): the same rays hit at bit-identical t, u, v; a
+different triangle is accepted only where it is hit at exactly the same t (a
+tie the two visit orders resolve differently), and ties are counted."""
 import numpy as np
 import pytest
 
@@ -23,12 +25,11 @@ def random_rays(n, seed, lo=-0.9, hi=0.9, tmin=0.0, tmax=FMAX):
     return r
 
 
-def assert_hits_equal(a, b):
-    assert np.array_equal(a["prim_id"], b["prim_id"])
-    hit = a["prim_id"] >= 0
-    assert np.array_equal(a["t"][hit], b["t"][hit])
-    assert np.array_equal(a["u"][hit], b["u"][hit])
-    assert np.array_equal(a["v"][hit], b["v"][hit])
+def assert_hits_match(got, want, tris, rays, any_hit=False, max_tie_frac=1e-3):
+    import oracle as O
+    ties = O.check_hits(got, want, tris, rays, any_hit)
+    print(f"[hits] {rays.size} rays, {int(np.sum(got['prim_id'] >= 0))} hits, {ties} ties", flush=True)
+    assert ties <= max_tie_frac * rays.size
 
 
 @pytest.mark.parametrize("any_hit", [False, True])
@@ -38,11 +39,9 @@ def test_golden_lbvh_hits(V, any_hit):
     bvh = V.bvh_from_nodes(g["tris"], g["lbvh_nodes"], g["lbvh_indices"])
     hits = V.trace(bvh, g["rays"], any_hit)
     ref = g["ref_any"] if any_hit else g["ref_closest"]
-    if any_hit:
-        assert np.array_equal(hits["prim_id"] >= 0, ref["prim_id"] >= 0)
-    else:
-        assert_hits_equal(hits, ref)
-        assert np.array_equal(hits["prim_id"], g["ref_brute"]["prim_id"])
+    assert_hits_match(hits, ref, g["tris"], g["rays"], any_hit)
+    if not any_hit:
+        assert_hits_match(hits, g["ref_brute"], g["tris"], g["rays"])
 
 
 def test_golden_sah_hits(V):
@@ -51,7 +50,7 @@ def test_golden_sah_hits(V):
     bvh = V.bvh_from_nodes(g["tris"], g["sah_nodes"], g["sah_indices"])
     assert bvh.info()["max_leaf_size"] > 1
     hits = V.closest_hit(bvh, g["rays"])
-    assert_hits_equal(hits, g["ref_sah_closest"])
+    assert_hits_match(hits, g["ref_sah_closest"], g["tris"], g["rays"])
 
 
 @pytest.mark.parametrize("scene", ["c0_level5", "clusters", "heightfield"])
@@ -66,7 +65,7 @@ def test_lbvh_trace_matches_oracle(V, O, scene, any_hit):
     rays = random_rays(30_000, seed=4)
     got = V.trace(bvh, rays, any_hit)
     want = O.trace(nodes, tris, idx, rays, any_hit)
-    assert_hits_equal(got, want)
+    assert_hits_match(got, want, tris, rays, any_hit)
 
 
 def test_ray_limits_and_axis_aligned_dirs(V, O):
@@ -86,7 +85,7 @@ def test_ray_limits_and_axis_aligned_dirs(V, O):
     for any_hit in (False, True):
         got = V.trace(bvh, rays, any_hit)
         want = O.trace(nodes, tris, idx, rays, any_hit)
-        assert_hits_equal(got, want)
+        assert_hits_match(got, want, tris, rays, any_hit)
 
 
 def test_intersect_kat_ray1tri4(V):

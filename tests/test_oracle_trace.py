@@ -2,6 +2,7 @@
 math/intersect.h) against the reference's intersect() (golden hits from
 oracle/_ref), its unit-test KATs and brute force over all triangles."""
 import numpy as np
+import pytest
 
 from conftest import load_golden
 from visionaray_amd import scenes as S
@@ -83,3 +84,35 @@ def test_any_hit_agrees_with_closest_hit(O):
     # an any-hit result is a real hit inside [tmin, tmax], no closer than the closest
     hit = a["prim_id"] >= 0
     assert np.all(a["t"][hit] >= c["t"][hit]) and np.all(a["t"][hit] <= r["tmax"][hit])
+
+
+def test_check_hits_accepts_ties_and_rejects_mistakes(O):
+    """The closest-hit comparison used against the GPU traversal: identical
+    records pass; a reported triangle that is hit at exactly the same t (a
+    duplicate of the closest one) passes as a tie; a wrong triangle, a wrong t
+    or a missing hit fails."""
+    g = load_golden("c0_small")
+    tris, rays, ref = g["tris"], g["rays"], g["ref_closest"]
+    assert O.check_hits(ref, ref, tris, rays) == 0
+    # duplicate every triangle under a new prim id: each hit has an exact tie
+    dup = tris.copy()
+    dup["prim_id"] = tris["prim_id"] + tris.size
+    both = np.concatenate([tris, dup])
+    tie = ref.copy()
+    hit = tie["prim_id"] >= 0
+    tie["prim_id"][hit] += tris.size
+    assert O.check_hits(tie, ref, both, rays) == int(hit.sum())
+    i = int(np.nonzero(hit)[0][0])
+    bad = ref.copy()
+    bad["prim_id"][i] = (bad["prim_id"][i] + 1) % tris.size
+    if O.intersect(tris, rays[i:i + 1], [bad["prim_id"][i]])["t"][0] != ref["t"][i]:
+        with pytest.raises(AssertionError):
+            O.check_hits(bad, ref, tris, rays)
+    bad = ref.copy()
+    bad["t"][i] = np.nextafter(bad["t"][i], np.float32(np.inf))
+    with pytest.raises(AssertionError):
+        O.check_hits(bad, ref, tris, rays)
+    bad = ref.copy()
+    bad["prim_id"][i] = -1
+    with pytest.raises(AssertionError):
+        O.check_hits(bad, ref, tris, rays)

@@ -259,7 +259,7 @@ def lbvh_build_host(prims: np.ndarray, max_leaf_size: int = 4):
 
 def trace(bvh: BVH, rays, any_hit_: bool = False, counted: bool = False, stream=None):
     """Trace rays (numpy RAY_DTYPE or DeviceArray); returns host hits (and
-    per-ray [pairs, triangles] test counts when counted)."""
+    per-ray [node visits, triangle tests] when counted)."""
     r = _as_device(rays, RAY_DTYPE)
     hits = DeviceArray(r.count, HIT_DTYPE)
     if counted:
@@ -476,7 +476,7 @@ class hip_sched:
 
     def frame(self, kernel, sp: sched_params, stream=None, stats: "DeviceArray | None" = None):
         """One frame; with `stats` (4 x uint64 device array) the counting
-        variant also accumulates node-pair and triangle tests."""
+        variant also accumulates traversal-node visits and triangle tests."""
         s = self._sp(sp)
         if isinstance(kernel, pathtracing.kernel) and stats is not None:
             check(lib().vr_render_pt_counted(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,

@@ -283,9 +283,6 @@ int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* st
         vr_bvh_destroy(h);
         return rc;
     }
-    // a Karras tree has at most 65 levels: the common prefix of a node's range
-    // (30 code bits + 32 index bits) grows strictly from parent to child
-    h->s.depth = n <= 1 ? (uint32_t)n : 65u;
     *out = h;
     return VR_OK;
     VR_TRY_END
@@ -407,7 +404,7 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
         vr_bvh_destroy(h);
         return code;
     };
-    if (depth > (uint32_t)MAX_STACK + 1) return bail(fail(VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: tree deeper than 129 levels"));
+    if (!stack_fits(s)) return bail(fail(VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: tree deeper than the traversal stack"));
     if (num_prims > 0) {
         if (hipMalloc(&s.pairs, std::max<size_t>(pairs.size(), 1) * sizeof(pair_node)) != hipSuccess ||
             hipMalloc(&s.tris, tris.size() * sizeof(tri_node)) != hipSuccess ||
@@ -422,6 +419,10 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
             hipMemcpy(s.ref_nodes, h_nodes, num_nodes * sizeof(vr_bvh_node), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s.indices, h_indices, num_prims * 4, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(VR_ERR_HIP, "vr_bvh_from_nodes: upload failed"));
+        const int rc = collapse_quads(s, 0u, nullptr);
+        if (rc == VR_OK && hipDeviceSynchronize() != hipSuccess)
+            return bail(fail(VR_ERR_HIP, "vr_bvh_from_nodes: collapse failed"));
+        if (rc) return bail(rc);
     }
     *out = h;
     return VR_OK;
@@ -477,8 +478,8 @@ int vr_bvh_destroy(vr_bvh bvh)
 {
     if (!bvh) return VR_OK;
     bvh_state& s = bvh->s;
-    void* ptrs[] = {s.pairs, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents, s.flags,
-                    s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
+    void* ptrs[] = {s.pairs, s.quads, s.collapse, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents,
+                    s.flags, s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete bvh;
@@ -630,7 +631,8 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
         depth = std::max(depth, sc.bvh->s.depth);
     }
     VR_CHECK(total < (1ull << 31), VR_ERR_INVALID_ARGUMENT, "vr_batch_create: more than 2^31 pixels");
-    VR_CHECK(depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "vr_batch_create: BVH deeper than the traversal stack");
+    VR_CHECK(3u * depth + 1u <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED,
+             "vr_batch_create: BVH deeper than the traversal stack");
     prefix[num_scenes] = (uint32_t)total;
     vr_batch b = new vr_batch_t;
     b->device = current_device();

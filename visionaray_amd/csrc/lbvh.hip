@@ -406,6 +406,11 @@ __device__ inline void store_ref_node(vr_bvh_node* o, f3 lo, f3 hi, uint32_t fir
 // visible to the next pass.
 constexpr int CHUNK = 1024;
 constexpr int TOP_BLOCK = 1024;
+// subtrees of at most this many triangles become one traversal leaf
+#ifndef VR_LBVH_LEAF_MAX
+#define VR_LBVH_LEAF_MAX 4
+#endif
+constexpr uint32_t LBVH_LEAF_MAX = VR_LBVH_LEAF_MAX;
 constexpr uint32_t SUPER = 64u * CHUNK;
 
 __device__ inline void publish_child(pair_node* pairs, uint32_t* heights, vr_bvh_node* ref_nodes, uint32_t p,
@@ -704,9 +709,10 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const size_t npairs = n > 1 ? n - 1 : 0;
     // (re)allocate owned arrays when the size changes
     if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
-        hipFree(bs.pairs); hipFree(bs.tris); hipFree(bs.prims); hipFree(bs.sorted_codes);
-        hipFree(bs.sorted_ids); hipFree(bs.flags); hipFree(bs.heights); hipFree(bs.ref_nodes);
-        hipFree(bs.indices); hipFree(bs.root_box); hipFree(bs.scratch);
+        void* owned[] = {bs.pairs, bs.quads, bs.collapse, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
+                         bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
+        for (void* q : owned)
+            if (q) (void)hipFree(q);
         bs = bvh_state{};
         bs.num_prims = n;
         bs.num_pairs = npairs;
@@ -773,10 +779,16 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, st);
     hipLaunchKernelGGL(k_iota_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.indices, n);
     VR_HIP_CHECK(hipGetLastError());
-
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
     bs.max_leaf = 1;
-    return VR_OK;
+
+    // 6: tree depth (one 4-byte read back: it bounds the collapse levels and
+    // the traversal stack), then the 4-wide traversal nodes
+    uint32_t depth = 1;
+    VR_HIP_CHECK(hipMemcpyAsync(&depth, bs.root_box + 6, 4, hipMemcpyDeviceToHost, st));
+    VR_HIP_CHECK(hipStreamSynchronize(st));
+    bs.depth = depth;
+    return collapse_quads(bs, LBVH_LEAF_MAX, st);
 }
 
 // Refit after the vertices moved: same ids, new boxes (detail/bvh/refit.h).
@@ -793,7 +805,8 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
                            bs.num_pairs, 0u);
     launch_bottom_up(bs, s, bs.ref_nodes, st);
     VR_HIP_CHECK(hipGetLastError());
-    return VR_OK;
+    // the collapse reads the new boxes; the depth is the build's
+    return collapse_quads(bs, LBVH_LEAF_MAX, st);
 }
 
 int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)

@@ -5,10 +5,12 @@
 // wavefront: each lane owns one pixel at a time and runs its spp samples in
 // sequence (the reference draws all samples of a pixel from one generator,
 // detail/sched_common.h:427-526, so a pixel cannot be split across lanes
-// without changing its random stream).  Every loop iteration traces one ray
-// per lane; a lane whose path ended starts the next sample, and a lane whose
-// pixel is done takes the next pixel, with one atomic per wave.  No lane idles
-// while the wave's longest path finishes.
+// without changing its random stream).  Each lane holds one ray query in
+// flight.  The wave advances its queries step by step until 16 lanes have a
+// finished query, then shades those lanes together: each starts its next
+// bounce, its next sample, or (pixel done) takes the next pixel, with one
+// atomic per wave.  Neither a long path nor a long query keeps the other
+// lanes of the wave idle.
 #include "vr_internal.h"
 
 #include <algorithm>
@@ -210,8 +212,24 @@ constexpr int ADJ_MAX_VERTICES = 16;
 //
 // BLK (threads per block) and LDSD (LDS stack entries per lane) are the
 // autotuned launch shape; A.tile orders pixels in tile x tile squares.
+// waves per SIMD the register allocation must leave room for (0: as many
+// registers as the kernel wants)
+#ifndef VR_PT_WAVES_PER_EU
+#define VR_PT_WAVES_PER_EU 6
+#endif
+#if VR_PT_WAVES_PER_EU > 0
+#define VR_PT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_PT_WAVES_PER_EU)))
+#else
+#define VR_PT_OCCUPANCY
+#endif
+
+// lanes that must wait for shading before the wave stops advancing its queries
+#ifndef VR_PT_REGEN_LANES
+#define VR_PT_REGEN_LANES 16
+#endif
+
 template <int MODE, bool BATCH, int BLK, int LDSD>
-__global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
+__global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = MODE == PT_COUNT;
     constexpr bool ADJ = MODE == PT_ADJOINT;
@@ -234,7 +252,6 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
     bool done = false;
     rng_t g{1u};
     uint32_t s = 0, bounce = 0;
-    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
     f3 tp = mk3(1, 1, 1), I = mk3(0, 0, 0);
     f3 tp0 = mk3(1, 1, 1);  // FROZEN: throughput under A.rr_mats
     bool fetched = false;   // PLAIN: the lane took its one pixel
@@ -242,13 +259,174 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
     float4 sum = make_float4(0, 0, 0, 0);
     unsigned long long n_primary = 0, n_secondary = 0;
     trace_counters tc{0, 0};
-    unsigned long long n_pairs = 0, n_tris = 0;
+    unsigned long long n_nodes = 0, n_tris = 0;
     f3 wgt = mk3(0, 0, 0);
     uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
     uint32_t nmid = 0;
+    // the lane's query: in flight (tracing) or finished and waiting to be
+    // shaded (pending)
+    trav_state T;
+    uint32_t spill[MAX_STACK - LDSD];
+    bool tracing = false, pending = false;
+    bvh_view bv{};  // BATCH: the BVH of the lane's scene, kept in registers
 
     for (;;) {
-        // regenerate: lanes without a pixel take the next ones, one atomic per wave
+        // 1. shade the finished queries; each such lane starts its next query
+        //    or finishes its sample or pixel
+        if (pending) {
+            pending = false;
+            const scene_desc& S = SCN;
+            const hit_t hr = T.r;
+            const f3 o = T.ori, d = T.dir;
+            f3 no = o, nd = d;  // next ray
+            if (COUNT) {
+                n_nodes += tc.nodes;
+                n_tris += tc.tris;
+                tc.nodes = 0;
+                tc.tris = 0;
+            }
+            bool finished = false;
+            if (hr.tri < 0) {
+                // escaped: ambient light times throughput (pathtracing.inl:69-77)
+                const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), tp);
+                I = add3(I, e);
+                finished = true;
+                if (ADJ) {
+                    for (uint32_t j = 0; j < nmid; ++j) {
+                        const vr_material* mj = S.mats + mids[j];
+                        const uint32_t slot = 6 * (S.grad_base + mids[j]);
+                        atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
+                        atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
+                        atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
+                    }
+                }
+            } else {
+                if (bounce == 0) rhit = true;
+                const f3 view = neg3(d);
+                const f3 pos = add3(o, muls3(d, hr.t));
+                const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
+                const float4 tb = tq[1], tcq = tq[2];
+                const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
+                uint32_t mid = __float_as_uint(tb.w);
+                mid = mid < S.num_mats ? mid : S.num_mats - 1u;
+                const vr_material* m = S.mats + mid;
+                f3 refl = mk3(0.0f, 0.0f, 0.0f);
+                float pdf;
+                f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
+                const bool emission = m->type == 1;
+                if (emission) {
+                    // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
+                    const float ls = m->ls;
+                    src = mk3((1.0f * m->ce[0]) * ls, (1.0f * m->ce[1]) * ls, (1.0f * m->ce[2]) * ls);
+                    pdf = 1.0f;
+                } else {
+                    // matte::sample -> lambertian::sample_f (matte.inl:43-56, brdf.h:50-84)
+                    const f3 n = faceforward3(gn, view, gn);
+                    f3 u, v;
+                    onb(n, u, v);
+                    float a1, a2;
+                    rng_pair(g, A.arg_order, a1, a2);
+                    const f3 sp = cosine_hemisphere(a1, a2);
+                    refl = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
+                    pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
+                    const float kd = m->kd;
+                    src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
+                              1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
+                              1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
+                    if (FROZEN) {
+                        const vr_material* m0 = A.rr_mats + mid;
+                        const float kd0 = m0->kd;
+                        src0 = mk3(1.0f * ((m0->cd[0] * kd0) * 3.18309886183790691216444201928e-01f),
+                                   1.0f * ((m0->cd[1] * kd0) * 3.18309886183790691216444201928e-01f),
+                                   1.0f * ((m0->cd[2] * kd0) * 3.18309886183790691216444201928e-01f));
+                    }
+                }
+                const bool zero_pdf = pdf <= 0.0f;
+                bool active = true;
+                if (emission) {
+                    // no light list: MIS weight 1 (pathtracing.inl:134-146)
+                    const f3 e = mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src);
+                    I = add3(I, e);
+                    active = false;
+                    if (ADJ) {
+                        const float ls = m->ls;
+                        const uint32_t es = 6 * (S.grad_base + mid);
+                        atomicAdd(&s_grad[es + 3], wgt.x * tp.x * ls);
+                        atomicAdd(&s_grad[es + 4], wgt.y * tp.y * ls);
+                        atomicAdd(&s_grad[es + 5], wgt.z * tp.z * ls);
+                        for (uint32_t j = 0; j < nmid; ++j) {
+                            const vr_material* mj = S.mats + mids[j];
+                            const uint32_t slot = 6 * (S.grad_base + mids[j]);
+                            atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
+                            atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
+                            atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
+                        }
+                    }
+                } else if (ADJ && !zero_pdf && nmid < ADJ_MAX_VERTICES) {
+                    mids[nmid++] = mid;
+                }
+                if (zero_pdf) active = false;
+                const f3 n = faceforward3(gn, view, gn);
+                const float q = dot3(n, refl) / pdf;
+                tp = mk3(tp.x * (src.x * q), tp.y * (src.y * q), tp.z * (src.z * q));
+                if (zero_pdf) tp = mk3(0.0f, 0.0f, 0.0f);
+                if (FROZEN) {
+                    tp0 = emission ? tp : mk3(tp0.x * (src0.x * q), tp0.y * (src0.y * q), tp0.z * (src0.z * q));
+                    if (zero_pdf) tp0 = mk3(0.0f, 0.0f, 0.0f);
+                }
+                if (bounce >= 2) {
+                    // Russian roulette (pathtracing.inl:201-213)
+                    const f3 rt = FROZEN ? tp0 : tp;
+                    float prob = rt.x;
+                    if (rt.y > prob) prob = rt.y;
+                    if (rt.z > prob) prob = rt.z;
+                    const bool terminate = rng_next(g) > prob;
+                    if (terminate) active = false;
+                    tp = mk3(tp.x / prob, tp.y / prob, tp.z / prob);
+                    if (FROZEN) tp0 = mk3(tp0.x / prob, tp0.y / prob, tp0.z / prob);
+                }
+                if (!active) {
+                    finished = true;
+                } else {
+                    no = add3(pos, muls3(refl, S.kp.epsilon));
+                    nd = refl;
+                    ++bounce;
+                    if (bounce >= S.kp.num_bounces) finished = true;
+                }
+            }
+            if (finished) {
+                if (!ADJ) {
+                    if (rhit) {
+                        sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
+                    } else {
+                        sum.x += S.kp.bg[0]; sum.y += S.kp.bg[1]; sum.z += S.kp.bg[2]; sum.w += 1.0f;
+                    }
+                }
+                ++s;
+                if (ADJ) nmid = 0;
+                if (s >= S.sp.spp) {
+                    if (!ADJ) write_pixel(S, (uint32_t)px, sum);
+                    px = -1;
+                } else {
+                    const uint32_t w = S.sp.width, h = S.sp.height;
+                    const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
+                    primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, no, nd);
+                    tp = mk3(1, 1, 1);
+                    tp0 = mk3(1, 1, 1);
+                    I = mk3(0, 0, 0);
+                    bounce = 0;
+                    rhit = false;
+                }
+            }
+            if (px >= 0) {
+                trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
+                tracing = true;
+                if (bounce == 0) ++n_primary;
+                else ++n_secondary;
+            }
+        }
+
+        // 2. lanes without a pixel take the next ones, one atomic per wave
         const bool need = !done && px < 0;
         const uint64_t needm = __ballot(need);
         if (needm) {
@@ -277,6 +455,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                         }
                         sid = lo;
                         lpx = my - A.prefix[lo];
+                        bv = A.scenes[sid].b;
                     }
                     const scene_desc& S = SCN;
                     const uint32_t w = S.sp.width, h = S.sp.height;
@@ -288,6 +467,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                         rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
                         s = 0;
                         sum = make_float4(0, 0, 0, 0);
+                        f3 o, d;
                         primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
                         tp = mk3(1, 1, 1);
                         tp0 = mk3(1, 1, 1);
@@ -301,6 +481,9 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
                             wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
                             nmid = 0;
                         }
+                        trav_begin(T, S.b, o, d, 0.0f, FLT_MAX);
+                        tracing = true;
+                        ++n_primary;
                     }
                 } else {
                     done = true;
@@ -308,150 +491,16 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
             }
         }
         if (__ballot(!done) == 0) break;
-        if (done || px < 0) continue;
 
-        const scene_desc& S = SCN;
-        const hit_t hr = traverse<false, COUNT, LDSD, BLK>(S.b, o, d, 0.0f, FLT_MAX, st, &tc);
-        if (COUNT) {
-            n_pairs += tc.pairs;
-            n_tris += tc.tris;
-            tc.pairs = 0;
-            tc.tris = 0;
-        }
-        if (bounce == 0) ++n_primary;
-        else ++n_secondary;
-
-        bool finished = false;
-        if (hr.tri < 0) {
-            // escaped: ambient light times throughput (pathtracing.inl:69-77)
-            const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), tp);
-            I = add3(I, e);
-            finished = true;
-            if (ADJ) {
-                for (uint32_t j = 0; j < nmid; ++j) {
-                    const vr_material* mj = S.mats + mids[j];
-                    const uint32_t slot = 6 * (S.grad_base + mids[j]);
-                    atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
-                    atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
-                    atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
-                }
-            }
-        } else {
-            if (bounce == 0) rhit = true;
-            const f3 view = neg3(d);
-            const f3 pos = add3(o, muls3(d, hr.t));
-            const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
-            const float4 tb = tq[1], tcq = tq[2];
-            const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
-            uint32_t mid = __float_as_uint(tb.w);
-            mid = mid < S.num_mats ? mid : S.num_mats - 1u;
-            const vr_material* m = S.mats + mid;
-            f3 refl = mk3(0.0f, 0.0f, 0.0f);
-            float pdf;
-            f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
-            const bool emission = m->type == 1;
-            if (emission) {
-                // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
-                const float ls = m->ls;
-                src = mk3((1.0f * m->ce[0]) * ls, (1.0f * m->ce[1]) * ls, (1.0f * m->ce[2]) * ls);
-                pdf = 1.0f;
-            } else {
-                // matte::sample -> lambertian::sample_f (matte.inl:43-56, brdf.h:50-84)
-                const f3 n = faceforward3(gn, view, gn);
-                f3 u, v;
-                onb(n, u, v);
-                float a1, a2;
-                rng_pair(g, A.arg_order, a1, a2);
-                const f3 sp = cosine_hemisphere(a1, a2);
-                refl = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
-                pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
-                const float kd = m->kd;
-                src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
-                          1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
-                          1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
-                if (FROZEN) {
-                    const vr_material* m0 = A.rr_mats + mid;
-                    const float kd0 = m0->kd;
-                    src0 = mk3(1.0f * ((m0->cd[0] * kd0) * 3.18309886183790691216444201928e-01f),
-                               1.0f * ((m0->cd[1] * kd0) * 3.18309886183790691216444201928e-01f),
-                               1.0f * ((m0->cd[2] * kd0) * 3.18309886183790691216444201928e-01f));
-                }
-            }
-            const bool zero_pdf = pdf <= 0.0f;
-            bool active = true;
-            if (emission) {
-                // no light list: MIS weight 1 (pathtracing.inl:134-146)
-                const f3 e = mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src);
-                I = add3(I, e);
-                active = false;
-                if (ADJ) {
-                    const float ls = m->ls;
-                    const uint32_t es = 6 * (S.grad_base + mid);
-                    atomicAdd(&s_grad[es + 3], wgt.x * tp.x * ls);
-                    atomicAdd(&s_grad[es + 4], wgt.y * tp.y * ls);
-                    atomicAdd(&s_grad[es + 5], wgt.z * tp.z * ls);
-                    for (uint32_t j = 0; j < nmid; ++j) {
-                        const vr_material* mj = S.mats + mids[j];
-                        const uint32_t slot = 6 * (S.grad_base + mids[j]);
-                        atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
-                        atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
-                        atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
-                    }
-                }
-            } else if (ADJ && !zero_pdf && nmid < ADJ_MAX_VERTICES) {
-                mids[nmid++] = mid;
-            }
-            if (zero_pdf) active = false;
-            const f3 n = faceforward3(gn, view, gn);
-            const float q = dot3(n, refl) / pdf;
-            tp = mk3(tp.x * (src.x * q), tp.y * (src.y * q), tp.z * (src.z * q));
-            if (zero_pdf) tp = mk3(0.0f, 0.0f, 0.0f);
-            if (FROZEN) {
-                tp0 = emission ? tp : mk3(tp0.x * (src0.x * q), tp0.y * (src0.y * q), tp0.z * (src0.z * q));
-                if (zero_pdf) tp0 = mk3(0.0f, 0.0f, 0.0f);
-            }
-            if (bounce >= 2) {
-                // Russian roulette (pathtracing.inl:201-213)
-                const f3 rt = FROZEN ? tp0 : tp;
-                float prob = rt.x;
-                if (rt.y > prob) prob = rt.y;
-                if (rt.z > prob) prob = rt.z;
-                const bool terminate = rng_next(g) > prob;
-                if (terminate) active = false;
-                tp = mk3(tp.x / prob, tp.y / prob, tp.z / prob);
-                if (FROZEN) tp0 = mk3(tp0.x / prob, tp0.y / prob, tp0.z / prob);
-            }
-            if (!active) {
-                finished = true;
-            } else {
-                o = add3(pos, muls3(refl, S.kp.epsilon));
-                d = refl;
-                ++bounce;
-                if (bounce >= S.kp.num_bounces) finished = true;
-            }
-        }
-        if (finished) {
-            if (!ADJ) {
-                if (rhit) {
-                    sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
-                } else {
-                    sum.x += S.kp.bg[0]; sum.y += S.kp.bg[1]; sum.z += S.kp.bg[2]; sum.w += 1.0f;
-                }
-            }
-            ++s;
-            if (ADJ) nmid = 0;
-            if (s >= S.sp.spp) {
-                if (!ADJ) write_pixel(S, (uint32_t)px, sum);
-                px = -1;
-            } else {
-                const uint32_t w = S.sp.width, h = S.sp.height;
-                const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
-                primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
-                tp = mk3(1, 1, 1);
-                tp0 = mk3(1, 1, 1);
-                I = mk3(0, 0, 0);
-                bounce = 0;
-                rhit = false;
+        // 3. advance the queries in flight until VR_PT_REGEN_LANES lanes wait
+        //    to be shaded (or none is in flight): a lane whose query ends
+        //    early does not idle until the wave's longest query is done
+        for (;;) {
+            if (__ballot(tracing) == 0) break;
+            if (__popcll(__ballot(pending)) >= VR_PT_REGEN_LANES) break;
+            if (tracing && trav_step<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc)) {
+                tracing = false;
+                pending = true;
             }
         }
     }
@@ -461,7 +510,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
             n_primary += __shfl_xor(n_primary, off);
             n_secondary += __shfl_xor(n_secondary, off);
             if (COUNT) {
-                n_pairs += __shfl_xor(n_pairs, off);
+                n_nodes += __shfl_xor(n_nodes, off);
                 n_tris += __shfl_xor(n_tris, off);
             }
         }
@@ -469,7 +518,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
             atomicAdd(&A.ray_counts[0], n_primary);
             atomicAdd(&A.ray_counts[1], n_secondary);
             if (COUNT) {
-                atomicAdd(&A.ray_counts[2], n_pairs);
+                atomicAdd(&A.ray_counts[2], n_nodes);
                 atomicAdd(&A.ray_counts[3], n_tris);
             }
         }
@@ -484,6 +533,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace(pt_args A)
     }
 #undef SCN
 }
+
 
 struct ao_args {
     bvh_view b;
@@ -663,7 +713,7 @@ int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, con
               unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st)
 {
     if ((size_t)sp.width * sp.height == 0) return VR_OK;
-    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "render_pt: BVH deeper than the traversal stack");
     return render_pt_shaped(bs, mats, num_mats, kp, cam, sp, accum, color, ray_counts, work, counted, g_shape, st);
 }
 
@@ -687,7 +737,7 @@ int render_pt_frozen_rr(const bvh_state& bs, const vr_material* mats, const vr_m
                         float* color, unsigned long long* ray_counts, uint32_t* work, hipStream_t st)
 {
     if ((size_t)sp.width * sp.height == 0) return VR_OK;
-    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_frozen_rr: BVH deeper than the stack");
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "render_pt_frozen_rr: BVH deeper than the stack");
     VR_CHECK((uint64_t)padded_pixels(sp.width, sp.height, g_shape.tile) < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
              "render_pt_frozen_rr: image too large");
     pt_args A =
@@ -706,7 +756,7 @@ int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_m
     VR_CHECK(num_mats <= ADJ_MAX_MATERIALS, VR_ERR_UNSUPPORTED, "render_pt_backward: more than 2048 materials");
     VR_CHECK(kp.num_bounces <= (uint32_t)ADJ_MAX_VERTICES, VR_ERR_UNSUPPORTED,
              "render_pt_backward: more than 16 bounces");
-    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_pt_backward: BVH deeper than the stack");
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "render_pt_backward: BVH deeper than the stack");
     pt_args A =
         single_args(bs, mats, num_mats, kp, cam, sp, nullptr, nullptr, dl, ray_counts, work, grad, g_arg_order);
     A.tile = g_shape.tile;
@@ -814,7 +864,7 @@ int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& 
     A.accum = accum;
     A.color = color;
     A.arg_order = g_arg_order;
-    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "render_ao: BVH deeper than the traversal stack");
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "render_ao: BVH deeper than the traversal stack");
     hipLaunchKernelGGL(k_ao, dim3(div_up(npix, BLOCK)), dim3(BLOCK), 0, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
     }
     *reinterpret_cast<float4*>(hits + i) = o;
     if (COUNT) {
-        counts[2 * i] = c.pairs;
+        counts[2 * i] = c.nodes;
         counts[2 * i + 1] = c.tris;
     }
 }
@@ -42,7 +42,7 @@ template <bool ANY, bool COUNT>
 int launch_trace_depth(const bvh_state& bs, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
                        hipStream_t st)
 {
-    VR_CHECK(bs.depth <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED, "trace: BVH deeper than the traversal stack");
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "trace: BVH deeper than the traversal stack");
     hipLaunchKernelGGL((k_trace<ANY, COUNT>), dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.view(), rays, n, hits,
                        counts);
     VR_HIP_CHECK(hipGetLastError());

@@ -65,6 +65,20 @@ __host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count)
     return LEAF_BIT | ((count - 1u) << 26) | first;
 }
 
+// Traversal node: up to four children in one 128-byte line (eight 16-byte
+// loads), boxes as structure of arrays so each load carries one coordinate
+// of all four children.  It is a collapse of the binary tree (pair records):
+// every child box is the box of a binary node, the same exact union.
+//   lo_x .. hi_z  child c's box component in lane c
+//   child[c]      entry as in pair_node (inner: quad index)
+//   meta.x        number of children (2..4); meta.y..w unused
+struct alignas(128) quad_node {
+    float4 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z;
+    uint4 child;
+    uint4 meta;
+};
+static_assert(sizeof(quad_node) == 128, "quad_node is one 128-byte line");
+
 // Triangle in traversal order, 48 bytes = three 16-byte loads:
 //   a = v1.xyz, prim_id    b = e1.xyz, geom_id    c = e2.xyz, original index
 struct alignas(16) tri_node {
@@ -77,11 +91,11 @@ static_assert(sizeof(tri_node) == 48, "tri_node layout");
 // Device view of a BVH, passed by value into kernels (the analogue of
 // index_bvh_ref_t, bvh.h:495-569).
 struct bvh_view {
-    const pair_node* pairs;
+    const quad_node* quads;
     const tri_node* tris;
     uint32_t num_prims;
-    uint32_t root;  // entry of the root: 0 (pair 0) or a leaf entry; unused when num_prims == 0
-    uint32_t num_pairs;
+    uint32_t root;  // entry of the root: 0 (quad 0) or a leaf entry; unused when num_prims == 0
+    uint32_t num_quads;
 };
 
 // Host-side state behind a vr_bvh handle.
@@ -94,6 +108,8 @@ struct bvh_state {
     uint32_t max_leaf = 1;
     bool is_lbvh = false;
     pair_node* pairs = nullptr;      // num_pairs
+    quad_node* quads = nullptr;      // traversal nodes, at most max(num_pairs, 1)
+    uint32_t* collapse = nullptr;    // collapse queues and counters
     tri_node* tris = nullptr;        // num_prims, traversal order
     vr_triangle* prims = nullptr;    // num_prims, caller order (tree.primitives())
     uint32_t* sorted_codes = nullptr;
@@ -108,7 +124,7 @@ struct bvh_state {
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
-    bvh_view view() const { return bvh_view{pairs, tris, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
+    bvh_view view() const { return bvh_view{quads, tris, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
 };
 
 // ---------------------------------------------------------------------------
@@ -186,124 +202,167 @@ struct hit_t {
 };
 
 // ---------------------------------------------------------------------------
-// BVH traversal (detail/bvh/intersect.inl:166-356, device order).
+// BVH traversal (detail/bvh/intersect.inl:166-356) over the 4-wide collapse.
 //
-// The root box is not tested.  At an inner node both child boxes are slab
-// tested with tnear = max(tmin, max_el(min(tlo, thi))) and tfar =
-// min(tmax, min_el(max(tlo, thi))); a child is entered when tfar >= tnear and
-// tnear < t_closest, and with both entered the nearer one (tnear0 < tnear1 ?
-// 0 : 1) goes first while the other is deferred.  Primitives are accepted
-// with is_closer (update_if.h:68-107): hit && t < t_closest && tmin <= t <=
-// tmax, strictly closer, so the first hit found wins a tie.  The reference
-// keeps deferred nodes in a 5-entry register stack plus a restart trail; the
-// full stack here visits the same nodes in the same order.
+// The root box is not tested.  At a node every child box is slab tested
+// exactly as the reference tests a binary node's children: tnear =
+// max(tmin, max_el(min(tlo, thi))) and tfar = min(tmax, min_el(max(tlo,
+// thi))) with tlo = (lo - ori) * inv_dir, and the child is entered when
+// tfar >= tnear and tnear < t_closest.  Entered children are visited nearest
+// first (a sorting network on tnear); the others are deferred on the stack,
+// farthest deepest.  Primitives are accepted with is_closer (update_if.h:
+// 68-107): hit && t < t_closest && tmin <= t <= tmax.  The closest hit is the
+// reference's; only which of several triangles hit at exactly the same t is
+// reported can differ (the reference keeps the first found in its binary
+// near-first order).
 //
 // The stack keeps its top LDS_DEPTH entries in LDS, one column per lane
 // (stack[k * STRIDE], so the 64 lanes of a wave touch 64 consecutive words);
 // deeper entries go to a per-lane private array that only rays on unusually
-// deep paths touch.  MAX_STACK entries must cover the tree depth (checked on
-// the host before launch).
+// deep paths touch.  A node defers at most three children, so MAX_STACK
+// entries cover trees of depth (MAX_STACK - 1) / 3 (checked on the host).
 // ---------------------------------------------------------------------------
 
 struct trace_counters {
-    uint32_t pairs;
+    uint32_t nodes;
     uint32_t tris;
 };
 
-constexpr int MAX_STACK = 128;
+constexpr int MAX_STACK = 192;
+
+// slab test of one child box; returns tnear, or +inf when the child is not entered
+__device__ __forceinline__ float slab_enter(float lx, float ly, float lz, float hx, float hy, float hz, f3 ori,
+                                            f3 inv, float tmin, float tmax, float tclosest)
+{
+    const f3 tlo = mul3(sub3(mk3(lx, ly, lz), ori), inv);
+    const f3 thi = mul3(sub3(mk3(hx, hy, hz), ori), inv);
+    const float tn = tmax_f(tmin, max_el3(min3v(tlo, thi)));
+    const float tf = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+    return (tf >= tn && tn < tclosest) ? tn : __builtin_inff();
+}
+
+__device__ __forceinline__ void cswap(float& ka, uint32_t& ea, float& kb, uint32_t& eb)
+{
+    const bool s = kb < ka;
+    const float k = s ? kb : ka;
+    kb = s ? ka : kb;
+    ka = k;
+    const uint32_t e = s ? eb : ea;
+    eb = s ? ea : eb;
+    ea = e;
+}
+
+// One query in flight.  A query advances in steps (trav_step) so that a
+// lane can hold its query while other lanes of the wave shade and start new
+// ones; traverse() runs one query to the end.
+struct trav_state {
+    f3 ori, dir, inv;
+    float tmin, tmax;
+    uint32_t cur;     // entry being visited
+    int sp;           // deferred entries on the stack
+    uint32_t budget;  // visits left: a ray visits each node at most once
+    hit_t r;
+};
+
+__device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax)
+{
+    T.ori = ori;
+    T.dir = dir;
+    T.inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
+                dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
+    T.tmin = tmin;
+    T.tmax = tmax;
+    T.cur = b.root;
+    T.sp = 0;
+    T.budget = 2u * b.num_prims + 2u;
+    T.r.t = FLT_MAX;
+    T.r.u = 0.0f;
+    T.r.v = 0.0f;
+    T.r.tri = -1;
+}
+
+// One step: descend through inner nodes to a leaf, test the leaf's triangles
+// in order, take the next deferred entry.  Returns true when the query is
+// finished (T.r is its result).  A longer walk than the budget means a
+// corrupt tree and ends the query instead of looping.
+template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
+__device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
+                                          uint32_t* __restrict__ spill, trace_counters* cnt)
+{
+    if (b.num_prims == 0) return true;
+    auto push = [&](uint32_t e) {
+        if (T.sp < LDS_DEPTH) stack[T.sp * STRIDE] = e;
+        else spill[T.sp - LDS_DEPTH] = e;
+        ++T.sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        --T.sp;
+        return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
+    };
+    while (!(T.cur & LEAF_BIT)) {
+        if (--T.budget == 0 || T.cur >= b.num_quads) return true;
+        const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
+        const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+        const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
+        const uint32_t nch = reinterpret_cast<const uint32_t*>(p + 7)[0];
+        if (COUNT) cnt->nodes++;
+        const float tc = T.r.t;
+        float k0 = slab_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T.ori, T.inv, T.tmin, T.tmax, tc);
+        float k1 = slab_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, T.ori, T.inv, T.tmin, T.tmax, tc);
+        float k2 = slab_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, T.ori, T.inv, T.tmin, T.tmax, tc);
+        float k3 = slab_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, T.ori, T.inv, T.tmin, T.tmax, tc);
+        if (nch < 3) k2 = __builtin_inff();
+        if (nch < 4) k3 = __builtin_inff();
+        uint32_t e0 = ch.x, e1 = ch.y, e2 = ch.z, e3 = ch.w;
+        cswap(k0, e0, k1, e1);
+        cswap(k2, e2, k3, e3);
+        cswap(k0, e0, k2, e2);
+        cswap(k1, e1, k3, e3);
+        cswap(k1, e1, k2, e2);
+        if (!(k0 < __builtin_inff())) {
+            if (T.sp == 0) return true;
+            T.cur = pop();
+            continue;
+        }
+        if (k3 < __builtin_inff()) push(e3);
+        if (k2 < __builtin_inff()) push(e2);
+        if (k1 < __builtin_inff()) push(e1);
+        T.cur = e0;
+    }
+    // leaf: test its triangles in order
+    const uint32_t first = T.cur & LEAF_FIRST_MASK;
+    const uint32_t count = ((T.cur >> 26) & 31u) + 1u;
+    if (first + count > b.num_prims) return true;  // corrupt entry
+    for (uint32_t k = first; k < first + count; ++k) {
+        const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
+        const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+        if (COUNT) cnt->tris++;
+        float t, u, v;
+        if (intersect_tri(T.ori, T.dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
+            if (t < T.r.t && t >= T.tmin && t <= T.tmax) {
+                T.r.t = t;
+                T.r.u = u;
+                T.r.v = v;
+                T.r.tri = (int32_t)k;
+                if (ANY_HIT) return true;
+            }
+        }
+    }
+    if (T.sp == 0) return true;
+    T.cur = pop();
+    return false;
+}
 
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
 __device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
                                           uint32_t* __restrict__ stack, trace_counters* cnt)
 {
-    hit_t r;
-    r.t = FLT_MAX;
-    r.u = 0.0f;
-    r.v = 0.0f;
-    r.tri = -1;
-    if (b.num_prims == 0) return r;
-
-    const f3 inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
-                       dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
-    int sp = 0;
-    uint32_t cur = b.root;
+    trav_state T;
     uint32_t spill[MAX_STACK - LDS_DEPTH];
-    auto push = [&](uint32_t e) {
-        if (sp < LDS_DEPTH) stack[sp * STRIDE] = e;
-        else spill[sp - LDS_DEPTH] = e;
-        ++sp;
-    };
-    auto pop = [&]() -> uint32_t {
-        --sp;
-        return sp < LDS_DEPTH ? stack[sp * STRIDE] : spill[sp - LDS_DEPTH];
-    };
-
-    // a ray visits each node at most once; a longer walk means a corrupt tree
-    // and ends the query instead of looping
-    uint32_t budget = 2u * b.num_prims + 2u;
-    for (;;) {
-        // descend through inner nodes
-        while (!(cur & LEAF_BIT)) {
-            if (--budget == 0 || cur >= b.num_pairs) return r;
-            const float4* p = reinterpret_cast<const float4*>(b.pairs + cur);
-            const float4 a0 = p[0], q1 = p[1], q2 = p[2];
-            const uint4 q3 = *reinterpret_cast<const uint4*>(p + 3);
-            if (COUNT) cnt->pairs++;
-            // child 0
-            float tn0, tf0, tn1, tf1;
-            {
-                f3 lo = mk3(a0.x, a0.y, a0.z), hi = mk3(a0.w, q1.x, q1.y);
-                f3 tlo = mul3(sub3(lo, ori), inv);
-                f3 thi = mul3(sub3(hi, ori), inv);
-                tn0 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf0 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            {
-                f3 lo = mk3(q1.z, q1.w, q2.x), hi = mk3(q2.y, q2.z, q2.w);
-                f3 tlo = mul3(sub3(lo, ori), inv);
-                f3 thi = mul3(sub3(hi, ori), inv);
-                tn1 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf1 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            const bool h0 = tf0 >= tn0 && tn0 < r.t;
-            const bool h1 = tf1 >= tn1 && tn1 < r.t;
-            if (h0 && h1) {
-                const bool near1 = !(tn0 < tn1);
-                push(near1 ? q3.x : q3.y);
-                cur = near1 ? q3.y : q3.x;
-            } else if (h0) {
-                cur = q3.x;
-            } else if (h1) {
-                cur = q3.y;
-            } else {
-                if (sp == 0) return r;
-                cur = pop();
-            }
-        }
-        // leaf: test its triangles in order
-        {
-            const uint32_t first = cur & LEAF_FIRST_MASK;
-            const uint32_t count = ((cur >> 26) & 31u) + 1u;
-            if (first + count > b.num_prims) return r;  // corrupt entry
-            for (uint32_t k = first; k < first + count; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
-                const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                if (COUNT) cnt->tris++;
-                float t, u, v;
-                if (intersect_tri(ori, dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tc.x, tc.y, tc.z), t, u, v)) {
-                    if (t < r.t && t >= tmin && t <= tmax) {
-                        r.t = t;
-                        r.u = u;
-                        r.v = v;
-                        r.tri = (int32_t)k;
-                        if (ANY_HIT) return r;
-                    }
-                }
-            }
-        }
-        if (sp == 0) return r;
-        cur = pop();
+    trav_begin(T, b, ori, dir, tmin, tmax);
+    while (!trav_step<ANY_HIT, COUNT, LDS_DEPTH, STRIDE>(T, b, stack, spill, cnt)) {
     }
+    return T.r;
 }
 
 // LDS part of the traversal stack (entries per lane)
@@ -314,5 +373,14 @@ constexpr int LDS_STACK = 16;
 // ---------------------------------------------------------------------------
 int current_device();
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// the traversal stack holds the deferred children of a tree of this depth
+inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32_t)MAX_STACK; }
+
+// Collapse the binary pair records of `s` into its 4-wide traversal nodes
+// (s.quads, s.num_quads, s.root).  leaf_max > 0 (LBVH only: pair e[3] holds
+// the range end) turns inner subtrees of at most leaf_max triangles into one
+// leaf over their contiguous range.  Needs s.depth.
+int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
 
 }  // namespace vr

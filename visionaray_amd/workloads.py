@@ -56,22 +56,18 @@ def bench_batched(args, V, S, log):
         p0, s0 = sched.rays()
         return ms, (p0 + s0) / steps
 
-    # plain launches: one thread per pixel in 16 x 16 pixel blocks, one
-        # launch per scene (cuda_sched::frame per scene)
-        plain_ms, plain_rays = per_scene(dict(block_size=256, stack_lds=16, tile=16,
-                                              schedule=V.SCHEDULE_PER_PIXEL))
-        # the same launches on the persistent scheduler
-        pers_ms, pers_rays = per_scene(dict(block_size=256, stack_lds=16, tile=1))
-        # autotune the persistent shape on the scene with the most pixels x triangles
-        big = max(range(len(scenes)),
-                  key=lambda i: scenes[i][1].rt.width * scenes[i][1].rt.height * scenes[i][0].bvh.info()["num_prims"])
-        shape, tuned_ms = V.autotune(scenes[big][0], scenes[big][1], reps=2, stream=st)
-        log(f"c2: autotuned shape {shape} ({tuned_ms:.2f} ms on scene {big})")
-
-        # one persistent launch for all scenes, captured in a graph
-        b = V.batch(scenes)
-    finally:
-        pass
+    # plain launches: one thread per pixel in 16 x 16 pixel blocks, one launch
+    # per scene (cuda_sched::frame per scene)
+    plain_ms, plain_rays = per_scene(dict(block_size=256, stack_lds=16, tile=16, schedule=V.SCHEDULE_PER_PIXEL))
+    # the same launches on the persistent scheduler
+    pers_ms, pers_rays = per_scene(dict(block_size=256, stack_lds=16, tile=1))
+    # autotune the persistent shape on the scene with the most pixels x triangles
+    big = max(range(len(scenes)),
+              key=lambda i: scenes[i][1].rt.width * scenes[i][1].rt.height * scenes[i][0].bvh.info()["num_prims"])
+    shape, tuned_ms = V.autotune(scenes[big][0], scenes[big][1], reps=2, stream=st)
+    log(f"c2: autotuned shape {shape} ({tuned_ms:.2f} ms on scene {big})")
+    # one persistent launch for all scenes with that shape, captured in a graph
+    b = V.batch(scenes)
     b.render(st)
     b.capture_graph(st)
     b.launch_graph(st)
