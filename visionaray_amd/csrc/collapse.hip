@@ -138,8 +138,9 @@ __global__ __launch_bounds__(BLOCK) void k_collapse_level(const pair_node* __res
         }
         quad_node qn;
         const float inf = __builtin_inff();
+        // unused slots: lo = hi = +inf, a box the slab test never enters
         float lx[4] = {inf, inf, inf, inf}, ly[4] = {inf, inf, inf, inf}, lz[4] = {inf, inf, inf, inf};
-        float hx[4] = {-inf, -inf, -inf, -inf}, hy[4] = {-inf, -inf, -inf, -inf}, hz[4] = {-inf, -inf, -inf, -inf};
+        float hx[4] = {inf, inf, inf, inf}, hy[4] = {inf, inf, inf, inf}, hz[4] = {inf, inf, inf, inf};
         uint32_t ce[4] = {LEAF_BIT, LEAF_BIT, LEAF_BIT, LEAF_BIT};
         for (int k = 0; k < nc; ++k) {
             lx[k] = c[k].lo.x; ly[k] = c[k].lo.y; lz[k] = c[k].lo.z;

@@ -6,7 +6,7 @@
 // sequence (the reference draws all samples of a pixel from one generator,
 // detail/sched_common.h:427-526, so a pixel cannot be split across lanes
 // without changing its random stream).  Each lane holds one ray query in
-// flight.  The wave advances its queries step by step until 16 lanes have a
+// flight.  The wave advances its queries step by step until 24 lanes have a
 // finished query, then shades those lanes together: each starts its next
 // bounce, its next sample, or (pixel done) takes the next pixel, with one
 // atomic per wave.  Neither a long path nor a long query keeps the other
@@ -225,7 +225,7 @@ constexpr int ADJ_MAX_VERTICES = 16;
 
 // lanes that must wait for shading before the wave stops advancing its queries
 #ifndef VR_PT_REGEN_LANES
-#define VR_PT_REGEN_LANES 16
+#define VR_PT_REGEN_LANES 24
 #endif
 
 template <int MODE, bool BATCH, int BLK, int LDSD>

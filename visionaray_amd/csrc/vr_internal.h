@@ -69,7 +69,8 @@ __host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count)
 // loads), boxes as structure of arrays so each load carries one coordinate
 // of all four children.  It is a collapse of the binary tree (pair records):
 // every child box is the box of a binary node, the same exact union.
-//   lo_x .. hi_z  child c's box component in lane c
+//   lo_x .. hi_z  child c's box component in lane c; an unused slot has
+//                 lo = hi = +inf, a box no ray enters
 //   child[c]      entry as in pair_node (inner: quad index)
 //   meta.x        number of children (2..4); meta.y..w unused
 struct alignas(128) quad_node {
@@ -230,14 +231,19 @@ struct trace_counters {
 
 constexpr int MAX_STACK = 192;
 
-// slab test of one child box; returns tnear, or +inf when the child is not entered
+// Slab test of one child box; returns tnear, or +inf when the child is not
+// entered.  The slab distances round exactly as the reference's; the min/max
+// reductions use the hardware min/max (v_min3/v_max3), which agree with the
+// reference's ternary forms on every non-NaN input up to the sign of a zero,
+// and a signed zero changes no comparison below.
 __device__ __forceinline__ float slab_enter(float lx, float ly, float lz, float hx, float hy, float hz, f3 ori,
                                             f3 inv, float tmin, float tmax, float tclosest)
 {
-    const f3 tlo = mul3(sub3(mk3(lx, ly, lz), ori), inv);
-    const f3 thi = mul3(sub3(mk3(hx, hy, hz), ori), inv);
-    const float tn = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-    const float tf = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+    const float x0 = (lx - ori.x) * inv.x, x1 = (hx - ori.x) * inv.x;
+    const float y0 = (ly - ori.y) * inv.y, y1 = (hy - ori.y) * inv.y;
+    const float z0 = (lz - ori.z) * inv.z, z1 = (hz - ori.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
     return (tf >= tn && tn < tclosest) ? tn : __builtin_inff();
 }
 
@@ -281,38 +287,50 @@ __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 
     T.r.tri = -1;
 }
 
-// One step: descend through inner nodes to a leaf, test the leaf's triangles
-// in order, take the next deferred entry.  Returns true when the query is
-// finished (T.r is its result).  A longer walk than the budget means a
-// corrupt tree and ends the query instead of looping.
+// entry of a finished query (never a valid leaf: first + count would exceed
+// any primitive count)
+constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
+
+// speculative traversal: a lane that reaches a leaf keeps it and goes on
+// descending while other lanes of the wave still look for theirs
+#ifndef VR_TRAV_SPECULATIVE
+#define VR_TRAV_SPECULATIVE 1
+#endif
+
+// One step: descend through inner nodes to a leaf, test one leaf's triangles
+// in order.  Returns true when the query is finished (T.r is its result).  A
+// longer walk than the budget means a corrupt tree and ends the query
+// instead of looping.  With VR_TRAV_SPECULATIVE the first leaf a lane meets
+// is postponed while it keeps descending until every lane of the wave still
+// in the loop holds one (Aila and Laine 2009); only the order in which
+// triangles are tested changes.
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
 __device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
                                           uint32_t* __restrict__ spill, trace_counters* cnt)
 {
-    if (b.num_prims == 0) return true;
+    if (b.num_prims == 0 || T.cur == TRAV_DONE) return true;
     auto push = [&](uint32_t e) {
         if (T.sp < LDS_DEPTH) stack[T.sp * STRIDE] = e;
         else spill[T.sp - LDS_DEPTH] = e;
         ++T.sp;
     };
     auto pop = [&]() -> uint32_t {
+        if (T.sp == 0) return TRAV_DONE;
         --T.sp;
         return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
     };
+    uint32_t leaf = 0;  // postponed leaf (leaf entries are never 0)
     while (!(T.cur & LEAF_BIT)) {
         if (--T.budget == 0 || T.cur >= b.num_quads) return true;
         const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
         const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
         const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
-        const uint32_t nch = reinterpret_cast<const uint32_t*>(p + 7)[0];
         if (COUNT) cnt->nodes++;
         const float tc = T.r.t;
         float k0 = slab_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T.ori, T.inv, T.tmin, T.tmax, tc);
         float k1 = slab_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, T.ori, T.inv, T.tmin, T.tmax, tc);
         float k2 = slab_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, T.ori, T.inv, T.tmin, T.tmax, tc);
         float k3 = slab_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, T.ori, T.inv, T.tmin, T.tmax, tc);
-        if (nch < 3) k2 = __builtin_inff();
-        if (nch < 4) k3 = __builtin_inff();
         uint32_t e0 = ch.x, e1 = ch.y, e2 = ch.z, e3 = ch.w;
         cswap(k0, e0, k1, e1);
         cswap(k2, e2, k3, e3);
@@ -320,18 +338,39 @@ __device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint
         cswap(k1, e1, k3, e3);
         cswap(k1, e1, k2, e2);
         if (!(k0 < __builtin_inff())) {
-            if (T.sp == 0) return true;
             T.cur = pop();
-            continue;
+        } else {
+            // deferred children, farthest deepest: e_k goes to sp + n - k
+            const int n = (k1 < __builtin_inff()) + (k2 < __builtin_inff()) + (k3 < __builtin_inff());
+            if (T.sp + 3 <= LDS_DEPTH) {
+                if (n >= 1) stack[(T.sp + n - 1) * STRIDE] = e1;
+                if (n >= 2) stack[(T.sp + n - 2) * STRIDE] = e2;
+                if (n >= 3) stack[T.sp * STRIDE] = e3;
+                T.sp += n;
+            } else {
+                if (n >= 3) push(e3);
+                if (n >= 2) push(e2);
+                if (n >= 1) push(e1);
+            }
+            T.cur = e0;
         }
-        if (k3 < __builtin_inff()) push(e3);
-        if (k2 < __builtin_inff()) push(e2);
-        if (k1 < __builtin_inff()) push(e1);
-        T.cur = e0;
+        if (VR_TRAV_SPECULATIVE) {
+            if ((T.cur & LEAF_BIT) && T.cur != TRAV_DONE && leaf == 0) {
+                leaf = T.cur;
+                T.cur = pop();
+            }
+            if (__ballot(leaf == 0) == 0) break;
+        }
     }
-    // leaf: test its triangles in order
-    const uint32_t first = T.cur & LEAF_FIRST_MASK;
-    const uint32_t count = ((T.cur >> 26) & 31u) + 1u;
+    // test one leaf: the postponed one, else the one the descent stopped at
+    uint32_t L = leaf;
+    if (L == 0) {
+        if (T.cur == TRAV_DONE) return true;
+        L = T.cur;
+        T.cur = pop();
+    }
+    const uint32_t first = L & LEAF_FIRST_MASK;
+    const uint32_t count = ((L >> 26) & 31u) + 1u;
     if (first + count > b.num_prims) return true;  // corrupt entry
     for (uint32_t k = first; k < first + count; ++k) {
         const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
@@ -348,9 +387,7 @@ __device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint
             }
         }
     }
-    if (T.sp == 0) return true;
-    T.cur = pop();
-    return false;
+    return T.cur == TRAV_DONE;
 }
 
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
