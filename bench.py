@@ -64,6 +64,8 @@ def main():
                     help="for hardware-counter runs: one build, one frame, and one device copy of "
                          "--copy-mb MiB (the calibration kernel of known traffic); prints a short JSON line")
     ap.add_argument("--copy-mb", type=int, default=1024)
+    ap.add_argument("--trace-spp", type=int, default=4,
+                    help="c1: bounce rays per pixel of the incoherent closest-hit batch (vr_trace); 0 skips it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -163,6 +165,9 @@ def main():
     log(f"per ray: {st[2] / rays_frame:.2f} node visits, {st[3] / rays_frame:.2f} triangles; "
         f"{alg_bytes / 1e9:.2f} GB algorithmic per frame -> {achieved:.0f} GB/s")
 
+    # ---- a batch of incoherent closest-hit queries (vr_trace) ----
+    closest = trace_batch_bench(args, V, tris, bvh, cam, stream) if args.trace_spp > 0 else None
+
     traffic = None
     tfile = ROOT / "profiles" / "pathtrace_c1_traffic.json"
     if tfile.exists():
@@ -201,6 +206,7 @@ def main():
         "total_mrays_per_s": round(total_mrays, 3),
         "rays_per_frame": {"primary": int(prim_per), "secondary": int(sec_per)},
         "adjoint_vs_fd_speedup": None,
+        "closest_hit_batch": closest,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -219,6 +225,86 @@ def main():
     print(line, flush=True)
     if args.out:
         Path(args.out).write_text(line + "\n")
+
+
+def cosine_about(nrm, rng):
+    """cosine-distributed unit directions about the unit normals nrm (n x 3)"""
+    n = nrm.shape[0]
+    t = np.where(np.abs(nrm[:, 0:1]) > 0.9, np.array([[0, 1, 0]], np.float32), np.array([[1, 0, 0]], np.float32))
+    u = np.cross(t, nrm)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = np.cross(nrm, u)
+    r1, r2 = rng.random(n, dtype=np.float32), rng.random(n, dtype=np.float32)
+    r, phi = np.sqrt(r1), 2 * np.pi * r2
+    d = u * (r * np.cos(phi))[:, None] + w * (r * np.sin(phi))[:, None] + nrm * np.sqrt(1 - r1)[:, None]
+    return (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+
+
+def secondary_rays(V, tris, bvh, cam, width, height, spp, seed=5):
+    """The first diffuse bounce of a frame: jittered primary rays through
+    every pixel (spp per pixel, pixel order), traced with vr_trace; at each
+    hit a cosine-distributed direction about the face normal turned toward
+    the viewer, origin offset 1e-4 along it.  Returns the bounce rays."""
+    rng = np.random.default_rng(seed)
+    n = width * height * spp
+    pix = np.repeat(np.arange(width * height), spp)
+    x = (pix % width).astype(np.float32) + rng.random(n, dtype=np.float32)
+    y = (pix // width).astype(np.float32) + rng.random(n, dtype=np.float32)
+    c = cam[0]
+    d = (c["U"][None, :] * (2 * x / width - 1)[:, None] + c["V"][None, :] * (2 * y / height - 1)[:, None]
+         + c["W"][None, :])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    prim = np.zeros(n, V.RAY_DTYPE)
+    prim["ori"] = c["eye"]
+    prim["dir"] = d
+    prim["tmax"] = np.finfo(np.float32).max
+    hits = V.closest_hit(bvh, prim)
+    hit = hits["prim_id"] >= 0
+    k = hits["prim_id"][hit]  # prim_id is the triangle index in these scenes
+    pos = prim["ori"][hit] + prim["dir"][hit] * hits["t"][hit][:, None]
+    nrm = np.cross(tris["e1"][k], tris["e2"][k])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    nrm *= np.where(np.sum(nrm * prim["dir"][hit], axis=1) > 0, -1.0, 1.0).astype(np.float32)[:, None]
+    dirs = cosine_about(nrm, rng)
+    rays = np.zeros(int(hit.sum()), V.RAY_DTYPE)
+    rays["ori"] = pos + 1e-4 * dirs
+    rays["dir"] = dirs
+    rays["tmax"] = np.finfo(np.float32).max
+    return rays
+
+
+def trace_batch_bench(args, V, tris, bvh, cam, stream, reps=3):
+    """vr_trace closest-hit over the first diffuse bounce of a 1920x1080 frame
+    at 4 rays per pixel (rays in device memory): queries per second (HIP
+    events) and node / triangle visits per query."""
+    t0 = time.time()
+    rays = secondary_rays(V, tris, bvh, cam, args.width, args.height, args.trace_spp)
+    d_rays = V.DeviceArray.from_host(rays)
+    d_hits = V.DeviceArray(rays.size * 4, np.float32)
+    lib = V.lib()
+    log(f"closest-hit batch: {rays.size} bounce rays generated in {time.time() - t0:.1f}s")
+    V.check(lib.vr_trace(bvh.handle, 0, d_rays.ptr, rays.size, d_hits.ptr, stream.handle))
+    ms = []
+    for _ in range(reps):
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        V.check(lib.vr_trace(bvh.handle, 0, d_rays.ptr, rays.size, d_hits.ptr, stream.handle))
+        e1.record(stream)
+        ms.append(e0.elapsed_ms(e1))
+    counts = V.DeviceArray(2 * rays.size, np.uint32)
+    V.check(lib.vr_trace_counted(bvh.handle, 0, d_rays.ptr, rays.size, d_hits.ptr, counts.ptr, stream.handle))
+    stream.synchronize()
+    c = counts.to_host().reshape(-1, 2).astype(np.float64)
+    hits = d_hits.to_host().view(np.int32).reshape(-1, 4)[:, 0]
+    t = float(np.median(ms))
+    out = {"rays": int(rays.size), "ms": round(t, 4), "grays_per_s": round(rays.size / (t * 1e-3) / 1e9, 4),
+           "hit_fraction": round(float(np.mean(hits >= 0)), 4),
+           "per_ray": {"nodes": round(float(c[:, 0].mean()), 3), "triangles": round(float(c[:, 1].mean()), 3)},
+           "rays_kind": f"first diffuse bounce of a {args.width}x{args.height} frame at {args.trace_spp} rays "
+                        "per pixel, pixel order"}
+    log(f"closest-hit batch: {t:.2f} ms -> {out['grays_per_s']:.3f} Grays/s; {out['per_ray']['nodes']:.2f} node "
+        f"visits, {out['per_ray']['triangles']:.2f} triangles per ray; {out['hit_fraction']:.3f} hit")
+    return out
 
 
 def profile_pass(args, V, S, tris, d_tris, d_mats, stream):

@@ -4,6 +4,7 @@
 #include "vr_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -16,7 +17,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
 int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st);
 int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st);
 int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
-                hipStream_t st);
+                uint32_t* work, hipStream_t st);
 int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
               const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
               unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st);
@@ -52,9 +53,14 @@ void make_scene_desc(void* out, const bvh_state& bs, const vr_material* mats, ui
 namespace {
 thread_local std::string g_error;
 
+// Work counters of persistent kernels: every launch takes the next of
+// WORK_SLOTS counters (each on its own 128-byte line) and resets it on its
+// stream first, so launches in flight on different streams never share one.
+constexpr uint32_t WORK_SLOTS = 1024;
 struct device_state {
     bool ready = false;
-    uint32_t* work = nullptr;  // persistent-kernel work counter
+    uint32_t* work = nullptr;  // WORK_SLOTS counters, 32 words apart
+    std::atomic<uint32_t> next{0};
 };
 std::mutex g_mutex;
 device_state g_dev[64];
@@ -117,10 +123,17 @@ static int ensure_device()
     if (ds->ready) return VR_OK;
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!ds->ready) {
-        VR_HIP_CHECK(hipMalloc(&ds->work, 256));
+        VR_HIP_CHECK(hipMalloc(&ds->work, (size_t)WORK_SLOTS * 128));
         ds->ready = true;
     }
     return VR_OK;
+}
+
+// the work counter of the next persistent launch on the current device
+static uint32_t* work_counter()
+{
+    device_state* ds = dev_state();
+    return ds->work + 32u * (ds->next.fetch_add(1u, std::memory_order_relaxed) % WORK_SLOTS);
 }
 
 extern "C" {
@@ -509,7 +522,9 @@ int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, 
 int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)
 {
     VR_CHECK(bvh && (n == 0 || (d_rays && d_hits)), VR_ERR_INVALID_ARGUMENT, "vr_trace: null argument");
-    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, nullptr, (hipStream_t)stream);
+    int rc = ensure_device();
+    if (rc) return rc;
+    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, nullptr, work_counter(), (hipStream_t)stream);
 }
 
 int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, uint32_t* d_counts,
@@ -517,7 +532,9 @@ int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr
 {
     VR_CHECK(bvh && (n == 0 || (d_rays && d_hits && d_counts)), VR_ERR_INVALID_ARGUMENT,
              "vr_trace_counted: null argument");
-    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, d_counts, (hipStream_t)stream);
+    int rc = ensure_device();
+    if (rc) return rc;
+    return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, d_counts, work_counter(), (hipStream_t)stream);
 }
 
 int vr_render_ao(vr_bvh bvh, const vr_camera* cam, const vr_sched_params* sp, uint32_t ao_samples, float ao_radius,
@@ -542,7 +559,7 @@ int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_material
     int rc = ensure_device();
     if (rc) return rc;
     return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
-                     reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work, false,
+                     reinterpret_cast<unsigned long long*>(d_ray_counts), work_counter(), false,
                      (hipStream_t)stream);
 }
 
@@ -560,7 +577,7 @@ int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_
     int rc = ensure_device();
     if (rc) return rc;
     return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
-                     reinterpret_cast<unsigned long long*>(d_stats), dev_state()->work, true, (hipStream_t)stream);
+                     reinterpret_cast<unsigned long long*>(d_stats), work_counter(), true, (hipStream_t)stream);
 }
 
 int vr_image_l2_loss(const float* d_image, const float* d_target, size_t num_pixels, float* d_dl_dimage,
@@ -584,7 +601,7 @@ int vr_render_pt_backward(vr_bvh bvh, const vr_material* d_materials, size_t num
     int rc = ensure_device();
     if (rc) return rc;
     return render_pt_adjoint(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_dl_dimage, d_grad,
-                             reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work,
+                             reinterpret_cast<unsigned long long*>(d_ray_counts), work_counter(),
                              (hipStream_t)stream);
 }
 
@@ -601,7 +618,7 @@ int vr_render_pt_frozen_rr(vr_bvh bvh, const vr_material* d_materials, const vr_
     int rc = ensure_device();
     if (rc) return rc;
     return render_pt_frozen_rr(bvh->s, d_materials, d_rr_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
-                               reinterpret_cast<unsigned long long*>(d_ray_counts), dev_state()->work,
+                               reinterpret_cast<unsigned long long*>(d_ray_counts), work_counter(),
                                (hipStream_t)stream);
 }
 
@@ -715,12 +732,12 @@ int vr_autotune_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_materi
             for (uint32_t t : tiles) {
                 const vr_launch_shape shp{b, d, t, VR_SCHEDULE_PERSISTENT};
                 rc = render_pt_shaped(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color, nullptr,
-                                      dev_state()->work, false, shp, st);
+                                      work_counter(), false, shp, st);
                 if (rc) break;
                 VR_HIP_CHECK(hipEventRecord(e0, st));
                 for (int r = 0; r < reps && rc == 0; ++r)
                     rc = render_pt_shaped(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
-                                          nullptr, dev_state()->work, false, shp, st);
+                                          nullptr, work_counter(), false, shp, st);
                 VR_HIP_CHECK(hipEventRecord(e1, st));
                 VR_HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.0f;

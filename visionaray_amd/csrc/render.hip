@@ -6,7 +6,8 @@
 // sequence (the reference draws all samples of a pixel from one generator,
 // detail/sched_common.h:427-526, so a pixel cannot be split across lanes
 // without changing its random stream).  Each lane holds one ray query in
-// flight.  The wave advances its queries step by step until 24 lanes have a
+// flight.  The wave advances its queries an iteration at a time (a node
+// visit or a triangle test, whichever more lanes need) until 24 lanes have a
 // finished query, then shades those lanes together: each starts its next
 // bounce, its next sample, or (pixel done) takes the next pixel, with one
 // atomic per wave.  Neither a long path nor a long query keeps the other
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         for (;;) {
             if (__ballot(tracing) == 0) break;
             if (__popcll(__ballot(pending)) >= VR_PT_REGEN_LANES) break;
-            if (tracing && trav_step<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc)) {
+            if (tracing && trav_iter<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc)) {
                 tracing = false;
                 pending = true;
             }

@@ -258,15 +258,16 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& ea, float& kb, uint32
     ea = e;
 }
 
-// One query in flight.  A query advances in steps (trav_step) so that a
-// lane can hold its query while other lanes of the wave shade and start new
-// ones; traverse() runs one query to the end.
+// One query in flight.  A query advances one iteration at a time
+// (trav_iter) so that a lane can hold its query while other lanes of the
+// wave shade or start new ones; traverse() runs one query to the end.
 struct trav_state {
     f3 ori, dir, inv;
     float tmin, tmax;
     uint32_t cur;     // entry being visited
     int sp;           // deferred entries on the stack
     uint32_t budget;  // visits left: a ray visits each node at most once
+    uint32_t lf, ln;  // trav_iter: triangles [lf, lf + ln) of a leaf still to test
     hit_t r;
 };
 
@@ -281,6 +282,8 @@ __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 
     T.cur = b.root;
     T.sp = 0;
     T.budget = 2u * b.num_prims + 2u;
+    T.lf = 0;
+    T.ln = 0;
     T.r.t = FLT_MAX;
     T.r.u = 0.0f;
     T.r.v = 0.0f;
@@ -291,24 +294,25 @@ __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 
 // any primitive count)
 constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
 
-// speculative traversal: a lane that reaches a leaf keeps it and goes on
-// descending while other lanes of the wave still look for theirs
-#ifndef VR_TRAV_SPECULATIVE
-#define VR_TRAV_SPECULATIVE 1
+// node iterations are chosen while at least this many times as many lanes
+// have node work as have a triangle to test (a node visit costs about twice a
+// triangle test)
+#ifndef VR_TRAV_NODE_BIAS
+#define VR_TRAV_NODE_BIAS 2.0f
 #endif
 
-// One step: descend through inner nodes to a leaf, test one leaf's triangles
-// in order.  Returns true when the query is finished (T.r is its result).  A
-// longer walk than the budget means a corrupt tree and ends the query
-// instead of looping.  With VR_TRAV_SPECULATIVE the first leaf a lane meets
-// is postponed while it keeps descending until every lane of the wave still
-// in the loop holds one (Aila and Laine 2009); only the order in which
-// triangles are tested changes.
+// One iteration of the query, either one node visit or one triangle test,
+// whichever more lanes of the wave can do (the wave votes).  A lane keeps
+// one leaf's triangle range and, after taking it, goes on descending (the
+// leaf's triangles wait for a triangle iteration), so lanes that meet leaves
+// at different depths stay busy; only the order in which triangles are
+// tested changes (Aila and Laine 2009, speculative traversal).  Returns true
+// when this lane's query is finished (T.r is its result).
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
-__device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
+__device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
                                           uint32_t* __restrict__ spill, trace_counters* cnt)
 {
-    if (b.num_prims == 0 || T.cur == TRAV_DONE) return true;
+    if (b.num_prims == 0) return true;
     auto push = [&](uint32_t e) {
         if (T.sp < LDS_DEPTH) stack[T.sp * STRIDE] = e;
         else spill[T.sp - LDS_DEPTH] = e;
@@ -319,8 +323,19 @@ __device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint
         --T.sp;
         return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
     };
-    uint32_t leaf = 0;  // postponed leaf (leaf entries are never 0)
-    while (!(T.cur & LEAF_BIT)) {
+    // a leaf at cur goes to the free triangle range
+    if (T.ln == 0 && (T.cur & LEAF_BIT) && T.cur != TRAV_DONE) {
+        T.lf = T.cur & LEAF_FIRST_MASK;
+        T.ln = ((T.cur >> 26) & 31u) + 1u;
+        if (T.lf + T.ln > b.num_prims) return true;  // corrupt entry
+        T.cur = pop();
+    }
+    const bool node_work = !(T.cur & LEAF_BIT);
+    if (!node_work && T.ln == 0) return true;  // cur is TRAV_DONE: finished
+    const uint32_t n_node = (uint32_t)__popcll(__ballot(node_work));
+    const uint32_t n_tri = (uint32_t)__popcll(__ballot(T.ln > 0));
+    if ((float)n_node >= VR_TRAV_NODE_BIAS * (float)n_tri) {
+        if (!node_work) return false;
         if (--T.budget == 0 || T.cur >= b.num_quads) return true;
         const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
         const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
@@ -354,40 +369,26 @@ __device__ __forceinline__ bool trav_step(trav_state& T, const bvh_view& b, uint
             }
             T.cur = e0;
         }
-        if (VR_TRAV_SPECULATIVE) {
-            if ((T.cur & LEAF_BIT) && T.cur != TRAV_DONE && leaf == 0) {
-                leaf = T.cur;
-                T.cur = pop();
-            }
-            if (__ballot(leaf == 0) == 0) break;
+        return false;
+    }
+    if (T.ln == 0) return false;
+    const uint32_t k = T.lf;
+    ++T.lf;
+    --T.ln;
+    const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
+    const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+    if (COUNT) cnt->tris++;
+    float t, u, v;
+    if (intersect_tri(T.ori, T.dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
+        if (t < T.r.t && t >= T.tmin && t <= T.tmax) {
+            T.r.t = t;
+            T.r.u = u;
+            T.r.v = v;
+            T.r.tri = (int32_t)k;
+            if (ANY_HIT) return true;
         }
     }
-    // test one leaf: the postponed one, else the one the descent stopped at
-    uint32_t L = leaf;
-    if (L == 0) {
-        if (T.cur == TRAV_DONE) return true;
-        L = T.cur;
-        T.cur = pop();
-    }
-    const uint32_t first = L & LEAF_FIRST_MASK;
-    const uint32_t count = ((L >> 26) & 31u) + 1u;
-    if (first + count > b.num_prims) return true;  // corrupt entry
-    for (uint32_t k = first; k < first + count; ++k) {
-        const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
-        const float4 ta = tp[0], tb = tp[1], tq = tp[2];
-        if (COUNT) cnt->tris++;
-        float t, u, v;
-        if (intersect_tri(T.ori, T.dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
-            if (t < T.r.t && t >= T.tmin && t <= T.tmax) {
-                T.r.t = t;
-                T.r.u = u;
-                T.r.v = v;
-                T.r.tri = (int32_t)k;
-                if (ANY_HIT) return true;
-            }
-        }
-    }
-    return T.cur == TRAV_DONE;
+    return false;
 }
 
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
@@ -397,7 +398,7 @@ __device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, flo
     trav_state T;
     uint32_t spill[MAX_STACK - LDS_DEPTH];
     trav_begin(T, b, ori, dir, tmin, tmax);
-    while (!trav_step<ANY_HIT, COUNT, LDS_DEPTH, STRIDE>(T, b, stack, spill, cnt)) {
+    while (!trav_iter<ANY_HIT, COUNT, LDS_DEPTH, STRIDE>(T, b, stack, spill, cnt)) {
     }
     return T.r;
 }
