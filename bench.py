@@ -64,6 +64,8 @@ def main():
                     help="for hardware-counter runs: one build, one frame, and one device copy of "
                          "--copy-mb MiB (the calibration kernel of known traffic); prints a short JSON line")
     ap.add_argument("--copy-mb", type=int, default=1024)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c1: skip the reference-scheme frame and the heightfield-only run")
     ap.add_argument("--trace-spp", type=int, default=4,
                     help="c1: bounce rays per pixel of the incoherent closest-hit batch (vr_trace); 0 skips it")
     args = ap.parse_args()
@@ -177,6 +179,12 @@ def main():
             traffic = None
 
     # ---- CPU baseline: the reference's own tiled_sched path ----
+    # ---- the same frame on the reference's GPU scheme, and the @4M point ----
+    extras = {}
+    if not args.no_extras:
+        extras["reference_scheme_frame"] = reference_scheme_frame(V, kernel, sp, stream, kernel_ms)
+        extras["heightfield_only"] = heightfield_only(args, V, S, stream)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream)
@@ -207,6 +215,7 @@ def main():
         "rays_per_frame": {"primary": int(prim_per), "secondary": int(sec_per)},
         "adjoint_vs_fd_speedup": None,
         "closest_hit_batch": closest,
+        **extras,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -225,6 +234,77 @@ def main():
     print(line, flush=True)
     if args.out:
         Path(args.out).write_text(line + "\n")
+
+
+def reference_scheme_frame(V, kernel, sp, stream, kernel_ms):
+    """One frame of the same workload as the reference's GPU path runs it:
+    one thread per pixel (16 x 16 pixel blocks), binary-tree traversal, no
+    work queue (vr_launch_shape schedule VR_SCHEDULE_PER_PIXEL)."""
+    V.set_launch_shape(256, 16, 16, V.SCHEDULE_PER_PIXEL)
+    try:
+        sched = V.hip_sched()
+        sched.frame(kernel, sp, stream=stream)
+        stream.synchronize()
+        sched.reset_counters()
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        sched.frame(kernel, sp, stream=stream)
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1)
+        sec = sched.rays()[1]
+    finally:
+        V.set_launch_shape(256, 16, 1)
+    out = {"ms": round(ms, 3), "incoherent_mrays_per_s": round(sec / (ms * 1e-3) / 1e6, 3),
+           "speedup_of_this_build": round(ms / kernel_ms, 3),
+           "scheme": "one thread per pixel in 16x16 blocks, binary-tree traversal (cuda_sched + intersect.inl)"}
+    log(f"reference-scheme frame: {ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} incoherent Mrays/s); "
+        f"this build x{out['speedup_of_this_build']:.2f}")
+    return out
+
+
+def heightfield_only(args, V, S, stream, steps=2):
+    """The literal @4M point: the 1448 x 1448-cell heightfield alone
+    (4,193,408 triangles), LBVH build and the same frame."""
+    tris = S.config1_scene(cells=args.cells, spheres=0)
+    mats = S.config1_materials(tris.size)
+    d_tris = V.DeviceArray.from_host(tris)
+    d_mats = V.DeviceArray.from_host(mats)
+    builder = V.lbvh_builder()
+    ms_b = []
+    bvh = None
+    for i in range(4):
+        if bvh is not None:
+            bvh.destroy()
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        bvh = builder.build(d_tris, keep_nodes=False, stream=stream)
+        e1.record(stream)
+        if i > 0:
+            ms_b.append(e0.elapsed_ms(e1))
+    eye, center, up, fovy, _ = S.config1_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, args.width / args.height)
+    kp = V.make_kernel_params(num_bounces=args.depth, epsilon=1e-4, bg_color=(0.0, 0.0, 0.0, 1.0),
+                              ambient_color=(0.6, 0.7, 0.9, 1.0))
+    kernel = V.pathtracing.kernel(bvh, d_mats, kp)
+    rt = V.render_target(args.width, args.height)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=args.spp, sfactor=1.0, dfactor=0.0), cam, rt)
+    sched.frame(kernel, sp, stream=stream)
+    stream.synchronize()
+    sched.reset_counters()
+    e0, e1 = V.Event(), V.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        sched.frame(kernel, sp, stream=stream)
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1) / steps
+    _, sec = sched.rays()
+    out = {"triangles": int(tris.size), "lbvh_build_ms": round(float(np.median(ms_b)), 4),
+           "frame_ms": round(ms, 3), "incoherent_mrays_per_s": round(sec / steps / (ms * 1e-3) / 1e6, 3)}
+    log(f"heightfield only ({tris.size} tris): LBVH {out['lbvh_build_ms']:.3f} ms, frame {ms:.2f} ms, "
+        f"{out['incoherent_mrays_per_s']:.1f} incoherent Mrays/s")
+    bvh.destroy()
+    return out
 
 
 def cosine_about(nrm, rng):

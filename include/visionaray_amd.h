@@ -272,9 +272,11 @@ int vr_render_pt_frozen_rr(vr_bvh bvh, const vr_material* d_materials, const vr_
  * entries spill to private memory), the pixel-queue tile (pixels are
  * handed out in tile x tile squares, 1 = scanlines; 1..64) and the schedule:
  * VR_SCHEDULE_PERSISTENT (a resident grid whose lanes take the next pixel
- * when theirs is done) or VR_SCHEDULE_PER_PIXEL (one thread per pixel over
- * the whole image, as cuda_sched launches it, detail/cuda_sched.inl:27-70;
- * stack_lds 16 only).  Batches always run persistent. */
+ * when theirs is done, 4-wide traversal) or VR_SCHEDULE_PER_PIXEL (the
+ * reference's GPU scheme, for comparisons: one thread per pixel over the
+ * whole image as cuda_sched launches it, detail/cuda_sched.inl:27-70, with
+ * the binary-tree traversal of detail/bvh/intersect.inl:166-356; stack_lds 16
+ * only; same image up to ties).  Batches always run persistent. */
 #define VR_SCHEDULE_PERSISTENT 0u
 #define VR_SCHEDULE_PER_PIXEL 1u
 typedef struct vr_launch_shape {

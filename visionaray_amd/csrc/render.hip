@@ -182,14 +182,39 @@ __device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t
     return (uint32_t)x < w && (uint32_t)y < h;
 }
 
-enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_PLAIN = 3, PT_FROZEN_RR = 4 };
+enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_FROZEN_RR = 4 };
 constexpr int ADJ_MAX_VERTICES = 16;
 
-// One kernel for the frame (PT_RENDER), the frame with traversal counters
-// (PT_COUNT) and the path-replay adjoint (PT_ADJOINT): all three draw the same
-// random numbers in the same order, so the adjoint replays exactly the paths
-// of the forward frame with the same seeds, recomputing every hit.
-//
+// State of one pixel's path (the reference keeps it in the locals of
+// pathtracing::kernel and sample_pixel, detail/pathtracing.inl:43-233,
+// detail/sched_common.h:427-526).
+template <bool ADJ>
+struct path_t {
+    rng_t g;
+    uint32_t s;       // sample
+    uint32_t bounce;
+    f3 tp, tp0, I;    // throughput, throughput under A.rr_mats (frozen roulette), radiance
+    bool rhit;        // the primary ray hit
+    float4 sum;       // samples so far
+    f3 wgt;           // adjoint: d loss / d sample
+    uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
+    uint32_t nmid;
+};
+
+template <bool ADJ>
+__device__ __forceinline__ void start_sample(path_t<ADJ>& P)
+{
+    P.tp = mk3(1, 1, 1);
+    P.tp0 = mk3(1, 1, 1);
+    P.I = mk3(0, 0, 0);
+    P.bounce = 0;
+    P.rhit = false;
+    if (ADJ) P.nmid = 0;
+}
+
+// One bounce of pathtracing::kernel for the query (o, d) with result hr:
+// emission / ambient contributions, the matte sample, Russian roulette.
+// Returns true when the path ended; otherwise (no, nd) is the next ray.
 // Adjoint (no counterpart in the reference, which has no differentiation):
 // a path's estimate is I = T * E, E the ending emitter's ce * ls or the
 // ambient term on escape, T the product over its matte vertices of
@@ -198,13 +223,142 @@ constexpr int ADJ_MAX_VERTICES = 16;
 //   d I_c / d cd_{m,c} = I_c * (vertices with material m) / cd_{m,c}
 //   d I_c / d ce_{e,c} = T_c * ls_e.
 // Per-block FP32 partials in LDS; FP64 only in the final reduction.
-//
-// PT_PLAIN renders the frame as the reference's cuda_sched launches it: one
-// thread per pixel over the whole image and no work queue (a lane whose pixel
-// is done idles).  PT_FROZEN_RR renders the frame with the roulette
-// probabilities taken from the throughput under A.rr_mats instead of the
-// shading materials: with rr_mats = mats it is the frame itself, and for
-// mats near rr_mats it is the estimator the adjoint differentiates (central
+template <int MODE>
+__device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A, path_t<MODE == PT_ADJOINT>& P,
+                                      const hit_t& hr, f3 o, f3 d, f3& no, f3& nd, float* s_grad)
+{
+    constexpr bool ADJ = MODE == PT_ADJOINT;
+    constexpr bool FROZEN = MODE == PT_FROZEN_RR;
+    if (hr.tri < 0) {
+        // escaped: ambient light times throughput (pathtracing.inl:69-77)
+        const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), P.tp);
+        P.I = add3(P.I, e);
+        if (ADJ) {
+            for (uint32_t j = 0; j < P.nmid; ++j) {
+                const vr_material* mj = S.mats + P.mids[j];
+                const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
+                atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
+                atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);
+                atomicAdd(&s_grad[slot + 2], P.wgt.z * e.z / mj->cd[2]);
+            }
+        }
+        return true;
+    }
+    if (P.bounce == 0) P.rhit = true;
+    const f3 view = neg3(d);
+    const f3 pos = add3(o, muls3(d, hr.t));
+    const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
+    const float4 tb = tq[1], tcq = tq[2];
+    const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
+    uint32_t mid = __float_as_uint(tb.w);
+    mid = mid < S.num_mats ? mid : S.num_mats - 1u;
+    const vr_material* m = S.mats + mid;
+    f3 refl = mk3(0.0f, 0.0f, 0.0f);
+    float pdf;
+    f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
+    const bool emission = m->type == 1;
+    if (emission) {
+        // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
+        const float ls = m->ls;
+        src = mk3((1.0f * m->ce[0]) * ls, (1.0f * m->ce[1]) * ls, (1.0f * m->ce[2]) * ls);
+        pdf = 1.0f;
+    } else {
+        // matte::sample -> lambertian::sample_f (matte.inl:43-56, brdf.h:50-84)
+        const f3 n = faceforward3(gn, view, gn);
+        f3 u, v;
+        onb(n, u, v);
+        float a1, a2;
+        rng_pair(P.g, A.arg_order, a1, a2);
+        const f3 sp = cosine_hemisphere(a1, a2);
+        refl = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
+        pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
+        const float kd = m->kd;
+        src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
+                  1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
+                  1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
+        if (FROZEN) {
+            const vr_material* m0 = A.rr_mats + mid;
+            const float kd0 = m0->kd;
+            src0 = mk3(1.0f * ((m0->cd[0] * kd0) * 3.18309886183790691216444201928e-01f),
+                       1.0f * ((m0->cd[1] * kd0) * 3.18309886183790691216444201928e-01f),
+                       1.0f * ((m0->cd[2] * kd0) * 3.18309886183790691216444201928e-01f));
+        }
+    }
+    const bool zero_pdf = pdf <= 0.0f;
+    bool active = true;
+    if (emission) {
+        // no light list: MIS weight 1 (pathtracing.inl:134-146)
+        const f3 e = mul3(mk3(1.0f * P.tp.x, 1.0f * P.tp.y, 1.0f * P.tp.z), src);
+        P.I = add3(P.I, e);
+        active = false;
+        if (ADJ) {
+            const float ls = m->ls;
+            const uint32_t es = 6 * (S.grad_base + mid);
+            atomicAdd(&s_grad[es + 3], P.wgt.x * P.tp.x * ls);
+            atomicAdd(&s_grad[es + 4], P.wgt.y * P.tp.y * ls);
+            atomicAdd(&s_grad[es + 5], P.wgt.z * P.tp.z * ls);
+            for (uint32_t j = 0; j < P.nmid; ++j) {
+                const vr_material* mj = S.mats + P.mids[j];
+                const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
+                atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
+                atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);
+                atomicAdd(&s_grad[slot + 2], P.wgt.z * e.z / mj->cd[2]);
+            }
+        }
+    } else if (ADJ && !zero_pdf && P.nmid < ADJ_MAX_VERTICES) {
+        P.mids[P.nmid++] = mid;
+    }
+    if (zero_pdf) active = false;
+    const f3 n = faceforward3(gn, view, gn);
+    const float q = dot3(n, refl) / pdf;
+    P.tp = mk3(P.tp.x * (src.x * q), P.tp.y * (src.y * q), P.tp.z * (src.z * q));
+    if (zero_pdf) P.tp = mk3(0.0f, 0.0f, 0.0f);
+    if (FROZEN) {
+        P.tp0 = emission ? P.tp : mk3(P.tp0.x * (src0.x * q), P.tp0.y * (src0.y * q), P.tp0.z * (src0.z * q));
+        if (zero_pdf) P.tp0 = mk3(0.0f, 0.0f, 0.0f);
+    }
+    if (P.bounce >= 2) {
+        // Russian roulette (pathtracing.inl:201-213)
+        const f3 rt = FROZEN ? P.tp0 : P.tp;
+        float prob = rt.x;
+        if (rt.y > prob) prob = rt.y;
+        if (rt.z > prob) prob = rt.z;
+        const bool terminate = rng_next(P.g) > prob;
+        if (terminate) active = false;
+        P.tp = mk3(P.tp.x / prob, P.tp.y / prob, P.tp.z / prob);
+        if (FROZEN) P.tp0 = mk3(P.tp0.x / prob, P.tp0.y / prob, P.tp0.z / prob);
+    }
+    if (!active) return true;
+    no = add3(pos, muls3(refl, S.kp.epsilon));
+    nd = refl;
+    ++P.bounce;
+    return P.bounce >= S.kp.num_bounces;
+}
+
+// A path ended: add its sample (the background on a primary miss); returns
+// true when the pixel's last sample is done.
+template <bool ADJ>
+__device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
+{
+    if (!ADJ) {
+        if (P.rhit) {
+            P.sum.x += P.I.x; P.sum.y += P.I.y; P.sum.z += P.I.z; P.sum.w += 1.0f;
+        } else {
+            P.sum.x += S.kp.bg[0]; P.sum.y += S.kp.bg[1]; P.sum.z += S.kp.bg[2]; P.sum.w += 1.0f;
+        }
+    }
+    ++P.s;
+    return P.s >= S.sp.spp;
+}
+
+// One kernel for the frame (PT_RENDER), the frame with traversal counters
+// (PT_COUNT), the path-replay adjoint (PT_ADJOINT) and the frozen-roulette
+// frame (PT_FROZEN_RR): all draw the same random numbers in the same order,
+// so the adjoint replays exactly the paths of the forward frame with the
+// same seeds, recomputing every hit.  PT_FROZEN_RR takes the roulette
+// probabilities from the throughput under A.rr_mats instead of the shading
+// materials: with rr_mats = mats it is the frame itself, and for mats near
+// rr_mats it is the estimator the adjoint differentiates (central
 // differences of it at common random numbers keep every roulette decision).
 //
 // BATCH: many scenes in one persistent launch.  The global pixel index runs
@@ -216,7 +370,7 @@ constexpr int ADJ_MAX_VERTICES = 16;
 // waves per SIMD the register allocation must leave room for (0: as many
 // registers as the kernel wants)
 #ifndef VR_PT_WAVES_PER_EU
-#define VR_PT_WAVES_PER_EU 6
+#define VR_PT_WAVES_PER_EU 5
 #endif
 #if VR_PT_WAVES_PER_EU > 0
 #define VR_PT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_PT_WAVES_PER_EU)))
@@ -234,8 +388,6 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = MODE == PT_COUNT;
     constexpr bool ADJ = MODE == PT_ADJOINT;
-    constexpr bool PLAIN = MODE == PT_PLAIN;
-    constexpr bool FROZEN = MODE == PT_FROZEN_RR;
     __shared__ uint32_t stack[LDSD * BLK];
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
@@ -251,19 +403,15 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 
     int32_t px = -1;
     bool done = false;
-    rng_t g{1u};
-    uint32_t s = 0, bounce = 0;
-    f3 tp = mk3(1, 1, 1), I = mk3(0, 0, 0);
-    f3 tp0 = mk3(1, 1, 1);  // FROZEN: throughput under A.rr_mats
-    bool fetched = false;   // PLAIN: the lane took its one pixel
-    bool rhit = false;
-    float4 sum = make_float4(0, 0, 0, 0);
+    path_t<ADJ> P;
+    P.g = rng_t{1u};
+    P.s = 0;
+    P.sum = make_float4(0, 0, 0, 0);
+    P.wgt = mk3(0, 0, 0);
+    start_sample(P);
     unsigned long long n_primary = 0, n_secondary = 0;
     trace_counters tc{0, 0};
     unsigned long long n_nodes = 0, n_tris = 0;
-    f3 wgt = mk3(0, 0, 0);
-    uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
-    uint32_t nmid = 0;
     // the lane's query: in flight (tracing) or finished and waiting to be
     // shaded (pending)
     trav_state T;
@@ -277,152 +425,28 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         if (pending) {
             pending = false;
             const scene_desc& S = SCN;
-            const hit_t hr = T.r;
-            const f3 o = T.ori, d = T.dir;
-            f3 no = o, nd = d;  // next ray
+            f3 no, nd;  // next ray
             if (COUNT) {
                 n_nodes += tc.nodes;
                 n_tris += tc.tris;
                 tc.nodes = 0;
                 tc.tris = 0;
             }
-            bool finished = false;
-            if (hr.tri < 0) {
-                // escaped: ambient light times throughput (pathtracing.inl:69-77)
-                const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), tp);
-                I = add3(I, e);
-                finished = true;
-                if (ADJ) {
-                    for (uint32_t j = 0; j < nmid; ++j) {
-                        const vr_material* mj = S.mats + mids[j];
-                        const uint32_t slot = 6 * (S.grad_base + mids[j]);
-                        atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
-                        atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
-                        atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
-                    }
-                }
-            } else {
-                if (bounce == 0) rhit = true;
-                const f3 view = neg3(d);
-                const f3 pos = add3(o, muls3(d, hr.t));
-                const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
-                const float4 tb = tq[1], tcq = tq[2];
-                const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
-                uint32_t mid = __float_as_uint(tb.w);
-                mid = mid < S.num_mats ? mid : S.num_mats - 1u;
-                const vr_material* m = S.mats + mid;
-                f3 refl = mk3(0.0f, 0.0f, 0.0f);
-                float pdf;
-                f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
-                const bool emission = m->type == 1;
-                if (emission) {
-                    // emissive::sample: tex_color * ce * ls, pdf 1 (emissive.inl:31-40)
-                    const float ls = m->ls;
-                    src = mk3((1.0f * m->ce[0]) * ls, (1.0f * m->ce[1]) * ls, (1.0f * m->ce[2]) * ls);
-                    pdf = 1.0f;
-                } else {
-                    // matte::sample -> lambertian::sample_f (matte.inl:43-56, brdf.h:50-84)
-                    const f3 n = faceforward3(gn, view, gn);
-                    f3 u, v;
-                    onb(n, u, v);
-                    float a1, a2;
-                    rng_pair(g, A.arg_order, a1, a2);
-                    const f3 sp = cosine_hemisphere(a1, a2);
-                    refl = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
-                    pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
-                    const float kd = m->kd;
-                    src = mk3(1.0f * ((m->cd[0] * kd) * 3.18309886183790691216444201928e-01f),
-                              1.0f * ((m->cd[1] * kd) * 3.18309886183790691216444201928e-01f),
-                              1.0f * ((m->cd[2] * kd) * 3.18309886183790691216444201928e-01f));
-                    if (FROZEN) {
-                        const vr_material* m0 = A.rr_mats + mid;
-                        const float kd0 = m0->kd;
-                        src0 = mk3(1.0f * ((m0->cd[0] * kd0) * 3.18309886183790691216444201928e-01f),
-                                   1.0f * ((m0->cd[1] * kd0) * 3.18309886183790691216444201928e-01f),
-                                   1.0f * ((m0->cd[2] * kd0) * 3.18309886183790691216444201928e-01f));
-                    }
-                }
-                const bool zero_pdf = pdf <= 0.0f;
-                bool active = true;
-                if (emission) {
-                    // no light list: MIS weight 1 (pathtracing.inl:134-146)
-                    const f3 e = mul3(mk3(1.0f * tp.x, 1.0f * tp.y, 1.0f * tp.z), src);
-                    I = add3(I, e);
-                    active = false;
-                    if (ADJ) {
-                        const float ls = m->ls;
-                        const uint32_t es = 6 * (S.grad_base + mid);
-                        atomicAdd(&s_grad[es + 3], wgt.x * tp.x * ls);
-                        atomicAdd(&s_grad[es + 4], wgt.y * tp.y * ls);
-                        atomicAdd(&s_grad[es + 5], wgt.z * tp.z * ls);
-                        for (uint32_t j = 0; j < nmid; ++j) {
-                            const vr_material* mj = S.mats + mids[j];
-                            const uint32_t slot = 6 * (S.grad_base + mids[j]);
-                            atomicAdd(&s_grad[slot + 0], wgt.x * e.x / mj->cd[0]);
-                            atomicAdd(&s_grad[slot + 1], wgt.y * e.y / mj->cd[1]);
-                            atomicAdd(&s_grad[slot + 2], wgt.z * e.z / mj->cd[2]);
-                        }
-                    }
-                } else if (ADJ && !zero_pdf && nmid < ADJ_MAX_VERTICES) {
-                    mids[nmid++] = mid;
-                }
-                if (zero_pdf) active = false;
-                const f3 n = faceforward3(gn, view, gn);
-                const float q = dot3(n, refl) / pdf;
-                tp = mk3(tp.x * (src.x * q), tp.y * (src.y * q), tp.z * (src.z * q));
-                if (zero_pdf) tp = mk3(0.0f, 0.0f, 0.0f);
-                if (FROZEN) {
-                    tp0 = emission ? tp : mk3(tp0.x * (src0.x * q), tp0.y * (src0.y * q), tp0.z * (src0.z * q));
-                    if (zero_pdf) tp0 = mk3(0.0f, 0.0f, 0.0f);
-                }
-                if (bounce >= 2) {
-                    // Russian roulette (pathtracing.inl:201-213)
-                    const f3 rt = FROZEN ? tp0 : tp;
-                    float prob = rt.x;
-                    if (rt.y > prob) prob = rt.y;
-                    if (rt.z > prob) prob = rt.z;
-                    const bool terminate = rng_next(g) > prob;
-                    if (terminate) active = false;
-                    tp = mk3(tp.x / prob, tp.y / prob, tp.z / prob);
-                    if (FROZEN) tp0 = mk3(tp0.x / prob, tp0.y / prob, tp0.z / prob);
-                }
-                if (!active) {
-                    finished = true;
-                } else {
-                    no = add3(pos, muls3(refl, S.kp.epsilon));
-                    nd = refl;
-                    ++bounce;
-                    if (bounce >= S.kp.num_bounces) finished = true;
-                }
-            }
-            if (finished) {
-                if (!ADJ) {
-                    if (rhit) {
-                        sum.x += I.x; sum.y += I.y; sum.z += I.z; sum.w += 1.0f;
-                    } else {
-                        sum.x += S.kp.bg[0]; sum.y += S.kp.bg[1]; sum.z += S.kp.bg[2]; sum.w += 1.0f;
-                    }
-                }
-                ++s;
-                if (ADJ) nmid = 0;
-                if (s >= S.sp.spp) {
-                    if (!ADJ) write_pixel(S, (uint32_t)px, sum);
+            if (shade<MODE>(S, A, P, T.r, T.ori, T.dir, no, nd, s_grad)) {
+                if (end_sample(S, P)) {
+                    if (!ADJ) write_pixel(S, (uint32_t)px, P.sum);
                     px = -1;
                 } else {
                     const uint32_t w = S.sp.width, h = S.sp.height;
                     const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
-                    primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, no, nd);
-                    tp = mk3(1, 1, 1);
-                    tp0 = mk3(1, 1, 1);
-                    I = mk3(0, 0, 0);
-                    bounce = 0;
-                    rhit = false;
+                    primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, no, nd);
+                    start_sample(P);
                 }
             }
             if (px >= 0) {
                 trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
                 tracing = true;
-                if (bounce == 0) ++n_primary;
+                if (P.bounce == 0) ++n_primary;
                 else ++n_secondary;
             }
         }
@@ -431,20 +455,12 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         const bool need = !done && px < 0;
         const uint64_t needm = __ballot(need);
         if (needm) {
+            const int leader = __ffsll((unsigned long long)needm) - 1;
             uint32_t base = 0;
-            if (!PLAIN) {
-                const int leader = __ffsll((unsigned long long)needm) - 1;
-                if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(needm));
-                base = __shfl(base, leader);
-            }
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(needm));
+            base = __shfl(base, leader);
             if (need) {
-                uint32_t my;
-                if (PLAIN) {
-                    my = fetched ? npix : blockIdx.x * (uint32_t)BLK + threadIdx.x;
-                    fetched = true;
-                } else {
-                    my = base + (uint32_t)lane_mask_lt_count(needm);
-                }
+                const uint32_t my = base + (uint32_t)lane_mask_lt_count(needm);
                 if (my < npix) {
                     uint32_t lpx = my;
                     if (BATCH) {
@@ -465,22 +481,17 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                     if (tile_pixel(lpx, w, h, A.tile, x, y)) {
                         lpx = (uint32_t)y * w + (uint32_t)x;
                         px = (int32_t)lpx;
-                        rng_seed(g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
-                        s = 0;
-                        sum = make_float4(0, 0, 0, 0);
+                        rng_seed(P.g, make_random_seed(y * (int)w + x, (int32_t)(S.sp.frame_id + frame_add)));
+                        P.s = 0;
+                        P.sum = make_float4(0, 0, 0, 0);
                         f3 o, d;
-                        primary_ray(S.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
-                        tp = mk3(1, 1, 1);
-                        tp0 = mk3(1, 1, 1);
-                        I = mk3(0, 0, 0);
-                        bounce = 0;
-                        rhit = false;
+                        primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                        start_sample(P);
                         if (ADJ) {
                             // the image is accum = mean(samples) * sfactor + ...
                             const float4 dl = reinterpret_cast<const float4*>(S.dl)[lpx];
                             const float k = S.sp.sfactor / (float)S.sp.spp;
-                            wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
-                            nmid = 0;
+                            P.wgt = mk3(dl.x * k, dl.y * k, dl.z * k);
                         }
                         trav_begin(T, S.b, o, d, 0.0f, FLT_MAX);
                         tracing = true;
@@ -534,6 +545,59 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
     }
 #undef SCN
 }
+
+// The frame as the reference's GPU path renders it, for comparisons: the
+// render kernel of cuda_sched (detail/cuda_sched.inl:27-70) launches one
+// thread per pixel over the image, which runs its samples and bounces in
+// sequence (sample_pixel, detail/sched_common.h:427-526), tracing each ray
+// with the binary-tree traversal of detail/bvh/intersect.inl:166-356 (the
+// pair records, near child first); a thread whose pixel is done idles.  The
+// same random numbers as k_pathtrace, so the same image up to ties.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_pathtrace_plain(pt_args A)
+{
+    __shared__ uint32_t stack[LDS_STACK * BLK];
+    uint32_t* st = stack + threadIdx.x;
+    const scene_desc& S = A.single;
+    const uint32_t w = S.sp.width, h = S.sp.height;
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const uint32_t idx = blockIdx.x * (uint32_t)BLK + threadIdx.x;
+    unsigned long long n_primary = 0, n_secondary = 0;
+    int x, y;
+    if (idx < padded_pixels(w, h, A.tile) && tile_pixel(idx, w, h, A.tile, x, y)) {
+        const uint32_t px = (uint32_t)y * w + (uint32_t)x;
+        path_t<false> P;
+        rng_seed(P.g, make_random_seed(y * (int)w + x, (int32_t)S.sp.frame_id));
+        P.sum = make_float4(0, 0, 0, 0);
+        for (P.s = 0; P.s < S.sp.spp;) {
+            f3 o, d;
+            primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, o, d);
+            start_sample(P);
+            for (;;) {
+                if (P.bounce == 0) ++n_primary;
+                else ++n_secondary;
+                const hit_t hr = traverse_binary<false, LDS_STACK, BLK>(S.b, o, d, 0.0f, FLT_MAX, st);
+                f3 no, nd;
+                if (shade<PT_RENDER>(S, A, P, hr, o, d, no, nd, nullptr)) break;
+                o = no;
+                d = nd;
+            }
+            end_sample(S, P);
+        }
+        write_pixel(S, px, P.sum);
+    }
+    if (A.ray_counts) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            n_primary += __shfl_xor(n_primary, off);
+            n_secondary += __shfl_xor(n_secondary, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&A.ray_counts[0], n_primary);
+            atomicAdd(&A.ray_counts[1], n_secondary);
+        }
+    }
+}
+
 
 
 struct ao_args {
@@ -622,12 +686,13 @@ int launch_variant(const pt_args& A, size_t dyn_lds, hipStream_t st)
     return VR_OK;
 }
 
-// One thread per pixel (the reference's cuda_sched): a grid over the queue.
+// One thread per pixel with the reference's binary traversal (cuda_sched's
+// launch): a grid over the pixel order.
 template <int BLK>
 int launch_plain(const pt_args& A, hipStream_t st)
 {
     const uint32_t npix = padded_pixels(A.single.sp.width, A.single.sp.height, A.tile);
-    hipLaunchKernelGGL((k_pathtrace<PT_PLAIN, false, BLK, LDS_STACK>), dim3(div_up(npix, BLK)), dim3(BLK), 0, st, A);
+    hipLaunchKernelGGL((k_pathtrace_plain<BLK>), dim3(div_up(npix, BLK)), dim3(BLK), 0, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }

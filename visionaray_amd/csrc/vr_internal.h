@@ -94,9 +94,10 @@ static_assert(sizeof(tri_node) == 48, "tri_node layout");
 struct bvh_view {
     const quad_node* quads;
     const tri_node* tris;
+    const pair_node* pairs;  // the binary tree (traverse_binary)
     uint32_t num_prims;
-    uint32_t root;  // entry of the root: 0 (quad 0) or a leaf entry; unused when num_prims == 0
-    uint32_t num_quads;
+    uint32_t root;  // entry of the root: 0 (quad 0 / pair 0) or a leaf entry; unused when num_prims == 0
+    uint32_t num_quads;  // also the number of pair records
 };
 
 // Host-side state behind a vr_bvh handle.
@@ -125,7 +126,7 @@ struct bvh_state {
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
-    bvh_view view() const { return bvh_view{quads, tris, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
+    bvh_view view() const { return bvh_view{quads, tris, pairs, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
 };
 
 // ---------------------------------------------------------------------------
@@ -401,6 +402,95 @@ __device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, flo
     while (!trav_iter<ANY_HIT, COUNT, LDS_DEPTH, STRIDE>(T, b, stack, spill, cnt)) {
     }
     return T.r;
+}
+
+// The reference's traversal of the binary tree as written
+// (detail/bvh/intersect.inl:166-356, device order): at a pair both child
+// boxes are slab tested with the reference's exact arithmetic, the nearer
+// entered child (tnear0 < tnear1 ? 0 : 1) is visited first and the other is
+// deferred; triangles are accepted with is_closer.  Used for the
+// reference-scheme comparison launch, not by the fast paths.
+template <bool ANY_HIT, int LDS_DEPTH, int STRIDE>
+__device__ __forceinline__ hit_t traverse_binary(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
+                                                 uint32_t* __restrict__ stack)
+{
+    hit_t r;
+    r.t = FLT_MAX;
+    r.u = 0.0f;
+    r.v = 0.0f;
+    r.tri = -1;
+    if (b.num_prims == 0) return r;
+    const f3 inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
+                       dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
+    int sp = 0;
+    uint32_t cur = b.root;
+    uint32_t spill[MAX_STACK - LDS_DEPTH];
+    auto push = [&](uint32_t e) {
+        if (sp < LDS_DEPTH) stack[sp * STRIDE] = e;
+        else spill[sp - LDS_DEPTH] = e;
+        ++sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        --sp;
+        return sp < LDS_DEPTH ? stack[sp * STRIDE] : spill[sp - LDS_DEPTH];
+    };
+    // a ray visits each node at most once; a longer walk means a corrupt tree
+    uint32_t budget = 2u * b.num_prims + 2u;
+    for (;;) {
+        while (!(cur & LEAF_BIT)) {
+            if (--budget == 0 || cur >= b.num_quads) return r;
+            const float4* p = reinterpret_cast<const float4*>(b.pairs + cur);
+            const float4 a0 = p[0], q1 = p[1], q2 = p[2];
+            const uint4 q3 = *reinterpret_cast<const uint4*>(p + 3);
+            float tn0, tf0, tn1, tf1;
+            {
+                const f3 tlo = mul3(sub3(mk3(a0.x, a0.y, a0.z), ori), inv);
+                const f3 thi = mul3(sub3(mk3(a0.w, q1.x, q1.y), ori), inv);
+                tn0 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
+                tf0 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+            }
+            {
+                const f3 tlo = mul3(sub3(mk3(q1.z, q1.w, q2.x), ori), inv);
+                const f3 thi = mul3(sub3(mk3(q2.y, q2.z, q2.w), ori), inv);
+                tn1 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
+                tf1 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
+            }
+            const bool h0 = tf0 >= tn0 && tn0 < r.t;
+            const bool h1 = tf1 >= tn1 && tn1 < r.t;
+            if (h0 && h1) {
+                const bool near1 = !(tn0 < tn1);
+                if (sp >= MAX_STACK) return r;  // cannot happen for trees that pass stack_fits
+                push(near1 ? q3.x : q3.y);
+                cur = near1 ? q3.y : q3.x;
+            } else if (h0) {
+                cur = q3.x;
+            } else if (h1) {
+                cur = q3.y;
+            } else {
+                if (sp == 0) return r;
+                cur = pop();
+            }
+        }
+        const uint32_t first = cur & LEAF_FIRST_MASK;
+        const uint32_t count = ((cur >> 26) & 31u) + 1u;
+        if (first + count > b.num_prims) return r;  // corrupt entry
+        for (uint32_t k = first; k < first + count; ++k) {
+            const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
+            const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+            float t, u, v;
+            if (intersect_tri(ori, dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
+                if (t < r.t && t >= tmin && t <= tmax) {
+                    r.t = t;
+                    r.u = u;
+                    r.v = v;
+                    r.tri = (int32_t)k;
+                    if (ANY_HIT) return r;
+                }
+            }
+        }
+        if (sp == 0) return r;
+        cur = pop();
+    }
 }
 
 // LDS part of the traversal stack (entries per lane)

@@ -56,8 +56,9 @@ def bench_batched(args, V, S, log):
         p0, s0 = sched.rays()
         return ms, (p0 + s0) / steps
 
-    # plain launches: one thread per pixel in 16 x 16 pixel blocks, one launch
-    # per scene (cuda_sched::frame per scene)
+    # plain launches, the reference's GPU scheme: one thread per pixel in
+    # 16 x 16 pixel blocks with the binary-tree traversal, one launch per
+    # scene (cuda_sched::frame per scene)
     plain_ms, plain_rays = per_scene(dict(block_size=256, stack_lds=16, tile=16, schedule=V.SCHEDULE_PER_PIXEL))
     # the same launches on the persistent scheduler
     pers_ms, pers_rays = per_scene(dict(block_size=256, stack_lds=16, tile=1))
@@ -97,7 +98,8 @@ def bench_batched(args, V, S, log):
                                "resolutions from {256^2, 512^2, 1024^2, 1920x1080}, 4 spp, depth 4",
                    "scenes": count, "triangles": int(total_tris), "batch_shape": shape},
         "per_scene_plain_launches": {"ms": round(plain_ms, 4), "mrays_per_s": round(plain_rate, 3),
-                                     "schedule": "one thread per pixel, 16x16 pixel blocks (cuda_sched)"},
+                                     "schedule": "one thread per pixel, 16x16 pixel blocks, binary-tree "
+                                                 "traversal (the reference's cuda_sched scheme)"},
         "per_scene_persistent_launches": {"ms": round(pers_ms, 4), "mrays_per_s": round(pers_rate, 3)},
         "batched_graph": {"ms": round(batch_ms, 4), "mrays_per_s": round(batch_rate, 3)},
         "speedup": round(batch_rate / plain_rate, 3),
