@@ -160,6 +160,11 @@ int vr_camera_look_at(const float eye[3], const float center[3], const float up[
  * hierarchy, bottom-up boxes.  d_prims stays owned by the caller and is
  * copied into the BVH.  n == 0 yields an empty BVH. */
 int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* stream, vr_bvh* out);
+/* The same build into an existing LBVH handle (a new frame of moving
+ * geometry): its device arrays are reused when n is unchanged, so no device
+ * memory is allocated; the node layout is kept if the handle kept it.  The
+ * reference rebuilds with a new lbvh_builder::build call per frame. */
+int vr_lbvh_rebuild(vr_bvh bvh, const vr_triangle* d_prims, size_t n, void* stream);
 
 /* assign_morton_codes (detail/bvh/lbvh.h:242-269) alone: codes[i] for
  * primitive i, unsorted. */

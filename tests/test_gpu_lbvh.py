@@ -129,3 +129,52 @@ def test_large_lbvh_properties(V):
     lo = np.minimum(nodes["lo"][c0], nodes["lo"][c0 + 1])
     hi = np.maximum(nodes["hi"][c0], nodes["hi"][c0 + 1])
     assert np.array_equal(lo, nodes["lo"][inner]) and np.array_equal(hi, nodes["hi"][inner])
+
+
+def _moved(tris, seed=3):
+    moved = tris.copy()
+    rng = np.random.default_rng(seed)
+    moved["v1"] += (0.01 * np.sin(rng.uniform(0, 6.28, size=(tris.size, 1)) + moved["v1"] * 7)).astype(np.float32)
+    return moved
+
+
+def _rays(n, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, [("ori", "<f4", (3,)), ("dir", "<f4", (3,)), ("tmin", "<f4"), ("tmax", "<f4")])
+    r["ori"] = rng.uniform(-0.9, 0.9, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r["tmax"] = np.finfo(np.float32).max
+    return r
+
+
+def test_rebuild_in_place_matches_fresh_build(V):
+    """vr_lbvh_rebuild into an existing handle gives the fresh build's tree
+    and the same hits."""
+    tris = SCENES["clusters_150k"]
+    moved = _moved(tris)
+    bvh = V.lbvh_builder().build(tris)
+    bvh.rebuild(moved)
+    fresh = V.lbvh_builder().build(moved)
+    assert bvh.nodes().tobytes() == fresh.nodes().tobytes()
+    rays = _rays(20_000, 12)
+    a, b = V.closest_hit(bvh, rays), V.closest_hit(fresh, rays)
+    assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("name", ["c0_level5", "clusters_150k"])
+def test_refit_traversal_matches_oracle(V, O, name):
+    """After a refit the traversal nodes carry the new boxes: hits on the
+    moved geometry match the oracle's traversal of the refitted tree."""
+    tris = SCENES[name]
+    bvh = V.lbvh_builder().build(tris)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    moved = _moved(tris, seed=5)
+    bvh.refit(moved)
+    rnodes = O.refit(nodes, moved, idx)
+    rays = _rays(20_000, 13)
+    for any_hit in (False, True):
+        got = V.trace(bvh, rays, any_hit)
+        want = O.trace(rnodes, moved, idx, rays, any_hit)
+        ties = O.check_hits(got, want, moved, rays, any_hit)
+        assert ties <= 20

@@ -185,6 +185,13 @@ class BVH:
         check(lib().vr_bvh_sorted_refs(self.handle, c.ptr, i.ptr, None))
         return c.to_host(), i.to_host()
 
+    def rebuild(self, prims, stream=None):
+        """vr_lbvh_rebuild: a new LBVH of `prims` into this handle, reusing its
+        device arrays when the primitive count is unchanged."""
+        p = _as_device(prims, TRI_DTYPE)
+        check(lib().vr_lbvh_rebuild(self.handle, p.ptr, p.count, _stream(stream)))
+        self.prims = p
+
     def refit(self, prims, stream=None):
         """bvh_refitter::refit (detail/bvh/refit.h:28-114) for moved vertices."""
         p = _as_device(prims, TRI_DTYPE)

@@ -46,6 +46,7 @@ SIGNATURES = {
     "vr_device_copy": (I, [P, P, SZ, P]),
     "vr_camera_look_at": (I, [P, P, P, F, F, P]),
     "vr_lbvh_build": (I, [P, SZ, U32, P, C.POINTER(P)]),
+    "vr_lbvh_rebuild": (I, [P, P, SZ, P]),
     "vr_lbvh_morton": (I, [P, SZ, P, P]),
     "vr_bvh_sorted_refs": (I, [P, P, P, P]),
     "vr_bvh_from_nodes": (I, [P, SZ, P, SZ, P, C.POINTER(P)]),

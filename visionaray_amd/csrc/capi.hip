@@ -301,6 +301,18 @@ int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* st
     VR_TRY_END
 }
 
+int vr_lbvh_rebuild(vr_bvh bvh, const vr_triangle* d_prims, size_t n, void* stream)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(bvh, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_rebuild: null handle");
+    VR_CHECK(bvh->s.is_lbvh || bvh->s.num_prims == 0, VR_ERR_UNSUPPORTED, "vr_lbvh_rebuild: not an LBVH handle");
+    VR_CHECK(n == 0 || d_prims, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_rebuild: null primitives");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, (hipStream_t)stream);
+    VR_TRY_END
+}
+
 int vr_lbvh_morton(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, void* stream)
 {
     VR_CHECK(n == 0 || (d_prims && d_codes), VR_ERR_INVALID_ARGUMENT, "vr_lbvh_morton: null pointer");

@@ -713,7 +713,9 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
                          bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
         for (void* q : owned)
             if (q) (void)hipFree(q);
+        const int device = bs.device;
         bs = bvh_state{};
+        bs.device = device;
         bs.num_prims = n;
         bs.num_pairs = npairs;
         bs.is_lbvh = true;
@@ -805,8 +807,8 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
                            bs.num_pairs, 0u);
     launch_bottom_up(bs, s, bs.ref_nodes, st);
     VR_HIP_CHECK(hipGetLastError());
-    // the collapse reads the new boxes; the depth is the build's
-    return collapse_quads(bs, LBVH_LEAF_MAX, st);
+    // same topology: the traversal nodes take the new boxes
+    return refit_quads(bs, st);
 }
 
 int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)

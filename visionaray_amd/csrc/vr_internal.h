@@ -510,5 +510,8 @@ inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32
 // the range end) turns inner subtrees of at most leaf_max triangles into one
 // leaf over their contiguous range.  Needs s.depth.
 int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
+// After the pair boxes of `s` were refitted: copy every traversal child box
+// again from the pair record that holds it (the collapse is kept).
+int refit_quads(bvh_state& s, hipStream_t st);
 
 }  // namespace vr

@@ -327,12 +327,10 @@ def bench_dynamic(args, V, S, log):
         fwd_ms.append(timed(lambda: b.render(st)))
         bwd_ms.append(timed(lambda: b.backward(grad, st)))
     prim, sec = b.rays()
-    # rebuild cost for comparison (fresh builds of the same moved geometry)
+    # rebuild cost for comparison: a new LBVH of the same moved geometry into
+    # each scene's handle (vr_lbvh_rebuild, no allocation)
     for f in range(frames):
-        def rebuild():
-            for it in items:
-                V.lbvh_builder().build(it["moved"][f], keep_nodes=False, stream=st).destroy()
-        rebuild_ms.append(timed(rebuild))
+        rebuild_ms.append(timed(lambda: [it["bvh"].rebuild(it["moved"][f], stream=st) for it in items]))
     frame_ms = np.mean(refit_ms) + np.mean(fwd_ms) + np.mean(bwd_ms)
     rays_s = (prim + sec) / (frames * frame_ms * 1e-3) / 1e6
     grads_s = count / (frame_ms * 1e-3)
