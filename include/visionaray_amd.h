@@ -336,6 +336,39 @@ int vr_batch_capture_graph(vr_batch batch, uint64_t* d_ray_counts, void* stream)
 int vr_batch_launch_graph(vr_batch batch, void* stream);
 int vr_batch_destroy(vr_batch batch);
 
+/* ---- the minimum export set named for the drop-in boundary -------------
+ * The same operations under the names of the boundary a binding expects
+ * (a context per host thread, scene upload, node download, closest / any
+ * queries, forward / backward, autotune, destroy, error string); each is a
+ * thin equivalent of the entry point named in its comment. */
+typedef struct vr_context_t* vr_context;
+/* vr_init(device) and a handle that records the device; the library keeps
+ * its per-device state itself, so other calls do not take the context. */
+int vr_create_context(int device, vr_context* out);
+int vr_destroy_context(vr_context ctx);
+/* Device copy of n host triangles (44-B reference records) in *d_out,
+ * released with vr_free. */
+int vr_scene_upload(const vr_triangle* h_prims, size_t n, vr_triangle** d_out, void* stream);
+/* The reference node layout (vr_bvh_nodes) into host memory: at most
+ * *num_nodes records; *num_nodes returns the count (call with h_nodes NULL
+ * to query it). */
+int vr_bvh_download_nodes(vr_bvh bvh, vr_bvh_node* h_nodes, size_t* num_nodes);
+/* vr_trace with any_hit 0 / 1. */
+int vr_trace_closest(vr_bvh bvh, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
+int vr_trace_any(vr_bvh bvh, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
+/* vr_render_pt, the forward half of vr_render_pt_backward. */
+int vr_render_pt_forward(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                         const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                         uint64_t* d_ray_counts, void* stream);
+/* vr_autotune_pt. */
+int vr_autotune(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color, int reps,
+                vr_launch_shape* best, float* best_ms, void* stream);
+/* vr_bvh_destroy, vr_batch_destroy, vr_last_error. */
+int vr_destroy_bvh(vr_bvh bvh);
+int vr_destroy_batch(vr_batch batch);
+const char* vr_last_error_string(void);
+
 #ifdef __cplusplus
 }
 #endif

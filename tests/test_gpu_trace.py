@@ -124,3 +124,38 @@ def test_get_normal_kat(V):
         r["tmax"] = FMAX
         h = V.closest_hit(bvh, r)[0]
         assert h["prim_id"] == 0 and h["t"] == np.float32(1.0)
+
+
+def test_boundary_export_set(V, O):
+    """The boundary's export-set names behave as the entry points they stand
+    for: context, scene upload, host node download, closest / any queries,
+    destroy, error string."""
+    import ctypes as C
+    from visionaray_amd._lib import check, ptr
+    lib = V.lib()
+    ctx = C.c_void_p()
+    check(lib.vr_create_context(0, C.byref(ctx)))
+    tris = S.config0_scene(level=3)
+    d = C.c_void_p()
+    check(lib.vr_scene_upload(ptr(tris), tris.size, C.byref(d), None))
+    h = C.c_void_p()
+    check(lib.vr_lbvh_build(d, tris.size, 1, None, C.byref(h)))
+    n = C.c_size_t(0)
+    check(lib.vr_bvh_download_nodes(h, None, C.byref(n)))
+    assert n.value == 2 * tris.size - 1
+    nodes = np.zeros(n.value, V.NODE_DTYPE)
+    check(lib.vr_bvh_download_nodes(h, ptr(nodes), C.byref(n)))
+    want, _, _, _ = O.lbvh_build(tris)
+    assert np.array_equal(nodes["first"], want["first"]) and np.array_equal(nodes["lo"], want["lo"])
+    rays = random_rays(4096, seed=21)
+    d_rays = V.DeviceArray.from_host(rays)
+    for fn, flag in ((lib.vr_trace_closest, 0), (lib.vr_trace_any, 1)):
+        a, b = V.DeviceArray(4096, V.HIT_DTYPE), V.DeviceArray(4096, V.HIT_DTYPE)
+        check(fn(h, d_rays.ptr, 4096, a.ptr, None))
+        check(lib.vr_trace(h, flag, d_rays.ptr, 4096, b.ptr, None))
+        assert a.to_host().tobytes() == b.to_host().tobytes()
+    assert lib.vr_trace_closest(None, d_rays.ptr, 4096, None, None) != 0
+    assert b"null" in lib.vr_last_error_string()
+    check(lib.vr_destroy_bvh(h))
+    check(lib.vr_free(d))
+    check(lib.vr_destroy_context(ctx))

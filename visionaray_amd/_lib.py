@@ -74,6 +74,17 @@ SIGNATURES = {
     "vr_batch_capture_graph": (I, [P, P, P]),
     "vr_batch_launch_graph": (I, [P, P]),
     "vr_batch_destroy": (I, [P]),
+    "vr_create_context": (I, [I, C.POINTER(P)]),
+    "vr_destroy_context": (I, [P]),
+    "vr_scene_upload": (I, [P, SZ, C.POINTER(P), P]),
+    "vr_bvh_download_nodes": (I, [P, P, C.POINTER(SZ)]),
+    "vr_trace_closest": (I, [P, P, SZ, P, P]),
+    "vr_trace_any": (I, [P, P, SZ, P, P]),
+    "vr_render_pt_forward": (I, [P, P, SZ, P, P, P, P, P, P, P]),
+    "vr_autotune": (I, [P, P, SZ, P, P, P, P, P, I, P, P, P]),
+    "vr_destroy_bvh": (I, [P]),
+    "vr_destroy_batch": (I, [P]),
+    "vr_last_error_string": (C.c_char_p, []),
 }
 
 # record layouts shared with the C header

@@ -815,4 +815,96 @@ int vr_batch_destroy(vr_batch batch)
     return VR_OK;
 }
 
+// ---- the minimum export set named for the drop-in boundary ----
+
+struct vr_context_t {
+    int device = 0;
+};
+
+int vr_create_context(int device, vr_context* out)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(out, VR_ERR_INVALID_ARGUMENT, "vr_create_context: null out handle");
+    const int rc = vr_init(device);
+    if (rc) return rc;
+    vr_context c = new vr_context_t;
+    c->device = device;
+    *out = c;
+    return VR_OK;
+    VR_TRY_END
+}
+
+int vr_destroy_context(vr_context ctx)
+{
+    delete ctx;
+    return VR_OK;
+}
+
+int vr_scene_upload(const vr_triangle* h_prims, size_t n, vr_triangle** d_out, void* stream)
+{
+    VR_CHECK(d_out && (n == 0 || h_prims), VR_ERR_INVALID_ARGUMENT, "vr_scene_upload: null argument");
+    *d_out = nullptr;
+    if (n == 0) return VR_OK;
+    int rc = ensure_device();
+    if (rc) return rc;
+    void* d = nullptr;
+    VR_HIP_CHECK(hipMalloc(&d, n * sizeof(vr_triangle)));
+    if (hipMemcpyAsync(d, h_prims, n * sizeof(vr_triangle), hipMemcpyHostToDevice, (hipStream_t)stream) != hipSuccess ||
+        hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(VR_ERR_HIP, "vr_scene_upload: copy failed");
+    }
+    *d_out = static_cast<vr_triangle*>(d);
+    return VR_OK;
+}
+
+int vr_bvh_download_nodes(vr_bvh bvh, vr_bvh_node* h_nodes, size_t* num_nodes)
+{
+    VR_CHECK(bvh && num_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_download_nodes: null argument");
+    const bvh_state& s = bvh->s;
+    const size_t nn = s.num_prims == 0 ? 0 : (s.is_lbvh ? 2 * s.num_prims - 1 : s.num_ref_nodes);
+    if (!h_nodes) {
+        *num_nodes = nn;
+        return VR_OK;
+    }
+    VR_CHECK(*num_nodes >= nn, VR_ERR_INVALID_ARGUMENT, "vr_bvh_download_nodes: buffer too small");
+    *num_nodes = nn;
+    if (nn == 0) return VR_OK;
+    VR_CHECK(s.ref_nodes, VR_ERR_UNSUPPORTED,
+             "vr_bvh_download_nodes: build with VR_BVH_KEEP_NODES to keep the node layout");
+    VR_HIP_CHECK(hipDeviceSynchronize());
+    VR_HIP_CHECK(hipMemcpy(h_nodes, s.ref_nodes, nn * sizeof(vr_bvh_node), hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_trace_closest(vr_bvh bvh, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)
+{
+    return vr_trace(bvh, 0, d_rays, n, d_hits, stream);
+}
+
+int vr_trace_any(vr_bvh bvh, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)
+{
+    return vr_trace(bvh, 1, d_rays, n, d_hits, stream);
+}
+
+int vr_render_pt_forward(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                         const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                         uint64_t* d_ray_counts, void* stream)
+{
+    return vr_render_pt(bvh, d_materials, num_materials, kp, cam, sp, d_accum, d_color, d_ray_counts, stream);
+}
+
+int vr_autotune(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,
+                const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color, int reps,
+                vr_launch_shape* best, float* best_ms, void* stream)
+{
+    return vr_autotune_pt(bvh, d_materials, num_materials, kp, cam, sp, d_accum, d_color, reps, best, best_ms, stream);
+}
+
+int vr_destroy_bvh(vr_bvh bvh) { return vr_bvh_destroy(bvh); }
+
+int vr_destroy_batch(vr_batch batch) { return vr_batch_destroy(batch); }
+
+const char* vr_last_error_string(void) { return vr_last_error(); }
+
 }  // extern "C"

@@ -408,7 +408,7 @@ constexpr int CHUNK = 1024;
 constexpr int TOP_BLOCK = 1024;
 // subtrees of at most this many triangles become one traversal leaf
 #ifndef VR_LBVH_LEAF_MAX
-#define VR_LBVH_LEAF_MAX 4
+#define VR_LBVH_LEAF_MAX 2
 #endif
 constexpr uint32_t LBVH_LEAF_MAX = VR_LBVH_LEAF_MAX;
 constexpr uint32_t SUPER = 64u * CHUNK;

@@ -7,7 +7,7 @@
 // detail/sched_common.h:427-526, so a pixel cannot be split across lanes
 // without changing its random stream).  Each lane holds one ray query in
 // flight.  The wave advances its queries an iteration at a time (a node
-// visit or a triangle test, whichever more lanes need) until 24 lanes have a
+// visit or a triangle test, whichever more lanes need) until 32 lanes have a
 // finished query, then shades those lanes together: each starts its next
 // bounce, its next sample, or (pixel done) takes the next pixel, with one
 // atomic per wave.  Neither a long path nor a long query keeps the other
@@ -380,7 +380,7 @@ __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 
 // lanes that must wait for shading before the wave stops advancing its queries
 #ifndef VR_PT_REGEN_LANES
-#define VR_PT_REGEN_LANES 24
+#define VR_PT_REGEN_LANES 32
 #endif
 
 template <int MODE, bool BATCH, int BLK, int LDSD>
