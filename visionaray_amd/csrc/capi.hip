@@ -503,7 +503,7 @@ int vr_bvh_destroy(vr_bvh bvh)
 {
     if (!bvh) return VR_OK;
     bvh_state& s = bvh->s;
-    void* ptrs[] = {s.pairs, s.quads, s.collapse, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents,
+    void* ptrs[] = {s.pairs, s.quads, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents,
                     s.flags, s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -709,7 +709,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const size_t npairs = n > 1 ? n - 1 : 0;
     // (re)allocate owned arrays when the size changes
     if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
-        void* owned[] = {bs.pairs, bs.quads, bs.collapse, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
+        void* owned[] = {bs.pairs, bs.quads, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
                          bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
         for (void* q : owned)
             if (q) (void)hipFree(q);
@@ -784,12 +784,10 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
     bs.max_leaf = 1;
 
-    // 6: tree depth (one 4-byte read back: it bounds the collapse levels and
-    // the traversal stack), then the 4-wide traversal nodes
-    uint32_t depth = 1;
-    VR_HIP_CHECK(hipMemcpyAsync(&depth, bs.root_box + 6, 4, hipMemcpyDeviceToHost, st));
-    VR_HIP_CHECK(hipStreamSynchronize(st));
-    bs.depth = depth;
+    // 6: the 4-wide traversal nodes.  The exact depth stays on the device
+    // (vr_bvh_get_info reads it); 65 bounds every Karras tree, which the
+    // traversal stack covers, so the build never waits on the device.
+    bs.depth = n <= 1 ? (uint32_t)n : 65u;
     return collapse_quads(bs, LBVH_LEAF_MAX, st);
 }
 

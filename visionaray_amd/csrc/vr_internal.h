@@ -110,8 +110,8 @@ struct bvh_state {
     uint32_t max_leaf = 1;
     bool is_lbvh = false;
     pair_node* pairs = nullptr;      // num_pairs
-    quad_node* quads = nullptr;      // traversal nodes, at most max(num_pairs, 1)
-    uint32_t* collapse = nullptr;    // collapse queues and counters
+    quad_node* quads = nullptr;      // traversal nodes, one per pair (only those below quad 0 are reached)
+    uint32_t collapse_leaf_max = 0;  // leaf size the quads were collapsed with
     tri_node* tris = nullptr;        // num_prims, traversal order
     vr_triangle* prims = nullptr;    // num_prims, caller order (tree.primitives())
     uint32_t* sorted_codes = nullptr;
@@ -230,7 +230,10 @@ struct trace_counters {
     uint32_t tris;
 };
 
-constexpr int MAX_STACK = 192;
+// a Karras tree has at most 65 levels (the common prefix of a node's range,
+// 30 code bits + 32 index bits, grows strictly from parent to child), so its
+// collapse at most 64 quad levels: 3 * 64 + 1 entries always suffice
+constexpr int MAX_STACK = 196;
 
 // Slab test of one child box; returns tnear, or +inf when the child is not
 // entered.  The slab distances round exactly as the reference's; the min/max
@@ -506,12 +509,11 @@ inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b);
 inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32_t)MAX_STACK; }
 
 // Collapse the binary pair records of `s` into its 4-wide traversal nodes
-// (s.quads, s.num_quads, s.root).  leaf_max > 0 (LBVH only: pair e[3] holds
-// the range end) turns inner subtrees of at most leaf_max triangles into one
-// leaf over their contiguous range.  Needs s.depth.
+// (s.quads, s.root): quad p is the collapse rooted at pair p.  leaf_max > 0
+// (LBVH only: pair e[3] holds the range end) turns inner subtrees of at most
+// leaf_max triangles into one leaf over their contiguous range.
 int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
-// After the pair boxes of `s` were refitted: copy every traversal child box
-// again from the pair record that holds it (the collapse is kept).
+// After the pair boxes of `s` were refitted: the quads again from the new boxes.
 int refit_quads(bvh_state& s, hipStream_t st);
 
 }  // namespace vr
