@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--builds", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-spp", type=int, default=1)
+    ap.add_argument("--cpu-spp", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--out", type=str, default="")
     ap.add_argument("--config", type=str, default="c1", choices=["c1", "c2", "c3", "c4"],
@@ -170,11 +170,12 @@ def main():
     # ---- a batch of incoherent closest-hit queries (vr_trace) ----
     closest = trace_batch_bench(args, V, tris, bvh, cam, stream) if args.trace_spp > 0 else None
 
-    traffic = None
+    traffic, counters = None, None
     tfile = ROOT / "profiles" / "pathtrace_c1_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+            prof = json.loads(tfile.read_text())
+            traffic, counters = prof.get("bytes_per_launch"), prof.get("counters")
         except (OSError, ValueError):
             traffic = None
 
@@ -227,6 +228,11 @@ def main():
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": int(alg_bytes),
             "per_ray": {"nodes": round(st[2] / rays_frame, 3), "triangles": round(st[3] / rays_frame, 3)},
+            "hbm_traffic_gbs": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "counters": counters,
+            "note": "algorithmic bytes count every node/triangle fetch as HBM traffic; L2/MALL serve most, so "
+                    "achieved can exceed the HBM peak. Measured HBM bytes are `traffic`; the kernel is bound by "
+                    "VALU issue of divergent wave64 code (counters.valu_busy_pct)",
         },
         "cpu_baseline": cpu,
     }

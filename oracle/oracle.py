@@ -452,9 +452,11 @@ def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, bina
         args = ["bench", t.put("tris.bin", tris), tris.size, t.put("mats.bin", mats), mats.size,
                 t.put("cam.bin", cam_in), t.put("params.bin", params), packet, build, frames, out]
         b = str(binary or ref_binary())
-        res = subprocess.run([b, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+        # the driver's per-frame progress goes straight to this process's stderr
+        res = subprocess.run([b, *map(str, args)], stdout=subprocess.PIPE, stderr=None, text=True,
+                             timeout=timeout)
         if res.returncode != 0:
-            raise RuntimeError(f"vr_ref bench failed: {res.stderr.strip()}")
+            raise RuntimeError(f"vr_ref bench failed with status {res.returncode}")
         with open(out) as f:
             return json.load(f)
     finally:

@@ -475,6 +475,8 @@ int main(int argc, char** argv)
             tree = b.build(index_bvh<tri_t>{}, tris.data(), np, 4);
         }
         double build_s = now_s() - t0;
+        std::fprintf(stderr, "[cpu-ref] %s build of %zu triangles: %.3f s\n", build.c_str(), np, build_s);
+        std::fflush(stderr);
         accum_rt rt(p.w, p.h);
         std::vector<double> ft;
         for (int f = 0; f < frames + 1; ++f) {
@@ -490,6 +492,9 @@ int main(int argc, char** argv)
             }
             double s1 = now_s();
             if (f > 0) ft.push_back(s1 - s0);
+            std::fprintf(stderr, "[cpu-ref] frame %d of %d (%s): %.3f s\n", f, frames, f ? "timed" : "warm-up",
+                         s1 - s0);
+            std::fflush(stderr);
         }
         std::sort(ft.begin(), ft.end());
         double med = ft.empty() ? 0.0 : ft[ft.size() / 2];
