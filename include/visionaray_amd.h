@@ -369,6 +369,77 @@ int vr_destroy_bvh(vr_bvh bvh);
 int vr_destroy_batch(vr_batch batch);
 const char* vr_last_error_string(void);
 
+/* ---- scene ingestion -----------------------------------------------------
+ * model::load (src/common/model.cpp:109-200) for Wavefront OBJ files with
+ * their MTL libraries (load_obj, src/common/obj_loader.cpp:306-520) and PLY
+ * files (load_ply, src/common/ply_loader.cpp:64-211).  A mesh keeps the
+ * triangles indexed (vertex positions, three indices and a material id per
+ * triangle); vr_mesh_triangles expands them on the host, vr_mesh_upload on
+ * the device.  Triangles, ids, per-corner normals and texture coordinates,
+ * materials and the bounding box are the reference model's. */
+
+/* sg::obj_material (src/common/sg/material.h:41-72) as filled by
+ * add_material (src/common/obj_loader.cpp:248-263). */
+typedef struct vr_obj_material {
+    float ca[3];
+    float cd[3];
+    float cs[3];
+    float ce[3];
+    float ior[3];
+    float transmission;
+    float specular_exp;
+    int32_t illum;
+} vr_obj_material;
+
+typedef struct vr_mesh_info {
+    uint64_t num_prims;           /* model::primitives */
+    uint64_t num_vertices;        /* positions read (every OBJ `v`, every PLY vertex) */
+    uint64_t num_shading_normals; /* model::shading_normals (3 per triangle that has them) */
+    uint64_t num_tex_coords;      /* model::tex_coords */
+    uint64_t num_materials;       /* model::materials */
+    uint64_t num_degenerate;      /* OBJ triangles of zero area, rejected (obj_loader.cpp:51-74) */
+    uint64_t num_missing;         /* mtllib files and usemtl names that were not found */
+    float bbox_lo[3];             /* model::bbox */
+    float bbox_hi[3];
+} vr_mesh_info;
+
+typedef struct vr_mesh_t* vr_mesh;
+
+/* Load num_paths files into one model, in order (model::load).  The format
+ * follows each file's extension (.obj/.OBJ, .ply/.PLY); other formats are
+ * VR_ERR_UNSUPPORTED.  An OBJ file is parsed by all host threads
+ * (VR_IO_THREADS caps them). */
+int vr_mesh_load(const char* const* paths, size_t num_paths, vr_mesh* out);
+int vr_mesh_get_info(vr_mesh mesh, vr_mesh_info* info);
+/* model::primitives: num_prims 44-B triangles into host memory. */
+int vr_mesh_triangles(vr_mesh mesh, vr_triangle* h_prims);
+/* model::geometric_normals, 3 floats per triangle: normalize(cross(e1, e2))
+ * for OBJ triangles (obj_loader.cpp:506-511), cross(e1, e2) for PLY ones
+ * (ply_loader.cpp:169). */
+int vr_mesh_geometric_normals(vr_mesh mesh, float* h_normals);
+/* model::shading_normals (3 floats each) and model::tex_coords (2 each). */
+int vr_mesh_shading_normals(vr_mesh mesh, float* h_normals);
+int vr_mesh_tex_coords(vr_mesh mesh, float* h_tex_coords);
+/* model::materials (num_materials records) and the name of material i
+ * (the MTL newmtl name, empty for defaults) into buf[cap]. */
+int vr_mesh_materials(vr_mesh mesh, vr_obj_material* h_materials);
+int vr_mesh_material_name(vr_mesh mesh, size_t i, char* buf, size_t cap);
+/* Path-tracing materials from model::materials by map_material
+ * (src/viewer/viewer.cpp:745-795): emissive when length(ce) > 0 (ce,
+ * ls = 1), matte when illum == 1 (cd, kd = 1).  The other kinds it makes
+ * (mirror, glass, plastic) are VR_ERR_UNSUPPORTED, unless diffuse_fallback is
+ * set: then they become matte with cd. */
+int vr_mesh_pt_materials(vr_mesh mesh, int diffuse_fallback, vr_material* h_materials);
+/* The triangles (and, when d_normals is not NULL, the geometric normals)
+ * built on the device from the indexed mesh: vertex positions, indices and
+ * material ids are copied over (12 B per vertex + 16 B per triangle rather
+ * than 44 B per triangle) and expanded by one kernel into d_prims[num_prims]
+ * (and d_normals[3 num_prims]), identical to vr_mesh_triangles and
+ * vr_mesh_geometric_normals.  The mesh keeps its device copy until
+ * vr_mesh_destroy. */
+int vr_mesh_upload(vr_mesh mesh, vr_triangle* d_prims, float* d_normals, void* stream);
+int vr_mesh_destroy(vr_mesh mesh);
+
 #ifdef __cplusplus
 }
 #endif

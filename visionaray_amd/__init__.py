@@ -30,7 +30,7 @@ __all__ = [
     "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build",
     "closest_hit", "any_hit", "trace", "pinhole_camera", "make_kernel_params", "make_sched_params",
     "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
-    "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "synchronize",
+    "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "synchronize", "model",
 ]
 
 _initialized = False
@@ -529,3 +529,6 @@ class hip_sched:
 
     def reset_counters(self):
         check(lib().vr_memset(self.ray_counts.ptr, 0, self.ray_counts.nbytes))
+
+
+from .model import model  # noqa: E402  (scene ingestion, src/common/model.h)

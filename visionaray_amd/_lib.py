@@ -85,6 +85,17 @@ SIGNATURES = {
     "vr_destroy_bvh": (I, [P]),
     "vr_destroy_batch": (I, [P]),
     "vr_last_error_string": (C.c_char_p, []),
+    "vr_mesh_load": (I, [C.POINTER(C.c_char_p), SZ, C.POINTER(P)]),
+    "vr_mesh_get_info": (I, [P, P]),
+    "vr_mesh_triangles": (I, [P, P]),
+    "vr_mesh_geometric_normals": (I, [P, P]),
+    "vr_mesh_shading_normals": (I, [P, P]),
+    "vr_mesh_tex_coords": (I, [P, P]),
+    "vr_mesh_materials": (I, [P, P]),
+    "vr_mesh_material_name": (I, [P, SZ, C.c_char_p, SZ]),
+    "vr_mesh_pt_materials": (I, [P, I, P]),
+    "vr_mesh_upload": (I, [P, P, P, P]),
+    "vr_mesh_destroy": (I, [P]),
 }
 
 # record layouts shared with the C header
@@ -104,6 +115,13 @@ SCENE_DTYPE = np.dtype([
 ])
 BVH_INFO_DTYPE = np.dtype([("num_prims", "<u8"), ("num_nodes", "<u8"), ("depth", "<u4"),
                            ("max_leaf_size", "<u4")])
+OBJ_MATERIAL_DTYPE = np.dtype([("ca", "<f4", (3,)), ("cd", "<f4", (3,)), ("cs", "<f4", (3,)), ("ce", "<f4", (3,)),
+                               ("ior", "<f4", (3,)), ("transmission", "<f4"), ("specular_exp", "<f4"),
+                               ("illum", "<i4")])
+MESH_INFO_DTYPE = np.dtype([("num_prims", "<u8"), ("num_vertices", "<u8"), ("num_shading_normals", "<u8"),
+                            ("num_tex_coords", "<u8"), ("num_materials", "<u8"), ("num_degenerate", "<u8"),
+                            ("num_missing", "<u8"), ("bbox_lo", "<f4", (3,)), ("bbox_hi", "<f4", (3,))])
+assert OBJ_MATERIAL_DTYPE.itemsize == 72 and MESH_INFO_DTYPE.itemsize == 80
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE_DTYPE.itemsize == 32
 assert CAMERA_DTYPE.itemsize == 48 and KERNEL_PARAMS_DTYPE.itemsize == 32 and SCHED_PARAMS_DTYPE.itemsize == 24
 assert SCENE_DTYPE.itemsize == 152

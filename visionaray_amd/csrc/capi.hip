@@ -110,13 +110,7 @@ struct vr_batch_t {
     hipGraphExec_t exec = nullptr;
 };
 
-#define VR_TRY_BEGIN try {
-#define VR_TRY_END                                                   \
-    }                                                                \
-    catch (const std::bad_alloc&) { return fail(VR_ERR_OUT_OF_MEMORY, "host allocation failed"); } \
-    catch (const std::exception& e) { return fail(VR_ERR_INVALID_ARGUMENT, e.what()); }
-
-static int ensure_device()
+int vr::ensure_device()
 {
     device_state* ds = dev_state();
     VR_CHECK(ds != nullptr, VR_ERR_HIP, "no current HIP device");

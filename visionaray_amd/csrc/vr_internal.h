@@ -35,6 +35,16 @@ int fail(int code, const std::string& msg);
         if (!(cond)) return ::vr::fail((code), (msg));                                     \
     } while (0)
 
+// C entry points: host exceptions become status codes
+#define VR_TRY_BEGIN try {
+#define VR_TRY_END                                                                                      \
+    }                                                                                                   \
+    catch (const std::bad_alloc&) { return ::vr::fail(VR_ERR_OUT_OF_MEMORY, "host allocation failed"); } \
+    catch (const std::exception& e) { return ::vr::fail(VR_ERR_INVALID_ARGUMENT, e.what()); }
+
+// create the current device's library state on first use
+int ensure_device();
+
 // ---------------------------------------------------------------------------
 // device layouts
 // ---------------------------------------------------------------------------
