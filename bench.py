@@ -55,9 +55,12 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--out", type=str, default="")
-    ap.add_argument("--config", type=str, default="c1", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--config", type=str, default="c1", choices=["c1", "c2", "c3", "c4", "ingest"],
                     help="c1 (default): the headline workload; c2 batched scenes; c3 adjoint vs finite "
-                         "differences; c4 dynamic geometry (refit vs rebuild) with batched adjoint")
+                         "differences; c4 dynamic geometry (refit vs rebuild) with batched adjoint; ingest: "
+                         "OBJ / PLY scene loading and device triangle assembly")
+    ap.add_argument("--grid", type=int, default=1000,
+                    help="ingest: the mesh is a grid of grid x grid quads (2 grid^2 triangles)")
     ap.add_argument("--scenes", type=int, default=0, help="c2/c4: number of scenes (default 64 / 32)")
     ap.add_argument("--fd-params", type=int, default=0, help="c3: finite-difference parameters (0 = all)")
     ap.add_argument("--profile-pass", action="store_true",
@@ -79,6 +82,12 @@ def main():
     import visionaray_amd as V
     from visionaray_amd import scenes as S
 
+    if args.config == "ingest":
+        line = bench_ingest(args, V)
+        print(json.dumps(line), flush=True)
+        if args.out:
+            Path(args.out).write_text(json.dumps(line) + "\n")
+        return
     if args.config != "c1":
         from visionaray_amd import workloads
         line = {"c2": workloads.bench_batched, "c3": workloads.bench_adjoint, "c4": workloads.bench_dynamic}[
@@ -468,6 +477,176 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
             "binary": O.ref_binary().name,
         }
     return {"value": None, "error": "oracle/_ref not built"}
+
+
+# ---------------------------------------------------------------------------
+# ingest: OBJ / PLY loading (host threads) and triangle assembly (k_assemble)
+# ---------------------------------------------------------------------------
+
+def grid_quads(n: int, seed: int = 3):
+    """An n x n-quad height field: (n+1)^2 vertices, n^2 quads (a, b, c, d)."""
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(n + 1, dtype=np.float32), np.arange(n + 1, dtype=np.float32), indexing="ij")
+    zs = (rng.normal(size=xs.shape) * 0.25).astype(np.float32)
+    pos = np.stack([xs, ys, zs], -1).reshape(-1, 3) * np.float32(1.0 / n)
+    i = (np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]).reshape(-1)
+    return pos, np.stack([i, i + n + 1, i + n + 2, i + 1], -1)
+
+
+def write_obj(path: Path, pos, quads):
+    """v / vt / vn per vertex and one 4-corner face v/t/n per quad (1-based)."""
+    with open(path, "w") as f:
+        f.write("# grid\n")
+        f.write("".join(f"v {x:.7g} {y:.7g} {z:.7g}\n" for x, y, z in pos.tolist()))
+        f.write("".join(f"vt {x:.7g} {y:.7g}\n" for x, y in pos[:, :2].tolist()))
+        f.write("vn 0 0 1\n" * len(pos))
+        f.write("".join(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c} {d}/{d}/{d}\n"
+                        for a, b, c, d in (quads + 1).tolist()))
+
+
+def write_ply(path: Path, pos, quads):
+    tris = np.concatenate([quads[:, [0, 1, 2]], quads[:, [0, 2, 3]]])
+    head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {len(pos)}\nproperty float x\n"
+            f"property float y\nproperty float z\nelement face {len(tris)}\n"
+            "property list uchar int vertex_indices\nend_header\n")
+    rec = np.zeros(len(tris), np.dtype([("n", "u1"), ("i", "<i4", (3,))]))
+    rec["n"] = 3
+    rec["i"] = tris
+    with open(path, "wb") as f:
+        f.write(head.encode())
+        f.write(np.ascontiguousarray(pos, "<f4").tobytes())
+        f.write(rec.tobytes())
+
+
+def bench_ingest(args, V):
+    """Scene files to device triangles: vr_mesh_load of an OBJ (all host
+    threads and one) and of a binary PLY, then vr_mesh_upload (PCIe copy of
+    the indexed mesh + k_assemble) and LBVH build.  k_assemble's roofline:
+    16 B read and 56 B written per triangle plus 12 B per vertex."""
+    import tempfile
+    from visionaray_amd.model import model
+    n = args.grid
+    pos, quads = grid_quads(n)
+    tmp = Path(tempfile.mkdtemp(prefix="vr_ingest_"))
+    t0 = time.time()
+    obj, ply = tmp / "grid.obj", tmp / "grid.ply"
+    write_obj(obj, pos, quads)
+    write_ply(ply, pos, quads)
+    obj_mb, ply_mb = obj.stat().st_size / 1e6, ply.stat().st_size / 1e6
+    log(f"ingest: {2 * n * n} triangles; OBJ {obj_mb:.0f} MB, PLY {ply_mb:.0f} MB written in {time.time() - t0:.1f}s")
+    threads = min(16, os.cpu_count() or 1)
+
+    def load(path, nthreads, reps):
+        os.environ["VR_IO_THREADS"] = str(nthreads)
+        ts, m = [], None
+        for _ in range(reps):
+            m = model()
+            t = time.perf_counter()
+            m.load(str(path))
+            ts.append(time.perf_counter() - t)
+        return m, float(np.median(ts))
+
+    m_obj, obj_s = load(obj, threads, 3)
+    log(f"ingest: OBJ {obj_s * 1e3:.0f} ms with {threads} threads")
+    _, obj1_s = load(obj, 1, 1)
+    log(f"ingest: OBJ {obj1_s * 1e3:.0f} ms with 1 thread")
+    m_ply, ply_s = load(ply, threads, 3)
+    log(f"ingest: PLY {ply_s * 1e3:.0f} ms")
+    os.environ.pop("VR_IO_THREADS", None)
+    ntri = int(m_obj.info["num_prims"])
+    nv = int(m_obj.info["num_vertices"])
+    assert ntri == 2 * n * n
+
+    V.init(0)
+    st = V.Stream()
+    tris = V.DeviceArray(ntri, m_obj.primitives.dtype)
+    nrm = V.DeviceArray(3 * ntri, np.float32)
+    lib = V.lib()
+    from visionaray_amd._lib import check
+
+    def upload():
+        check(lib.vr_mesh_upload(m_obj._h, tris.ptr, nrm.ptr, st.handle))
+
+    e0, e1 = V.Event(), V.Event()
+    e0.record(st)
+    upload()  # first call: indexed mesh over PCIe, then the kernel
+    e1.record(st)
+    st.synchronize()
+    first_ms = e0.elapsed_ms(e1)
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        upload()
+    e1.record(st)
+    st.synchronize()
+    kern_ms = e0.elapsed_ms(e1) / reps
+    host_tris = m_obj.primitives
+    ok = tris.to_host().tobytes() == host_tris.tobytes()
+    # the same triangles copied as 44-byte records (what a host-side loader hands over)
+    t = time.perf_counter()
+    d_plain = V.DeviceArray.from_host(host_tris)
+    plain_ms = (time.perf_counter() - t) * 1e3
+    del d_plain
+    # file to BVH
+    t = time.perf_counter()
+    m2 = model()
+    os.environ["VR_IO_THREADS"] = str(threads)
+    m2.load(str(obj))
+    os.environ.pop("VR_IO_THREADS", None)
+    d2, _ = m2.upload(stream=st, normals=False)
+    bvh = V.lbvh_builder().build(d2, keep_nodes=False, stream=st)
+    st.synchronize()
+    file_to_bvh_s = time.perf_counter() - t
+    bvh.destroy()
+    alg_bytes = ntri * (16 + 56) + nv * 12
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    log(f"ingest: upload first {first_ms:.2f} ms, k_assemble {kern_ms * 1e3:.1f} us ({achieved:.0f} GB/s), "
+        f"44-B records over PCIe {plain_ms:.1f} ms; file->BVH {file_to_bvh_s * 1e3:.0f} ms; bit-identical {ok}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = ingest_cpu_baseline(tmp, pos, quads)
+    for p in (obj, ply):
+        p.unlink()
+    tmp.rmdir()
+    return {
+        "metric": "OBJ scene ingestion Mtriangles/s",
+        "value": round(ntri / obj_s / 1e6, 3), "unit": "Mtriangles/s", "n_gpus": 1, "steps": 3, "warmup": 0,
+        "ms_per_step": round(obj_s * 1e3, 3), "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n}x{n}-quad height field as OBJ (v/vt/vn, 4-corner v/t/n faces, "
+                               f"{obj_mb:.0f} MB) and binary PLY ({ply_mb:.0f} MB); {ntri} triangles",
+                   "triangles": ntri, "vertices": nv, "host_threads": threads},
+        "obj_ms": round(obj_s * 1e3, 2), "obj_mb_per_s": round(obj_mb / obj_s, 1),
+        "obj_1thread_ms": round(obj1_s * 1e3, 2), "ply_ms": round(ply_s * 1e3, 2),
+        "ply_mb_per_s": round(ply_mb / ply_s, 1),
+        "upload_first_ms": round(first_ms, 3), "upload_44B_records_ms": round(plain_ms, 3),
+        "file_to_bvh_ms": round(file_to_bvh_s * 1e3, 1), "device_equals_host": ok,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_assemble",
+                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": cpu,
+    }
+
+
+def ingest_cpu_baseline(tmp: Path, pos, quads, rows: int = 60):
+    """The oracle's pure-Python restatement of load_obj on a bounded sample:
+    the first `rows` rows of the same grid (OBJ of the same form)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import mesh_oracle as MO
+    n = int(round(np.sqrt(len(quads))))
+    keep = quads[: rows * n]
+    used = np.unique(keep)
+    remap = np.zeros(len(pos), np.int64)
+    remap[used] = np.arange(len(used))
+    p = tmp / "sample.obj"
+    write_obj(p, pos[used], remap[keep])
+    t = time.perf_counter()
+    mod = MO.load([str(p)])
+    s = time.perf_counter() - t
+    p.unlink()
+    return {"value": round(len(mod.prims) / s / 1e6, 5), "unit": "Mtriangles/s", "cores": 1, "kind": "port",
+            "sample": f"first {rows} of {n} quad rows ({len(mod.prims)} triangles), same OBJ form; the "
+                      "reference's Boost.Spirit loader cannot be built here (no Boost)"}
 
 
 if __name__ == "__main__":

@@ -46,6 +46,17 @@
 
 namespace vr {
 
+void* huge_alloc(size_t bytes)
+{
+    constexpr size_t HUGE = (size_t)2 << 20;
+    const size_t sz = (bytes + HUGE - 1) / HUGE * HUGE;
+    void* p = std::aligned_alloc(HUGE, sz);
+    if (!p) throw std::bad_alloc();
+    madvise(p, sz, MADV_HUGEPAGE);
+    return p;
+}
+
+void huge_free(void* p) { std::free(p); }
 
 namespace {
 
@@ -69,13 +80,12 @@ struct mapped_file {
         }
         size = (size_t)sb.st_size;
         if (size > 0) {
-            map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
             if (map == MAP_FAILED) {
                 map = nullptr;
                 ::close(fd);
                 return false;
             }
-            madvise(map, size, MADV_SEQUENTIAL);
             data = static_cast<const char*>(map);
         }
         ::close(fd);
@@ -167,7 +177,16 @@ inline bool starts_with(const char* p, const char* e, const char* kw, size_t n)
 }
 
 // qi::float_: [+-] followed by a real (digits with an optional point and
-// exponent, or nan / inf / infinity), correctly rounded to float
+// exponent, or nan / inf / infinity), correctly rounded to float.  Short
+// decimals take Clinger's fast path (W. D. Clinger, "How to read floating
+// point numbers accurately", PLDI 1990): an integer significand m <= 2^24 and
+// a scale 10^k with |k| <= 10 are both exact floats, so one IEEE multiply or
+// divide is the correctly rounded value.  Everything else (and nan / inf)
+// goes to std::from_chars, which covers the same token.
+constexpr float POW10F[11] = {1e0f, 1e1f, 1e2f, 1e3f, 1e4f, 1e5f, 1e6f, 1e7f, 1e8f, 1e9f, 1e10f};
+
+inline bool is_digit(char c) { return (unsigned)(c - '0') < 10u; }
+
 inline bool parse_real(const char*& p, const char* e, float& out)
 {
     const char* s = p;
@@ -176,6 +195,48 @@ inline bool parse_real(const char*& p, const char* e, float& out)
         neg = *s == '-';
         ++s;
         if (s < e && (*s == '+' || *s == '-')) return false;
+    }
+    const char* q = s;
+    uint64_t m = 0;
+    int scale = 0;
+    bool any = false, wide = false;
+    for (; q < e && is_digit(*q); ++q) {
+        any = true;
+        if (m < 100000000000000000ull) m = m * 10 + (uint64_t)(*q - '0');
+        else wide = true;
+    }
+    if (q < e && *q == '.') {
+        for (++q; q < e && is_digit(*q); ++q) {
+            any = true;
+            if (m < 100000000000000000ull) {
+                m = m * 10 + (uint64_t)(*q - '0');
+                --scale;
+            } else {
+                wide = true;
+            }
+        }
+    }
+    if (any && !wide) {
+        int ex = 0;
+        const char* r = q;
+        if (r < e && (*r == 'e' || *r == 'E')) {
+            ++r;
+            bool eneg = false;
+            if (r < e && (*r == '+' || *r == '-')) eneg = *r++ == '-';
+            if (r < e && is_digit(*r)) {
+                for (; r < e && is_digit(*r); ++r)
+                    if (ex < 100000) ex = ex * 10 + (*r - '0');
+                q = r;
+                ex = eneg ? -ex : ex;
+            }
+        }
+        const int k = scale + ex;
+        if (m <= (1u << 24) && k >= -10 && k <= 10) {
+            const float v = k >= 0 ? (float)m * POW10F[k] : (float)m / POW10F[-k];
+            out = neg ? -v : v;
+            p = q;
+            return true;
+        }
     }
     float v = 0.0f;
     const auto r = std::from_chars(s, e, v, std::chars_format::general);
@@ -358,8 +419,8 @@ struct obj_piece {
     const char* b = nullptr;
     const char* e = nullptr;
     // pass 1
-    std::vector<float> v, vt, vn;
-    std::vector<int32_t> corner;  // 3 per corner: v, t, n (NO_INDEX when absent)
+    pod_vector<float> v, vt, vn;
+    pod_vector<int32_t> corner;  // 3 per corner: v, t, n (NO_INDEX when absent)
     struct face {
         uint64_t corner0;
         uint32_t n;
@@ -377,9 +438,10 @@ struct obj_piece {
     uint64_t vbase = 0, vtbase = 0, vnbase = 0;
     std::vector<std::pair<uint64_t, uint32_t>> runs;  // (first face, material id)
     // pass 3
-    std::vector<uint32_t> idx, geom;
-    std::vector<float> tex, shading;
+    pod_vector<uint32_t> idx, geom;
+    pod_vector<float> tex, shading;
     uint64_t degenerate = 0;
+    f3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};  // over v1, v1 + e1, v1 + e2
     int err = 0;
     std::string msg;
 };
@@ -488,6 +550,9 @@ void obj_pass3(obj_piece& pc, const char* file, const std::string& path, const f
                 ++pc.degenerate;  // rejected: zero area
                 continue;
             }
+            insert(pc.lo, pc.hi, v1);
+            insert(pc.lo, pc.hi, add3(v1, e1));
+            insert(pc.lo, pc.hi, add3(v1, e2));
             pc.idx.push_back((uint32_t)(model_vbase + i1));
             pc.idx.push_back((uint32_t)(model_vbase + i2));
             pc.idx.push_back((uint32_t)(model_vbase + i3));
@@ -519,10 +584,18 @@ struct obj_state {
     std::map<std::string, mtl> lib;
     std::vector<std::string> parsed_libs;
     uint64_t geom_id = 0;
+    uint64_t first_prim = 0;  // triangles of the model before this load_obj call
+    f3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};  // this call's triangles
 };
 
+inline void merge(f3& lo, f3& hi, f3 blo, f3 bhi)
+{
+    lo = f3{blo.x < lo.x ? blo.x : lo.x, blo.y < lo.y ? blo.y : lo.y, blo.z < lo.z ? blo.z : lo.z};
+    hi = f3{hi.x < bhi.x ? bhi.x : hi.x, hi.y < bhi.y ? bhi.y : hi.y, hi.z < bhi.z ? bhi.z : hi.z};
+}
+
 template <typename T>
-void append_parts(std::vector<T>& dst, std::vector<obj_piece>& pcs, std::vector<T> obj_piece::*part, unsigned nt)
+void append_parts(pod_vector<T>& dst, std::vector<obj_piece>& pcs, pod_vector<T> obj_piece::*part, unsigned nt)
 {
     std::vector<size_t> off(pcs.size() + 1, dst.size());
     for (size_t k = 0; k < pcs.size(); ++k) off[k + 1] = off[k] + (pcs[k].*part).size();
@@ -531,7 +604,7 @@ void append_parts(std::vector<T>& dst, std::vector<obj_piece>& pcs, std::vector<
         for (size_t k = t; k < pcs.size(); k += nt) {
             auto& src = pcs[k].*part;
             if (!src.empty()) std::memcpy(dst.data() + off[k], src.data(), src.size() * sizeof(T));
-            std::vector<T>().swap(src);
+            pod_vector<T>().swap(src);
         }
     });
 }
@@ -596,7 +669,7 @@ int load_obj_file(mesh_model& m, obj_state& S, const std::string& path)
     // vertex, texture coordinate and normal arrays of the whole file
     const uint64_t model_vbase = m.pos.size() / 3;
     append_parts(m.pos, pcs, &obj_piece::v, nt);
-    std::vector<float> ftex, fnrm;
+    pod_vector<float> ftex, fnrm;
     append_parts(ftex, pcs, &obj_piece::vt, nt);
     append_parts(fnrm, pcs, &obj_piece::vn, nt);
     const float* fpos = m.pos.data() + 3 * model_vbase;
@@ -608,7 +681,10 @@ int load_obj_file(mesh_model& m, obj_state& S, const std::string& path)
         if (pc.err) return fail(pc.err, pc.msg);
 
     const uint64_t first = m.geom.size();
-    for (auto& pc : pcs) m.degenerate += pc.degenerate;
+    for (auto& pc : pcs) {
+        m.degenerate += pc.degenerate;
+        merge(S.lo, S.hi, pc.lo, pc.hi);
+    }
     append_parts(m.idx, pcs, &obj_piece::idx, nt);
     append_parts(m.geom, pcs, &obj_piece::geom, nt);
     append_parts(m.tex, pcs, &obj_piece::tex, nt);
@@ -626,12 +702,14 @@ int load_obj_file(mesh_model& m, obj_state& S, const std::string& path)
 
 // after the files of one load_obj call (obj_loader.cpp:506-519): texture
 // coordinates padded to three per triangle, bounds over v1, v1 + e1, v1 + e2
-void finish_obj(mesh_model& m)
+// of every triangle of the model (this call's were gathered by its pieces)
+void finish_obj(mesh_model& m, const obj_state& S)
 {
     const size_t nprims = m.geom.size();
     if (m.tex.size() < 6 * nprims) m.tex.resize(6 * nprims, 0.0f);
+    merge(m.lo, m.hi, S.lo, S.hi);
     const float* pos = m.pos.data();
-    for (size_t i = 0; i < nprims; ++i) {
+    for (size_t i = 0; i < S.first_prim; ++i) {
         const f3 v1 = vertex(pos, m.idx[3 * i]);
         const f3 e1 = sub3(vertex(pos, m.idx[3 * i + 1]), v1);
         const f3 e2 = sub3(vertex(pos, m.idx[3 * i + 2]), v1);
@@ -833,7 +911,7 @@ int load_ply_file(mesh_model& m, const std::string& path)
     VR_CHECK(m.pos.size() / 3 + ve->count <= 0xFFFFFFFFull, VR_ERR_UNSUPPORTED, "ply: more than 2^32 vertices");
 
     const uint64_t vbase = m.pos.size() / 3;
-    std::vector<float> nrm, uv;
+    pod_vector<float> nrm, uv;
     std::vector<int64_t> fidx;
     ply_reader rd{p, e, format};
     for (const auto& el : elems) {
@@ -944,32 +1022,35 @@ int vr_mesh_load(const char* const* paths, size_t num_paths, vr_mesh* out)
     // model::load (model.cpp:144-200): files of one format go to one loader
     // call (load_obj keeps its material library and material id across its
     // files); a list of mixed formats loads file by file
+    auto type_of = [](const std::string& ext) {
+        return ext == ".obj" || ext == ".OBJ" ? 1 : ext == ".ply" || ext == ".PLY" ? 2 : 0;
+    };
     bool same = true;
     for (size_t k = 0; k < num_paths; ++k) {
         VR_CHECK(paths[k], VR_ERR_INVALID_ARGUMENT, "vr_mesh_load: null path");
-        same = same && extension(paths[k]) == extension(paths[0]);
+        same = same && type_of(extension(paths[k])) == type_of(extension(paths[0]));
     }
     obj_state S;
     bool in_obj = false;
     for (size_t k = 0; k < num_paths; ++k) {
         const std::string path(paths[k]);
         const std::string ext = extension(path);
-        const bool obj = ext == ".obj" || ext == ".OBJ";
-        const bool ply = ext == ".ply" || ext == ".PLY";
+        const bool obj = type_of(ext) == 1;
+        const bool ply = type_of(ext) == 2;
         VR_CHECK(obj || ply, VR_ERR_UNSUPPORTED, "vr_mesh_load: unsupported model format '" + ext + "' (" + path + ")");
-        if (obj && (!in_obj || !same)) {
-            if (in_obj) finish_obj(m);
+        if (obj && !in_obj) {
             S = obj_state();
+            S.first_prim = m.geom.size();
             in_obj = true;
         }
         const int rc = obj ? load_obj_file(m, S, path) : load_ply_file(m, path);
         if (rc) return rc;
         if (obj && !same) {
-            finish_obj(m);
+            finish_obj(m, S);
             in_obj = false;
         }
     }
-    if (in_obj) finish_obj(m);
+    if (in_obj) finish_obj(m, S);
     *out = mesh.release();
     return VR_OK;
     VR_TRY_END
