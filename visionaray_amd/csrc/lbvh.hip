@@ -827,7 +827,7 @@ int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hi
     uint32_t* cb = nullptr;
     VR_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&cb), 64, st));
     int rc = launch_morton(d_prims, n, cb, d_codes, st);
-    hipFreeAsync(cb, st);
+    (void)hipFreeAsync(cb, st);
     return rc;
 }
 

@@ -6,10 +6,11 @@
 // ply_loader.cpp:169).
 //
 // A block owns 256 consecutive triangles.  Their indices are read with
-// coalesced dword loads into LDS, each lane gathers its three vertices and
+// coalesced 16-byte loads into LDS, each lane gathers its three vertices and
 // assembles its record in LDS (an 11-dword stride, odd, so no bank
-// conflicts), and the block then writes its 256 x 11 dwords and 256 x 3
-// normal dwords as contiguous coalesced stores.  Bound by HBM: 16 B read and
+// conflicts), and the block then writes its 256 x 44 bytes of records and
+// 256 x 12 bytes of normals as contiguous 16-byte stores (dwords for a
+// partial last block or unaligned arrays).  Bound by HBM: 16 B read and
 // 56 B written per triangle, plus the vertex gathers (12 B per vertex once,
 // the rest from L2).
 #include "mesh.h"
@@ -27,16 +28,27 @@ static_assert(sizeof(vr_triangle) == TRI_DW * 4, "triangle record is 11 dwords")
 
 __global__ __launch_bounds__(BLOCK) void k_assemble(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
                                                     const uint32_t* __restrict__ geom, uint32_t first, uint32_t n,
-                                                    int normalized, uint32_t* __restrict__ out,
+                                                    int normalized, int vec, uint32_t* __restrict__ out,
                                                     float* __restrict__ normals)
 {
-    __shared__ uint32_t s_idx[3 * BLOCK];
-    __shared__ uint32_t s_tri[TRI_DW * BLOCK];
-    __shared__ float s_n[3 * BLOCK];
+    __shared__ uint4 s_idx4[3 * BLOCK / 4];
+    __shared__ uint4 s_tri4[TRI_DW * BLOCK / 4];
+    __shared__ uint4 s_n4[3 * BLOCK / 4];
+    uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_idx4);
+    uint32_t* s_tri = reinterpret_cast<uint32_t*>(s_tri4);
+    float* s_n = reinterpret_cast<float*>(s_n4);
     const uint32_t base = first + blockIdx.x * BLOCK;  // first triangle of the block
     const uint32_t cnt = min((uint32_t)BLOCK, first + n - base);
     const uint32_t t = threadIdx.x;
-    for (uint32_t k = t; k < 3 * cnt; k += BLOCK) s_idx[k] = idx[3 * (size_t)base + k];
+    // a full block with 16-byte aligned rows moves 16-byte vectors; the last
+    // (partial) block, or unaligned arrays, move dwords
+    const bool v4 = vec && cnt == BLOCK;
+    if (v4) {
+        const uint4* src = reinterpret_cast<const uint4*>(idx + 3 * (size_t)base);
+        for (uint32_t k = t; k < 3 * BLOCK / 4; k += BLOCK) s_idx4[k] = src[k];
+    } else {
+        for (uint32_t k = t; k < 3 * cnt; k += BLOCK) s_idx[k] = idx[3 * (size_t)base + k];
+    }
     __syncthreads();
     if (t < cnt) {
         const uint32_t a = s_idx[3 * t], b = s_idx[3 * t + 1], c = s_idx[3 * t + 2];
@@ -66,11 +78,20 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(const float* __restrict__ po
         }
     }
     __syncthreads();
-    uint32_t* o = out + (size_t)TRI_DW * base;
-    for (uint32_t k = t; k < TRI_DW * cnt; k += BLOCK) o[k] = s_tri[k];
-    if (normals) {
-        float* on = normals + 3 * (size_t)base;
-        for (uint32_t k = t; k < 3 * cnt; k += BLOCK) on[k] = s_n[k];
+    if (v4) {
+        uint4* o = reinterpret_cast<uint4*>(out + (size_t)TRI_DW * base);
+        for (uint32_t k = t; k < TRI_DW * BLOCK / 4; k += BLOCK) o[k] = s_tri4[k];
+        if (normals) {
+            uint4* on = reinterpret_cast<uint4*>(normals + 3 * (size_t)base);
+            for (uint32_t k = t; k < 3 * BLOCK / 4; k += BLOCK) on[k] = s_n4[k];
+        }
+    } else {
+        uint32_t* o = out + (size_t)TRI_DW * base;
+        for (uint32_t k = t; k < TRI_DW * cnt; k += BLOCK) o[k] = s_tri[k];
+        if (normals) {
+            float* on = normals + 3 * (size_t)base;
+            for (uint32_t k = t; k < 3 * cnt; k += BLOCK) on[k] = s_n[k];
+        }
     }
 }
 
@@ -95,8 +116,11 @@ int assemble_triangles(mesh_model& m, vr_triangle* d_prims, float* d_normals, hi
     VR_CHECK(m.device == dev, VR_ERR_INVALID_ARGUMENT, "vr_mesh_upload: the mesh was uploaded to another device");
     for (const mesh_span& s : m.spans) {
         const unsigned blocks = (unsigned)div_up(s.count, BLOCK);
+        // 16-byte rows: the span starts at a multiple of 4 triangles (44 and
+        // 12 bytes each) and the output arrays are 16-byte aligned
+        const bool vec = s.first % 4 == 0 && ((uintptr_t)d_prims % 16) == 0 && ((uintptr_t)d_normals % 16) == 0;
         hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(BLOCK), 0, st, m.d_pos, m.d_idx, m.d_geom,
-                           (uint32_t)s.first, (uint32_t)s.count, s.normalized ? 1 : 0,
+                           (uint32_t)s.first, (uint32_t)s.count, s.normalized ? 1 : 0, vec ? 1 : 0,
                            reinterpret_cast<uint32_t*>(d_prims), d_normals);
         VR_HIP_CHECK(hipGetLastError());
     }

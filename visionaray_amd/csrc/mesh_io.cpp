@@ -912,17 +912,76 @@ int load_ply_file(mesh_model& m, const std::string& path)
 
     const uint64_t vbase = m.pos.size() / 3;
     pod_vector<float> nrm, uv;
-    std::vector<int64_t> fidx;
+    pod_vector<int64_t> fidx;
     ply_reader rd{p, e, format};
+    const unsigned nt = io_threads();
     for (const auto& el : elems) {
         const bool is_v = &el == ve, is_f = &el == fe;
         if (is_v) {
-            m.pos.reserve(m.pos.size() + 3 * el.count);
-            if (has_n) nrm.reserve(3 * el.count);
-            if (has_uv) uv.reserve(2 * el.count);
+            m.pos.resize(3 * (vbase + el.count));
+            if (has_n) nrm.resize(3 * el.count);
+            if (has_uv) uv.resize(2 * el.count);
         }
-        if (is_f) fidx.reserve(3 * el.count);
-        float rec[8];
+        if (is_f) fidx.resize(3 * el.count);
+        // where each value of vertex r goes: slot 0-2 position, 3-5 normal, 6-7 u v
+        auto store = [&](uint64_t r, int slot, float v) {
+            if (slot < 3) m.pos[3 * (vbase + r) + slot] = v;
+            else if (slot < 6) nrm[3 * r + slot - 3] = v;
+            else uv[2 * r + slot - 6] = v;
+        };
+        std::vector<int> slot(el.props.size(), -1);
+        if (is_v) {
+            const int ks[8] = {px, py, pz, pnx, pny, pnz, pu, pv};
+            for (int j = 0; j < 8; ++j)
+                if (ks[j] >= 0 && (j < 3 || (j < 6 ? has_n : has_uv))) slot[ks[j]] = j;
+        }
+        // binary records of fixed size (every list but the face's index
+        // list, read as 3 indices, makes the size variable) are decoded by
+        // all threads
+        size_t rec = 0;
+        bool fixed = format != 0;
+        std::vector<size_t> off(el.props.size());
+        for (size_t k = 0; k < el.props.size(); ++k) {
+            const auto& pr = el.props[k];
+            off[k] = rec;
+            if (!pr.list) rec += ply_size(pr.type);
+            else if (is_f && (int)k == fi) rec += ply_size(pr.count_type) + 3 * ply_size(pr.type);
+            else fixed = false;
+        }
+        if (fixed && rec > 0) {
+            VR_CHECK(el.count <= (uint64_t)(e - rd.p) / rec, VR_ERR_INVALID_ARGUMENT, "ply: truncated data in " + path);
+            const char* base = rd.p;
+            std::vector<uint64_t> bad(nt, UINT64_MAX);
+            const uint64_t per = (el.count + nt - 1) / nt;
+            parallel_for(nt, [&](unsigned t) {
+                ply_reader q{nullptr, nullptr, format};
+                for (uint64_t r = t * per; r < std::min(el.count, (t + 1) * per); ++r) {
+                    const char* at = base + r * rec;
+                    for (size_t k = 0; k < el.props.size(); ++k) {
+                        const auto& pr = el.props[k];
+                        if (is_f && (int)k == fi) {
+                            q.p = at + off[k];
+                            q.e = at + rec;
+                            if (q.value(pr.count_type) != 3.0) {
+                                bad[t] = std::min(bad[t], r);
+                                break;
+                            }
+                            for (int j = 0; j < 3; ++j) fidx[3 * r + j] = (int64_t)q.value(pr.type);
+                        } else if (slot[k] >= 0) {
+                            q.p = at + off[k];
+                            q.e = at + rec;
+                            store(r, slot[k], q.real(pr.type));
+                        }
+                    }
+                    if (bad[t] != UINT64_MAX) break;
+                }
+            });
+            const uint64_t b = *std::min_element(bad.begin(), bad.end());
+            VR_CHECK(b == UINT64_MAX, VR_ERR_UNSUPPORTED,
+                     "ply: face " + std::to_string(b) + " is not a triangle (load_ply reads 3 indices per face)");
+            rd.p += el.count * rec;
+            continue;
+        }
         for (uint64_t r = 0; r < el.count && rd.ok; ++r) {
             for (size_t k = 0; k < el.props.size(); ++k) {
                 const auto& pr = el.props[k];
@@ -932,24 +991,15 @@ int load_ply_file(mesh_model& m, const std::string& path)
                     if (is_f && (int)k == fi) {
                         VR_CHECK(cnt == 3.0, VR_ERR_UNSUPPORTED,
                                  "ply: face " + std::to_string(r) + " is not a triangle (load_ply reads 3 indices per face)");
-                        for (int j = 0; j < 3; ++j) fidx.push_back((int64_t)rd.value(pr.type));
+                        for (int j = 0; j < 3; ++j) fidx[3 * r + j] = (int64_t)rd.value(pr.type);
                     } else {
                         for (uint64_t j = 0; j < (uint64_t)cnt; ++j) rd.value(pr.type);
                     }
-                } else if (is_v) {
-                    const int ik = (int)k;
-                    const int slot = ik == px ? 0 : ik == py ? 1 : ik == pz ? 2 : ik == pnx ? 3 : ik == pny ? 4
-                                   : ik == pnz ? 5 : ik == pu ? 6 : ik == pv ? 7 : -1;
-                    if (slot >= 0) rec[slot] = rd.real(pr.type);
-                    else rd.value(pr.type);
+                } else if (slot[k] >= 0) {
+                    store(r, slot[k], rd.real(pr.type));
                 } else {
                     rd.value(pr.type);
                 }
-            }
-            if (is_v) {
-                m.pos.insert(m.pos.end(), rec, rec + 3);
-                if (has_n) nrm.insert(nrm.end(), rec + 3, rec + 6);
-                if (has_uv) uv.insert(uv.end(), rec + 6, rec + 8);
             }
         }
         VR_CHECK(rd.ok, VR_ERR_INVALID_ARGUMENT, "ply: truncated or malformed data in " + path);
@@ -960,29 +1010,39 @@ int load_ply_file(mesh_model& m, const std::string& path)
     const uint64_t first = m.geom.size();
     VR_CHECK(first + nf <= 0x7FFFFFFFull, VR_ERR_UNSUPPORTED, "ply: more than 2^31 triangles");
     const uint32_t gid = (uint32_t)m.mats.size();
-    m.idx.reserve(m.idx.size() + 3 * nf);
-    m.geom.reserve(first + nf);
-    for (uint64_t i = 0; i < nf; ++i) {
-        int64_t ix[3];
-        for (int j = 0; j < 3; ++j) {
-            ix[j] = fidx[3 * i + j];
-            VR_CHECK(ix[j] >= 0 && (uint64_t)ix[j] < ve->count, VR_ERR_INVALID_ARGUMENT,
-                     "ply: vertex index out of range in face " + std::to_string(i) + " of " + path);
-        }
-        for (int j = 0; j < 3; ++j) m.idx.push_back((uint32_t)(vbase + (uint64_t)ix[j]));
-        m.geom.push_back(gid);
-        for (int j = 0; j < 3; ++j) {
-            if (has_n) m.shading.insert(m.shading.end(), &nrm[3 * ix[j]], &nrm[3 * ix[j]] + 3);
-            if (has_uv) {
-                m.tex.push_back(uv[2 * ix[j]]);
-                m.tex.push_back(uv[2 * ix[j] + 1]);
-            } else {
-                m.tex.push_back(0.0f);
-                m.tex.push_back(0.0f);
+    const size_t i0 = m.idx.size(), s0 = m.shading.size(), t0 = m.tex.size();
+    m.idx.resize(i0 + 3 * nf);
+    m.geom.resize(first + nf);
+    if (has_n) m.shading.resize(s0 + 9 * nf);
+    m.tex.resize(t0 + 6 * nf);
+    std::vector<uint64_t> bad(nt, UINT64_MAX);
+    std::vector<f3> blo(nt, f3{FLT_MAX, FLT_MAX, FLT_MAX}), bhi(nt, f3{-FLT_MAX, -FLT_MAX, -FLT_MAX});
+    const uint64_t per = (nf + nt - 1) / nt;
+    parallel_for(nt, [&](unsigned t) {
+        f3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (uint64_t i = t * per; i < std::min(nf, (t + 1) * per); ++i) {
+            for (int j = 0; j < 3; ++j) {
+                const int64_t ix = fidx[3 * i + j];
+                if (ix < 0 || (uint64_t)ix >= ve->count) {
+                    bad[t] = i;
+                    return;
+                }
+                m.idx[i0 + 3 * i + j] = (uint32_t)(vbase + (uint64_t)ix);
+                if (has_n)
+                    for (int c = 0; c < 3; ++c) m.shading[s0 + 9 * i + 3 * j + c] = nrm[3 * ix + c];
+                m.tex[t0 + 6 * i + 2 * j] = has_uv ? uv[2 * ix] : 0.0f;
+                m.tex[t0 + 6 * i + 2 * j + 1] = has_uv ? uv[2 * ix + 1] : 0.0f;
+                insert(lo, hi, vertex(m.pos.data(), (uint32_t)(vbase + (uint64_t)ix)));
             }
-            insert(m.lo, m.hi, vertex(m.pos.data(), (uint32_t)(vbase + (uint64_t)ix[j])));
+            m.geom[first + i] = gid;
         }
-    }
+        blo[t] = lo;
+        bhi[t] = hi;
+    });
+    const uint64_t b = *std::min_element(bad.begin(), bad.end());
+    VR_CHECK(b == UINT64_MAX, VR_ERR_INVALID_ARGUMENT,
+             "ply: vertex index out of range in face " + std::to_string(b) + " of " + path);
+    for (unsigned t = 0; t < nt; ++t) merge(m.lo, m.hi, blo[t], bhi[t]);
     if (nf) m.spans.push_back({first, nf, false});
     m.mats.push_back(default_material());
     m.names.emplace_back();
