@@ -93,6 +93,14 @@ typedef struct vr_material {
     float ls;
 } vr_material;
 
+/* area_light<float, basic_triangle<3, float>> (area_light.h:16-50,
+ * detail/area_light.inl): intensity cl * kl over the triangle.  60 bytes. */
+typedef struct vr_area_light {
+    float cl[3];
+    float kl;
+    vr_triangle geometry;
+} vr_area_light;
+
 /* kernel_params subset used without a light list (kernels.h:38-132):
  * bounces, epsilon, background and ambient intensity (cl * kl). */
 typedef struct vr_kernel_params {
@@ -238,6 +246,21 @@ int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_material
 int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
                          const vr_kernel_params* kp, const vr_camera* cam, const vr_sched_params* sp,
                          float* d_accum, float* d_color, uint64_t* d_stats, void* stream);
+
+/* pathtracing::kernel with a light list (make_kernel_params w/ lights,
+ * kernels.h:137-187; next-event estimation with multiple importance sampling,
+ * detail/pathtracing.inl:116-196): every hit samples one of the num_lights
+ * area lights uniformly (sample_random_light, sampling.h:194-210) and traces
+ * a shadow ray (any hit); emitter hits after the first bounce are MIS
+ * weighted.  d_lights is device memory.  num_lights == 0 is vr_render_pt.
+ * d_stats (optional, 2 x uint64, or 4 with `counted`) as in vr_render_pt /
+ * vr_render_pt_counted; shadow rays count as secondary queries.  The
+ * reference-scheme launch shape (VR_SCHEDULE_PER_PIXEL) does not apply: the
+ * persistent kernel runs. */
+int vr_render_pt_lights(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                        const vr_area_light* d_lights, size_t num_lights, const vr_kernel_params* kp,
+                        const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                        uint64_t* d_stats, int counted, void* stream);
 
 /* ---- differentiation (no reference counterpart: the reference renders
  * only; these two calls are the forward/backward pair of a differentiable

@@ -32,6 +32,8 @@ RAY_DTYPE = np.dtype([("ori", "<f4", (3,)), ("dir", "<f4", (3,)), ("tmin", "<f4"
 HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 CAMERA_DTYPE = np.dtype([("eye", "<f4", (3,)), ("U", "<f4", (3,)), ("V", "<f4", (3,)), ("W", "<f4", (3,))])
 MAT_DTYPE = np.dtype([("type", "<i4"), ("cd", "<f4", (3,)), ("kd", "<f4"), ("ce", "<f4", (3,)), ("ls", "<f4")])
+# area_light<float, basic_triangle<3, float>>: cl, kl, triangle (60 B)
+LIGHT_DTYPE = np.dtype([("cl", "<f4", (3,)), ("kl", "<f4"), ("geometry", TRI_DTYPE)])
 RPARAMS_DTYPE = np.dtype([
     ("width", "<u4"), ("height", "<u4"), ("spp", "<u4"), ("frame_id", "<u4"), ("num_bounces", "<u4"),
     ("epsilon", "<f4"), ("bg", "<f4", (3,)), ("ambient", "<f4", (3,)), ("sfactor", "<f4"), ("dfactor", "<f4"),
@@ -65,6 +67,7 @@ def lib():
         l.vro_refit.argtypes = [P, SZ, P, P]
         l.vro_render_ao.argtypes = [P, P, P, SZ, P, P, P, P, C.c_int, C.c_int]
         l.vro_render_pt.argtypes = [P, P, P, SZ, P, P, P, P, P, C.c_int, C.c_int, P]
+        l.vro_render_pt_lights.argtypes = [P, P, P, SZ, P, P, SZ, P, P, P, P, C.c_int, C.c_int, P]
         l.vro_render_pt_adjoint.argtypes = [P, P, P, SZ, P, P, P, P, P, P, C.c_int, C.c_int]
         _lib = l
     return _lib
@@ -235,8 +238,9 @@ def render_ao(nodes, tris, indices, cam, params, rows=None) -> np.ndarray:
     return col
 
 
-def render_pt(nodes, tris, indices, mats, cam, params, rows=None):
-    """-> (color image, [primary rays, secondary rays])"""
+def render_pt(nodes, tris, indices, mats, cam, params, rows=None, lights=None):
+    """-> (color image, [primary rays, secondary rays]); `lights` (LIGHT_DTYPE)
+    turns on next-event estimation (shadow rays are counted as secondary)."""
     nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
     tris = np.ascontiguousarray(tris, TRI_DTYPE)
     indices = np.ascontiguousarray(indices, np.uint32)
@@ -247,8 +251,13 @@ def render_pt(nodes, tris, indices, mats, cam, params, rows=None):
     col = np.zeros((h, w, 4), np.float32)
     counts = np.zeros(2, np.uint64)
     y0, y1 = rows if rows else (0, h)
-    lib().vro_render_pt(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(cam), _p(params), _p(acc),
-                        _p(col), y0, y1, _p(counts))
+    if lights is None or len(lights) == 0:
+        lib().vro_render_pt(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(cam), _p(params), _p(acc),
+                            _p(col), y0, y1, _p(counts))
+    else:
+        lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+        lib().vro_render_pt_lights(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(lights), lights.size,
+                                   _p(cam), _p(params), _p(acc), _p(col), y0, y1, _p(counts))
     return col, counts
 
 
@@ -419,6 +428,21 @@ def ref_pt(nodes, tris, indices, mats, cam_in, params):
         _run(["pt", t.put("nodes.bin", nodes), nodes.size, t.put("tris.bin", tris), tris.size,
               t.put("idx.bin", np.asarray(indices, np.uint32)), t.put("mats.bin", mats), mats.size,
               t.put("cam.bin", cam_in), t.put("params.bin", params), t.path("out.bin")])
+        w, h = int(params[0]), int(params[1])
+        return np.fromfile(t.path("out.bin"), np.float32).reshape(h, w, 4)
+    finally:
+        t.close()
+
+
+def ref_ptl(nodes, tris, indices, mats, lights, cam_in, params):
+    """The reference's pathtracing::kernel with an area-light list (NEE + MIS)."""
+    t = _Tmp()
+    try:
+        lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+        _run(["ptl", t.put("nodes.bin", nodes), nodes.size, t.put("tris.bin", tris), tris.size,
+              t.put("idx.bin", np.asarray(indices, np.uint32)), t.put("mats.bin", mats), mats.size,
+              t.put("lights.bin", lights), lights.size, t.put("cam.bin", cam_in), t.put("params.bin", params),
+              t.path("out.bin")])
         w, h = int(params[0]), int(params[1])
         return np.fromfile(t.path("out.bin"), np.float32).reshape(h, w, 4)
     finally:

@@ -17,6 +17,8 @@
 //   camera  CAMIN OUT                   pinhole_camera basis after begin_frame
 //   ao      NODES NN TRIS NP IDX CAMIN PARAMS OUT   AO example kernel, tiled_sched
 //   pt      NODES NN TRIS NP IDX MATS NM CAMIN PARAMS OUT  pathtracing::kernel
+//   ptl     NODES NN TRIS NP IDX MATS NM LIGHTS NL CAMIN PARAMS OUT
+//                                       pathtracing::kernel with area lights (NEE + MIS)
 //   sah     TRIS N OUTNODES OUTIDX      binned_sah_builder (no spatial splits)
 //   bench   ...                         CPU baseline timing (see below)
 
@@ -32,6 +34,7 @@
 
 #include <visionaray/math/math.h>
 #include <visionaray/bvh.h>
+#include <visionaray/area_light.h>
 #include <visionaray/generic_material.h>
 #include <visionaray/kernels.h>
 #include <visionaray/material.h>
@@ -278,6 +281,46 @@ static void render_pt(Sched& sched, index_bvh<tri_t> const& tree, std::vector<ma
     sched.frame(kernel, sparams);
 }
 
+// area light input: cl[3] kl, then the triangle (geom_id prim_id v1 e1 e2):
+// 15 x 4 B, the member order of area_light<float, basic_triangle<3, float>>
+using light_t = area_light<float, tri_t>;
+
+static std::vector<light_t> load_lights(const char* path, size_t nl)
+{
+    auto raw = load<float>(path, nl * 15);
+    std::vector<light_t> lights;
+    lights.reserve(nl);
+    for (size_t i = 0; i < nl; ++i) {
+        const float* r = &raw[i * 15];
+        tri_t t;
+        std::memcpy(&t, r + 4, sizeof(tri_t));
+        light_t l(t);
+        l.set_cl(vec3(r[0], r[1], r[2]));
+        l.set_kl(r[3]);
+        lights.push_back(l);
+    }
+    return lights;
+}
+
+template <typename R, typename Sched>
+static void render_ptl(Sched& sched, index_bvh<tri_t> const& tree, std::vector<mat_t> const& mats,
+                       std::vector<light_t> const& lights, cam_probe const& cam, rparams const& p, accum_rt& rt)
+{
+    std::vector<index_bvh<tri_t>::bvh_ref> bvhs{tree.ref()};
+    auto blend_params = pixel_sampler::jittered_blend_type{};
+    blend_params.spp = p.spp;
+    blend_params.sfactor = p.sf;
+    blend_params.dfactor = p.df;
+    auto sparams = make_sched_params(blend_params, cam, rt);
+    auto kparams = make_kernel_params(
+            bvhs.data(), bvhs.data() + bvhs.size(), mats.data(), lights.data(), lights.data() + lights.size(),
+            (unsigned)p.bounces, p.eps, vec4(p.bg[0], p.bg[1], p.bg[2], 1.0f),
+            vec4(p.amb[0], p.amb[1], p.amb[2], 1.0f));
+    pathtracing::kernel<decltype(kparams)> kernel;
+    kernel.params = kparams;
+    sched.frame(kernel, sparams);
+}
+
 static void save_rt(const char* path, accum_rt const& rt)
 {
     save(path, reinterpret_cast<const float*>(rt.color.data()), rt.color.size() * 4);
@@ -428,6 +471,20 @@ int main(int argc, char** argv)
         tiled_sched<basic_ray<float>> sched(p.threads);
         for (int f = 0; f <= p.frame_id; ++f) render_pt<basic_ray<float>>(sched, tree, mats, cam, p, rt);
         save_rt(a[9], rt);
+        return 0;
+    }
+    if (mode == "ptl" && na == 12) {
+        auto tree = load_tree(a);
+        size_t nm = std::strtoull(a[6], nullptr, 10);
+        auto mats = load_mats(a[5], nm);
+        size_t nl = std::strtoull(a[8], nullptr, 10);
+        auto lights = load_lights(a[7], nl);
+        auto cam = load_camera(a[9]);
+        auto p = load_params(a[10]);
+        accum_rt rt(p.w, p.h);
+        tiled_sched<basic_ray<float>> sched(p.threads);
+        for (int f = 0; f <= p.frame_id; ++f) render_ptl<basic_ray<float>>(sched, tree, mats, lights, cam, p, rt);
+        save_rt(a[11], rt);
         return 0;
     }
     if (mode == "lbvhcpu" && na == 4) {

@@ -647,18 +647,67 @@ void vro_render_ao(const vro_node* nodes, const vro_tri* tris, const uint32_t* i
     }
 }
 
+/* area_light<float, basic_triangle>::sample (detail/area_light.inl:31-56):
+ * a point by sample_surface (math/detail/triangle.inl:69-82; two draws in
+ * statement order), the direction and distance from the reference point,
+ * intensity cl * kl, the normal normalize(cross(e1, e2)) (get_normal.h:108-113),
+ * the area 0.5 |e1 x e2| (triangle.inl:36-39) and the solid-angle pdf. */
+typedef struct {
+    v3 dir, intensity, normal;
+    float dist, area, pdf;
+} light_sample;
+
+static light_sample sample_area_light(const vro_light* l, v3 ref, rng_t* g)
+{
+    light_sample r;
+    const float u1 = rng_next(g);
+    const float u2 = rng_next(g);
+    const v3 e1 = ld3(l->geometry.e1), e2 = ld3(l->geometry.e2);
+    const v3 a = ld3(l->geometry.v1);
+    const v3 b = add3(a, e1);
+    const v3 c = add3(a, e2);
+    const v3 pos = add3(add3(muls(a, 1.0f - sqrtf(u1)), muls(muls(b, sqrtf(u1)), 1.0f - u2)),
+                        muls(muls(c, sqrtf(u1)), u2));
+    r.dir = sub3(pos, ref);
+    r.dist = sqrtf(dot3(r.dir, r.dir));
+    r.intensity = muls(ld3(l->cl), l->kl);
+    const v3 cr = cross3(e1, e2);
+    r.normal = normalize3(cr);
+    r.area = 0.5f * sqrtf(dot3(cr, cr));
+    const float ldotln = fabsf(dot3(neg3(normalize3(r.dir)), r.normal));
+    const float solid_angle = (ldotln * r.area) / (r.dist * r.dist);
+    r.pdf = 1.0f / solid_angle;
+    return r;
+}
+
+/* power_heuristic (sampling.h:73-79) */
+static inline float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
+
 /* pathtracing::kernel::operator() (detail/pathtracing.inl:43-233) for scalar
- * rays, no light list (num_lights == 0, so every MIS weight is 1), with
- * generic_material<emissive, matte> (detail/material/matte.inl:43-56,
- * detail/material/emissive.inl:31-40, brdf.h:50-84) and geometric normals
+ * rays with generic_material<emissive, matte> (detail/material/matte.inl:28-71,
+ * detail/material/emissive.inl:14-40, brdf.h:39-84) and geometric normals
  * normalize(cross(e1, e2)) (get_normal.h:108-113, get_surface.h:135-160).
+ * Without lights every MIS weight is 1.  With num_lights > 0 (lines 116-196):
+ * an emitter hit at bounce > 0 is weighted by power_heuristic(1, pdf_light /
+ * num_lights), pdf_light the solid-angle density of the hit triangle (the
+ * emissive material's sample() reports pdf 1); and every hit surface samples
+ * one light uniformly (light = int(u * num_lights)), traces the shadow ray
+ * [pos + L eps, tmin eps, tmax dist - eps] for any hit, and adds
+ * power_heuristic(pdf_light / num_lights, pdf_brdf * max(throughput)) *
+ * throughput * f * (cos / pdf_light) * num_lights when the surface still
+ * scatters, the shadow ray is clear and both cosines are positive.  The
+ * light's three draws come after the BRDF sample and before the roulette, on
+ * every hit (emitters included).
  * A ray that leaves the path (emission or a zero pdf) before bounce 2 ends the
  * loop at the next closest_hit in the reference; that extra query changes no
- * output and is not performed (or counted) here. */
+ * output and is not performed (or counted) here; neither is a shadow ray
+ * whose contribution is masked. */
 static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
-                      const vro_material* mats, const vro_render_params* p, vro_ray ray, rng_t* g,
-                      float out[4], uint64_t* counts)
+                      const vro_material* mats, const vro_light* lights, size_t num_lights,
+                      const vro_render_params* p, vro_ray ray, rng_t* g, float out[4], uint64_t* counts)
 {
+    const float inv_pi = 3.18309886183790691216444201928e-01f;
+    const float pi = 3.14159265358979323846264338328e+00f;
     float I[3] = {0, 0, 0};
     float tp[3] = {1, 1, 1};
     int result_hit = 0;
@@ -692,18 +741,63 @@ static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t
             rng_pair(g, p->arg_order, &a, &b);
             v3 sp = cosine_hemisphere(a, b);
             refl = normalize3(add3(add3(smul(sp.x, u), smul(sp.y, v)), smul(sp.z, w)));
-            pdf = dot3(n, refl) * 3.18309886183790691216444201928e-01f;
-            for (int k = 0; k < 3; ++k)
-                src[k] = 1.0f * ((m->cd[k] * m->kd) * 3.18309886183790691216444201928e-01f);
+            pdf = dot3(n, refl) * inv_pi;
+            for (int k = 0; k < 3; ++k) src[k] = 1.0f * ((m->cd[k] * m->kd) * inv_pi);
         }
         int zero_pdf = pdf <= 0.0f;
         int active = 1;
         if (emission) {
-            for (int k = 0; k < 3; ++k) I[k] += (1.0f * tp[k]) * src[k];
+            float wgt = 1.0f;
+            if (num_lights > 0 && bounce > 0) {
+                const v3 ce1 = ld3(pr->e1), ce2 = ld3(pr->e2);
+                const v3 cc = cross3(ce1, ce2);
+                const float A = 0.5f * sqrtf(dot3(cc, cc));
+                const v3 dl = sub3(pos, ori);
+                const float ld = sqrtf(dot3(dl, dl));
+                const v3 L = normalize3(dl);
+                const float ldotln = fabsf(dot3(neg3(L), gn));
+                const float solid_angle = (ldotln * A) / (ld * ld);
+                const float light_pdf = 1.0f / solid_angle;
+                wgt = power_heuristic(pdf, light_pdf / (float)num_lights);
+            }
+            for (int k = 0; k < 3; ++k) I[k] += (wgt * tp[k]) * src[k];
             active = 0;
         }
         if (zero_pdf) active = 0;
         v3 n = faceforward3(sn, view, gn);
+        if (num_lights > 0) {
+            const float u = rng_next(g);
+            size_t id = (size_t)(int)(u * (float)num_lights);
+            if (id >= num_lights) id = num_lights - 1;
+            const light_sample ls = sample_area_light(&lights[id], pos, g);
+            const float ld = ls.dist;
+            const v3 L = normalize3(ls.dir);
+            const v3 ln = faceforward3(ls.normal, neg3(L), ls.normal);
+            const float ldotn = dot3(L, n);
+            const float ldotln = fabsf(dot3(neg3(L), ln));
+            if (active && ldotn > 0.0f && ldotln > 0.0f) {
+                const vro_ray shadow = mkray(add3(pos, muls(L, p->epsilon)), L, p->epsilon, ld - p->epsilon);
+                const hitrec lhr = trace_one(nodes, tris, indices, num_prims, &shadow, 1, NULL);
+                if (counts) counts[1]++;
+                if (!lhr.hit) {
+                    /* matte: pdf dot(n, L) / pi; shade ((f pi) E) max(0, dot(n, L)) */
+                    float bpdf = dot3(n, L) * inv_pi;
+                    float prob = tp[0];
+                    if (tp[1] > prob) prob = tp[1];
+                    if (tp[2] > prob) prob = tp[2];
+                    bpdf *= prob;
+                    const float ndotl = tmax_f(0.0f, dot3(n, L));
+                    const float E[3] = {ls.intensity.x, ls.intensity.y, ls.intensity.z};
+                    const float wgt = power_heuristic(ls.pdf / (float)num_lights, bpdf);
+                    for (int k = 0; k < 3; ++k) {
+                        const float f = 1.0f * ((m->cd[k] * m->kd) * inv_pi);
+                        const float sh = ((f * pi) * E[k]) * ndotl;
+                        const float s = (sh * inv_pi) / ldotn;
+                        I[k] += (((wgt * tp[k]) * s) * (ldotn / ls.pdf)) * (float)num_lights;
+                    }
+                }
+            }
+        }
         float q = dot3(n, refl) / pdf;
         for (int k = 0; k < 3; ++k) tp[k] = tp[k] * (src[k] * q);
         if (zero_pdf) tp[0] = tp[1] = tp[2] = 0.0f;
@@ -829,9 +923,10 @@ void vro_render_pt_adjoint(const vro_node* nodes, const vro_tri* tris, const uin
 
 /* sample_pixel_impl with jittered_blend (detail/sched_common.h:427-526) under
  * basic_sched::frame seeding (detail/basic_sched.inl:100-112). */
-void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
-                   const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
-                   float* accum, float* color, int y_begin, int y_end, uint64_t* ray_counts)
+void vro_render_pt_lights(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                          const vro_material* mats, const vro_light* lights, size_t num_lights,
+                          const vro_camera* cam, const vro_render_params* p, float* accum, float* color,
+                          int y_begin, int y_end, uint64_t* ray_counts)
 {
     int w = (int)p->width, h = (int)p->height;
     for (int y = y_begin; y < y_end; ++y) {
@@ -842,11 +937,19 @@ void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* i
             for (uint32_t s = 0; s < p->spp; ++s) {
                 vro_ray r = primary_ray(cam, &g, p->arg_order, x, y, w, h);
                 float c[4];
-                pt_sample(nodes, tris, indices, num_prims, mats, p, r, &g, c, ray_counts);
+                pt_sample(nodes, tris, indices, num_prims, mats, lights, num_lights, p, r, &g, c, ray_counts);
                 for (int k = 0; k < 4; ++k) sum[k] += c[k];
             }
             for (int k = 0; k < 4; ++k) sum[k] /= (float)p->spp;
             blend_store(accum, color, (size_t)y * w + x, sum, p->sfactor, p->dfactor);
         }
     }
+}
+
+void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                   const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
+                   float* accum, float* color, int y_begin, int y_end, uint64_t* ray_counts)
+{
+    vro_render_pt_lights(nodes, tris, indices, num_prims, mats, NULL, 0, cam, p, accum, color, y_begin, y_end,
+                         ray_counts);
 }

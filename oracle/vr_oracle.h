@@ -91,6 +91,14 @@ typedef struct {
     float ls;
 } vro_material;
 
+/* area_light<float, basic_triangle<3, float>> (area_light.h:16-50): cl, kl,
+ * then the triangle.  60 B. */
+typedef struct {
+    float cl[3];
+    float kl;
+    vro_tri geometry;
+} vro_light;
+
 /* Path-tracer / AO parameters (kernels.h:38-74, pixel_sampler_types.h:51-69). */
 typedef struct {
     uint32_t width;
@@ -139,6 +147,13 @@ void vro_render_ao(const vro_node* nodes, const vro_tri* tris, const uint32_t* i
 void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
                    const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
                    float* accum, float* color, int y_begin, int y_end, uint64_t* ray_counts);
+
+/* render_pt with a light list (make_kernel_params w/ lights, kernels.h:137-187):
+ * next-event estimation with multiple importance sampling. */
+void vro_render_pt_lights(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
+                          const vro_material* mats, const vro_light* lights, size_t num_lights,
+                          const vro_camera* cam, const vro_render_params* p, float* accum, float* color,
+                          int y_begin, int y_end, uint64_t* ray_counts);
 
 /* Adjoint of render_pt (no reference counterpart): FP64 gradients of a loss
  * with image gradient dl_dimg (RGBA per pixel) w.r.t. matte albedo cd and

@@ -89,9 +89,29 @@ def main():
     pt = O.ref_pt(n_m, tris_m, i_m, mats, cam_in, pt_p)
     np.savez_compressed(OUT / "images.npz", cam_in=cam_in, ao_params=ao_p, ao=ao, mats=mats, tris_m=tris_m,
                         pt_params=pt_p, pt=pt)
-    for f in sorted(OUT.glob("*.npz")):
-        print(f.name, f.stat().st_size)
+    nee()
+
+
+def nee():
+    """pathtracing::kernel with an area-light list (make_kernel_params w/
+    lights): the lit room (two emissive ceiling triangles as the lights)."""
+    tris, mats, lights = S.lit_room_scene(level=3)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    w = h = 48
+    cam_in = O.camera_input(*S.config0_camera(w, h))
+    p = O.ref_params(w, h, spp=4, frame_id=0, num_bounces=6, epsilon=1e-4, bg=(0.05, 0.05, 0.05),
+                     ambient=(0.0, 0.0, 0.0), threads=4)
+    img = O.ref_ptl(nodes, tris, idx, mats, lights, cam_in, p)
+    np.savez_compressed(OUT / "nee.npz", tris=tris, mats=mats, lights=lights, cam_in=cam_in,
+                        cam_basis=O.ref_camera(cam_in), params=p, pt=img)
 
 
 if __name__ == "__main__":
-    main()
+    assert O.ref_available(), "oracle/_ref/vr_ref is not built (make -C oracle ref)"
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+    else:
+        main()
+    for f in sorted(OUT.glob("*.npz")):
+        print(f.name, f.stat().st_size)

@@ -1,0 +1,104 @@
+"""Path tracing with an area-light list (vr_render_pt_lights: next-event
+estimation with multiple importance sampling, detail/pathtracing.inl:116-196)
+on the GPU against the reference's own kernel (tests/golden/nee.npz) and the
+oracle, under the image bar of the other path-tracing tests: most pixels
+bit-identical, the rest within 4 sigma of the per-pixel spread over frames."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from visionaray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def nee_frame(V, bvh, mats, lights, kp, cam, w, h, spp, frame_id=0, counted=False):
+    rt = V.render_target(w, h)
+    sched = V.hip_sched()
+    sched.frame_id = frame_id
+    d_mats = V.DeviceArray.from_host(mats)
+    d_lights = V.DeviceArray.from_host(lights) if lights is not None else None
+    stats = V.DeviceArray(4, np.uint64, zero=True) if counted else None
+    sched.frame(V.pathtracing.kernel(bvh, d_mats, kp, d_lights),
+                V.make_sched_params(V.jittered_blend_type(spp=spp), cam, rt), stats=stats)
+    img = rt.color_host()
+    if counted:
+        st = stats.to_host()
+        return img, (int(st[0]), int(st[1])), st
+    return img, sched.rays(), None
+
+
+def stats_compare(got, want, sigma, tag):
+    exact = np.mean(np.all(got == want, axis=-1))
+    diff = np.abs(got[..., :3] - want[..., :3])
+    within = np.mean(np.all(diff <= 4.0 * sigma[..., :3] + 1e-6, axis=-1))
+    rel = np.sqrt(np.mean(diff ** 2)) / max(np.sqrt(np.mean(want[..., :3] ** 2)), 1e-12)
+    print(f"[{tag}] exact pixels {exact:.4f}, within 4 sigma {within:.4f}, global rel RMSE {rel:.2e}", flush=True)
+    return exact, within, rel
+
+
+def golden_setup(V):
+    g = load_golden("nee")
+    p = g["params"]
+    w, h, spp = int(p[0]), int(p[1]), int(p[2])
+    cam = g["cam_basis"].astype(np.float32).view(V.CAMERA_DTYPE)
+    kp = V.make_kernel_params(num_bounces=int(p[4]), epsilon=float(p[5]), bg_color=(*p[6:9], 1.0),
+                              ambient_color=(*p[9:12], 1.0))
+    return g, w, h, spp, cam, kp
+
+
+def test_nee_matches_reference_image(V):
+    g, w, h, spp, cam, kp = golden_setup(V)
+    bvh = V.lbvh_builder().build(g["tris"])
+    img, (npri, nsec), _ = nee_frame(V, bvh, g["mats"], g["lights"], kp, cam, w, h, spp)
+    assert npri == w * h * spp
+    frames = np.stack([nee_frame(V, bvh, g["mats"], g["lights"], kp, cam, w, h, spp, frame_id=f)[0]
+                       for f in range(1, 17)])
+    exact, within, rel = stats_compare(img, g["pt"], frames.std(axis=0), "nee golden")
+    assert exact >= 0.8 and within >= 0.995 and rel < 0.05
+
+
+@pytest.mark.parametrize("spp", [1, 8])
+def test_nee_matches_oracle(V, O, spp):
+    tris, mats, lights = S.lit_room_scene(level=4)
+    w, h = 96, 72
+    eye, center, up, fovy, _ = S.config0_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+    kp = V.make_kernel_params(num_bounces=6, epsilon=1e-4, ambient_color=(0.6, 0.7, 0.9, 1.0))
+    bvh = V.lbvh_builder().build(tris)
+    img, (npri, nsec), st = nee_frame(V, bvh, mats, lights, kp, cam, w, h, spp, counted=True)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    p = O.render_params(w, h, spp=spp, num_bounces=6, epsilon=1e-4, ambient=kp["ambient"][0])
+    want, counts = O.render_pt(nodes, tris, idx, mats, cam.astype(O.CAMERA_DTYPE), p, lights=lights)
+    assert npri == counts[0] == w * h * spp
+    assert abs(nsec - int(counts[1])) <= 0.01 * counts[1]
+    assert st[2] > 0 and st[3] > 0
+    frames = np.stack([nee_frame(V, bvh, mats, lights, kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 9)])
+    exact, within, rel = stats_compare(img, want, frames.std(axis=0), f"nee oracle spp={spp}")
+    assert exact >= 0.8 and within >= 0.99 and rel < 0.05
+
+
+def test_empty_light_list_is_the_plain_frame(V):
+    tris, mats, lights = S.lit_room_scene(level=3)
+    w, h = 64, 48
+    eye, center, up, fovy, _ = S.config0_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+    kp = V.make_kernel_params(num_bounces=5, epsilon=1e-4)
+    bvh = V.lbvh_builder().build(tris)
+    plain, _, _ = nee_frame(V, bvh, mats, None, kp, cam, w, h, 2)
+    empty, _, _ = nee_frame(V, bvh, mats, lights[:0], kp, cam, w, h, 2)
+    assert np.array_equal(plain, empty)
+    lit, _, _ = nee_frame(V, bvh, mats, lights, kp, cam, w, h, 2)
+    assert not np.array_equal(plain, lit)
+
+
+def test_nee_counted_frame_is_the_frame(V):
+    tris, mats, lights = S.lit_room_scene(level=3)
+    w, h = 64, 48
+    eye, center, up, fovy, _ = S.config0_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+    kp = V.make_kernel_params(num_bounces=6, epsilon=1e-4)
+    bvh = V.lbvh_builder().build(tris)
+    a, (pa, sa), _ = nee_frame(V, bvh, mats, lights, kp, cam, w, h, 4)
+    b, (pb, sb), st = nee_frame(V, bvh, mats, lights, kp, cam, w, h, 4, counted=True)
+    assert np.array_equal(a, b) and pa == pb and sa == sb

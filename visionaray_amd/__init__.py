@@ -24,13 +24,13 @@ import numpy as np
 from . import _lib
 from ._lib import (BVH_INFO_DTYPE, CAMERA_DTYPE, HIT_DTYPE, KERNEL_PARAMS_DTYPE, NODE_DTYPE, RAY_DTYPE,
                    SCHED_PARAMS_DTYPE, VR_BVH_KEEP_NODES, VRError, check, ptr)
-from .scenes import MAT_DTYPE, TRI_DTYPE
+from .scenes import LIGHT_DTYPE, MAT_DTYPE, TRI_DTYPE
 
 __all__ = [
     "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build",
     "closest_hit", "any_hit", "trace", "pinhole_camera", "make_kernel_params", "make_sched_params",
     "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
-    "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "synchronize", "model",
+    "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "LIGHT_DTYPE", "synchronize", "model",
 ]
 
 _initialized = False
@@ -350,10 +350,13 @@ class pathtracing:
     @dataclass
     class kernel:
         """pathtracing::kernel (detail/pathtracing.inl:33-244): BVH, materials
-        indexed by geom_id, kernel params."""
+        indexed by geom_id, kernel params and, optionally, the area-light list
+        of make_kernel_params w/ lights (kernels.h:137-187; LIGHT_DTYPE on the
+        device), which turns on next-event estimation with MIS."""
         bvh: BVH
         materials: DeviceArray
         params: np.ndarray
+        lights: "DeviceArray | None" = None
 
 
 @dataclass
@@ -485,7 +488,14 @@ class hip_sched:
         """One frame; with `stats` (4 x uint64 device array) the counting
         variant also accumulates traversal-node visits and triangle tests."""
         s = self._sp(sp)
-        if isinstance(kernel, pathtracing.kernel) and stats is not None:
+        if isinstance(kernel, pathtracing.kernel) and kernel.lights is not None and kernel.lights.count > 0:
+            counted = stats is not None
+            check(lib().vr_render_pt_lights(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,
+                                            kernel.lights.ptr, kernel.lights.count, ptr(kernel.params),
+                                            ptr(sp.cam), ptr(s), sp.rt.accum.ptr, sp.rt.color.ptr,
+                                            (stats if counted else self.ray_counts).ptr, int(counted),
+                                            _stream(stream)))
+        elif isinstance(kernel, pathtracing.kernel) and stats is not None:
             check(lib().vr_render_pt_counted(kernel.bvh.handle, kernel.materials.ptr, kernel.materials.count,
                                              ptr(kernel.params), ptr(sp.cam), ptr(s), sp.rt.accum.ptr,
                                              sp.rt.color.ptr, stats.ptr, _stream(stream)))

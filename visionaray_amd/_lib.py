@@ -85,6 +85,7 @@ SIGNATURES = {
     "vr_destroy_bvh": (I, [P]),
     "vr_destroy_batch": (I, [P]),
     "vr_last_error_string": (C.c_char_p, []),
+    "vr_render_pt_lights": (I, [P, P, SZ, P, SZ, P, P, P, P, P, P, I, P]),
     "vr_mesh_load": (I, [C.POINTER(C.c_char_p), SZ, C.POINTER(P)]),
     "vr_mesh_get_info": (I, [P, P]),
     "vr_mesh_triangles": (I, [P, P]),

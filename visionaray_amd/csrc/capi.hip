@@ -23,6 +23,10 @@ int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, con
               unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st);
 int render_ao(const bvh_state& bs, const vr_camera& cam, const vr_sched_params& sp, uint32_t samples, float radius,
               const float bg[3], float* accum, float* color, hipStream_t st);
+int render_pt_lights(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_area_light* lights,
+                     size_t num_lights, const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp,
+                     float* accum, float* color, unsigned long long* ray_counts, uint32_t* work, bool counted,
+                     hipStream_t st);
 int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
                    uint32_t* indices);
 int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
@@ -567,6 +571,26 @@ int vr_render_pt(vr_bvh bvh, const vr_material* d_materials, size_t num_material
     return render_pt(bvh->s, d_materials, num_materials, *kp, *cam, *sp, d_accum, d_color,
                      reinterpret_cast<unsigned long long*>(d_ray_counts), work_counter(), false,
                      (hipStream_t)stream);
+}
+
+int vr_render_pt_lights(vr_bvh bvh, const vr_material* d_materials, size_t num_materials,
+                        const vr_area_light* d_lights, size_t num_lights, const vr_kernel_params* kp,
+                        const vr_camera* cam, const vr_sched_params* sp, float* d_accum, float* d_color,
+                        uint64_t* d_stats, int counted, void* stream)
+{
+    VR_CHECK(bvh && kp && cam && sp && d_accum && d_color && (d_stats || !counted), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_lights: null argument");
+    VR_CHECK(num_lights == 0 || d_lights, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_lights: null light list");
+    VR_CHECK(bvh->s.num_prims == 0 || (d_materials && num_materials > 0), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_lights: no materials");
+    VR_CHECK(sp->spp >= 1, VR_ERR_INVALID_ARGUMENT, "vr_render_pt_lights: spp must be >= 1");
+    VR_CHECK((uint64_t)sp->width * sp->height < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "vr_render_pt_lights: image too large");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return render_pt_lights(bvh->s, d_materials, num_materials, d_lights, num_lights, *kp, *cam, *sp, d_accum,
+                            d_color, reinterpret_cast<unsigned long long*>(d_stats), work_counter(), counted != 0,
+                            (hipStream_t)stream);
 }
 
 int vr_render_pt_counted(vr_bvh bvh, const vr_material* d_materials, size_t num_materials, const vr_kernel_params* kp,

@@ -127,6 +127,8 @@ struct scene_desc {
     float* accum;
     float* color;
     const float* dl;  // adjoint: d loss / d image, RGBA per pixel
+    const vr_area_light* lights;  // NEE kernels: the light list
+    uint32_t num_lights;
 };
 
 struct pt_args {
@@ -182,7 +184,9 @@ __device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t
     return (uint32_t)x < w && (uint32_t)y < h;
 }
 
-enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_FROZEN_RR = 4 };
+// kernel modes; PT_NEE (a light list: next-event estimation with MIS) is
+// combined with PT_RENDER or PT_COUNT
+enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_FROZEN_RR = 4, PT_NEE = 8 };
 constexpr int ADJ_MAX_VERTICES = 16;
 
 // State of one pixel's path (the reference keeps it in the locals of
@@ -200,6 +204,22 @@ struct path_t {
     uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
     uint32_t nmid;
 };
+
+// Next-event estimation state of a lane: the query in flight is the shadow
+// ray of the last vertex (its contribution `add` counts when nothing is hit),
+// and the path goes on with (co, cd) afterwards unless it ended there.
+template <bool NEE>
+struct nee_t {
+    static constexpr bool shadow = false;
+};
+template <>
+struct nee_t<true> {
+    bool shadow, cont;
+    f3 add, co, cd;
+};
+
+// power_heuristic (sampling.h:73-79)
+__device__ inline float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
 
 template <bool ADJ>
 __device__ __forceinline__ void start_sample(path_t<ADJ>& P)
@@ -223,12 +243,28 @@ __device__ __forceinline__ void start_sample(path_t<ADJ>& P)
 //   d I_c / d cd_{m,c} = I_c * (vertices with material m) / cd_{m,c}
 //   d I_c / d ce_{e,c} = T_c * ls_e.
 // Per-block FP32 partials in LDS; FP64 only in the final reduction.
+//
+// NEE (a light list, pathtracing.inl:116-196): an emitter hit at bounce > 0
+// is weighted by power_heuristic(1, pdf_light / num_lights) (the emissive
+// material's sample() reports pdf 1), pdf_light the solid-angle density of
+// the hit triangle; every hit draws a light (u), a point on it (u1, u2) after
+// the BRDF sample and before the roulette, and a vertex that still scatters
+// with both cosines positive gets a shadow ray [pos + L eps, tmin eps,
+// tmax dist - eps] (so, sd, smax; Q.shadow) whose contribution Q.add
+// = power_heuristic(pdf_light / num_lights, pdf_brdf max(throughput))
+//   * throughput * f * (cos / pdf_light) * num_lights
+// counts if it reaches the light.  The reference traces the masked shadow
+// rays too; they change no output and are not traced here.
 template <int MODE>
-__device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A, path_t<MODE == PT_ADJOINT>& P,
-                                      const hit_t& hr, f3 o, f3 d, f3& no, f3& nd, float* s_grad)
+__device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
+                                      path_t<(MODE & ~PT_NEE) == PT_ADJOINT>& P, nee_t<(MODE & PT_NEE) != 0>& Q,
+                                      const hit_t& hr, f3 o, f3 d, f3& no, f3& nd, f3& so, f3& sd, float& smax,
+                                      float* s_grad)
 {
-    constexpr bool ADJ = MODE == PT_ADJOINT;
-    constexpr bool FROZEN = MODE == PT_FROZEN_RR;
+    constexpr bool ADJ = (MODE & ~PT_NEE) == PT_ADJOINT;
+    constexpr bool FROZEN = (MODE & ~PT_NEE) == PT_FROZEN_RR;
+    constexpr bool NEE = (MODE & PT_NEE) != 0;
+    constexpr float INV_PI = 3.18309886183790691216444201928e-01f;
     if (hr.tri < 0) {
         // escaped: ambient light times throughput (pathtracing.inl:69-77)
         const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), P.tp);
@@ -287,8 +323,19 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A, pat
     const bool zero_pdf = pdf <= 0.0f;
     bool active = true;
     if (emission) {
-        // no light list: MIS weight 1 (pathtracing.inl:134-146)
-        const f3 e = mul3(mk3(1.0f * P.tp.x, 1.0f * P.tp.y, 1.0f * P.tp.z), src);
+        // MIS weight 1 without a light list (pathtracing.inl:116-146)
+        float wgt = 1.0f;
+        if (NEE && S.num_lights > 0 && P.bounce > 0) {
+            const f3 cc = cross3(ldtri3(tb), ldtri3(tcq));
+            const float area = 0.5f * sqrtf(dot3(cc, cc));
+            const f3 dl = sub3(pos, o);
+            const float ld = sqrtf(dot3(dl, dl));
+            const f3 L = normalize3(dl);
+            const float ldotln = fabsf(dot3(neg3(L), gn));
+            const float solid_angle = (ldotln * area) / (ld * ld);
+            wgt = power_heuristic(pdf, (1.0f / solid_angle) / (float)S.num_lights);
+        }
+        const f3 e = mul3(mk3(wgt * P.tp.x, wgt * P.tp.y, wgt * P.tp.z), src);
         P.I = add3(P.I, e);
         active = false;
         if (ADJ) {
@@ -310,6 +357,64 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A, pat
     }
     if (zero_pdf) active = false;
     const f3 n = faceforward3(gn, view, gn);
+    if constexpr (NEE) {
+      if (S.num_lights > 0) {
+        // sample_random_light (sampling.h:194-210): light int(u * num_lights);
+        // area_light::sample (detail/area_light.inl:31-56) with sample_surface
+        // (math/detail/triangle.inl:69-82)
+        const float ul = rng_next(P.g);
+        uint32_t li = (uint32_t)(int)(ul * (float)S.num_lights);
+        li = li < S.num_lights ? li : S.num_lights - 1u;
+        const vr_area_light* lt = S.lights + li;
+        const float u1 = rng_next(P.g);
+        const float u2 = rng_next(P.g);
+        const f3 lv1 = mk3(lt->geometry.v1[0], lt->geometry.v1[1], lt->geometry.v1[2]);
+        const f3 le1 = mk3(lt->geometry.e1[0], lt->geometry.e1[1], lt->geometry.e1[2]);
+        const f3 le2 = mk3(lt->geometry.e2[0], lt->geometry.e2[1], lt->geometry.e2[2]);
+        const f3 lv2 = add3(lv1, le1), lv3 = add3(lv1, le2);
+        const float s1 = sqrtf(u1);
+        const f3 lp = add3(add3(muls3(lv1, 1.0f - s1), muls3(muls3(lv2, s1), 1.0f - u2)), muls3(muls3(lv3, s1), u2));
+        const f3 ldir = sub3(lp, pos);
+        const float ldist = sqrtf(dot3(ldir, ldir));
+        const f3 lc = cross3(le1, le2);
+        const f3 lnorm = normalize3(lc);
+        const float larea = 0.5f * sqrtf(dot3(lc, lc));
+        const float lsolid = (fabsf(dot3(neg3(normalize3(ldir)), lnorm)) * larea) / (ldist * ldist);
+        const float lpdf = 1.0f / lsolid;
+        const f3 L = normalize3(ldir);
+        const f3 ln = faceforward3(lnorm, neg3(L), lnorm);
+        const float ldotn = dot3(L, n);
+        const float ldotln = fabsf(dot3(neg3(L), ln));
+        if (active && ldotn > 0.0f && ldotln > 0.0f) {
+            // matte::pdf and matte::shade (matte.inl:28-71): pdf n.L / pi,
+            // shade ((f pi) E) max(0, n.L), then * 1/pi / (L.n)
+            float bpdf = dot3(n, L) * INV_PI;
+            float prob = P.tp.x;
+            if (P.tp.y > prob) prob = P.tp.y;
+            if (P.tp.z > prob) prob = P.tp.z;
+            bpdf *= prob;
+            const float ndotl = tmax_f(0.0f, dot3(n, L));
+            const float wgt = power_heuristic(lpdf / (float)S.num_lights, bpdf);
+            const float nl = (float)S.num_lights;
+            const float kd = m->kd;
+            const float el[3] = {lt->cl[0] * lt->kl, lt->cl[1] * lt->kl, lt->cl[2] * lt->kl};
+            const float tpc[3] = {P.tp.x, P.tp.y, P.tp.z};
+            float add[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float f = 1.0f * ((m->cd[k] * kd) * INV_PI);
+                const float sh = ((f * 3.14159265358979323846264338328e+00f) * el[k]) * ndotl;
+                const float sc = (sh * INV_PI) / ldotn;
+                add[k] = (((wgt * tpc[k]) * sc) * (ldotn / lpdf)) * nl;
+            }
+            Q.add = mk3(add[0], add[1], add[2]);
+            Q.shadow = true;
+            so = add3(pos, muls3(L, S.kp.epsilon));
+            sd = L;
+            smax = ldist - S.kp.epsilon;
+        }
+      }
+    }
     const float q = dot3(n, refl) / pdf;
     P.tp = mk3(P.tp.x * (src.x * q), P.tp.y * (src.y * q), P.tp.z * (src.z * q));
     if (zero_pdf) P.tp = mk3(0.0f, 0.0f, 0.0f);
@@ -386,8 +491,9 @@ __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 template <int MODE, bool BATCH, int BLK, int LDSD>
 __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 {
-    constexpr bool COUNT = MODE == PT_COUNT;
-    constexpr bool ADJ = MODE == PT_ADJOINT;
+    constexpr bool COUNT = (MODE & ~PT_NEE) == PT_COUNT;
+    constexpr bool ADJ = (MODE & ~PT_NEE) == PT_ADJOINT;
+    constexpr bool NEE = (MODE & PT_NEE) != 0;
     __shared__ uint32_t stack[LDSD * BLK];
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
@@ -404,6 +510,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
     int32_t px = -1;
     bool done = false;
     path_t<ADJ> P;
+    nee_t<NEE> Q{};
     P.g = rng_t{1u};
     P.s = 0;
     P.sum = make_float4(0, 0, 0, 0);
@@ -432,22 +539,50 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                 tc.nodes = 0;
                 tc.tris = 0;
             }
-            if (shade<MODE>(S, A, P, T.r, T.ori, T.dir, no, nd, s_grad)) {
-                if (end_sample(S, P)) {
-                    if (!ADJ) write_pixel(S, (uint32_t)px, P.sum);
-                    px = -1;
-                } else {
-                    const uint32_t w = S.sp.width, h = S.sp.height;
-                    const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
-                    primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, no, nd);
-                    start_sample(P);
+            bool ended = false, shadow_now = false;
+            if (NEE && Q.shadow) {
+                // the vertex's shadow ray is done: the light counts if it is clear
+                if constexpr (NEE) {
+                    Q.shadow = false;
+                    if (T.r.tri < 0) P.I = add3(P.I, Q.add);
+                    ended = !Q.cont;
+                    no = Q.co;
+                    nd = Q.cd;
+                }
+            } else {
+                f3 so, sd;
+                float smax = 0.0f;
+                ended = shade<MODE>(S, A, P, Q, T.r, T.ori, T.dir, no, nd, so, sd, smax, s_grad);
+                if constexpr (NEE) {
+                    if (Q.shadow) {
+                        Q.cont = !ended;
+                        Q.co = no;
+                        Q.cd = nd;
+                        trav_begin(T, S.b, so, sd, S.kp.epsilon, smax);
+                        tracing = true;
+                        ++n_secondary;
+                        shadow_now = true;
+                    }
                 }
             }
-            if (px >= 0) {
-                trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
-                tracing = true;
-                if (P.bounce == 0) ++n_primary;
-                else ++n_secondary;
+            if (!shadow_now) {
+                if (ended) {
+                    if (end_sample(S, P)) {
+                        if (!ADJ) write_pixel(S, (uint32_t)px, P.sum);
+                        px = -1;
+                    } else {
+                        const uint32_t w = S.sp.width, h = S.sp.height;
+                        const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
+                        primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, no, nd);
+                        start_sample(P);
+                    }
+                }
+                if (px >= 0) {
+                    trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
+                    tracing = true;
+                    if (P.bounce == 0) ++n_primary;
+                    else ++n_secondary;
+                }
             }
         }
 
@@ -510,7 +645,8 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         for (;;) {
             if (__ballot(tracing) == 0) break;
             if (__popcll(__ballot(pending)) >= VR_PT_REGEN_LANES) break;
-            if (tracing && trav_iter<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc)) {
+            if (tracing &&
+                trav_iter<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc, NEE && Q.shadow)) {
                 tracing = false;
                 pending = true;
             }
@@ -577,8 +713,10 @@ __global__ __launch_bounds__(BLK) void k_pathtrace_plain(pt_args A)
                 if (P.bounce == 0) ++n_primary;
                 else ++n_secondary;
                 const hit_t hr = traverse_binary<false, LDS_STACK, BLK>(S.b, o, d, 0.0f, FLT_MAX, st);
-                f3 no, nd;
-                if (shade<PT_RENDER>(S, A, P, hr, o, d, no, nd, nullptr)) break;
+                f3 no, nd, so, sd;
+                float smax;
+                nee_t<false> Q;
+                if (shade<PT_RENDER>(S, A, P, Q, hr, o, d, no, nd, so, sd, smax, nullptr)) break;
                 o = no;
                 d = nd;
             }
@@ -796,6 +934,28 @@ int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_ma
     A.tile = shape.tile;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
     return counted ? launch_pt<PT_COUNT, false>(A, 0, shape, st) : launch_pt<PT_RENDER, false>(A, 0, shape, st);
+}
+
+// the frame with a light list (kernel_params w/ lights, kernels.h:137-187)
+int render_pt_lights(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_area_light* lights,
+                     size_t num_lights, const vr_kernel_params& kp, const vr_camera& cam, const vr_sched_params& sp,
+                     float* accum, float* color, unsigned long long* ray_counts, uint32_t* work, bool counted,
+                     hipStream_t st)
+{
+    if (num_lights == 0) return render_pt(bs, mats, num_mats, kp, cam, sp, accum, color, ray_counts, work, counted, st);
+    if ((size_t)sp.width * sp.height == 0) return VR_OK;
+    VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "render_pt_lights: BVH deeper than the traversal stack");
+    VR_CHECK(num_lights < (1ull << 31), VR_ERR_INVALID_ARGUMENT, "render_pt_lights: more than 2^31 lights");
+    VR_CHECK((uint64_t)padded_pixels(sp.width, sp.height, g_shape.tile) < (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "render_pt_lights: image too large");
+    pt_args A =
+        single_args(bs, mats, num_mats, kp, cam, sp, accum, color, nullptr, ray_counts, work, nullptr, g_arg_order);
+    A.tile = g_shape.tile;
+    A.single.lights = lights;
+    A.single.num_lights = (uint32_t)num_lights;
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
+    return counted ? launch_pt<PT_NEE | PT_COUNT, false>(A, 0, g_shape, st)
+                   : launch_pt<PT_NEE | PT_RENDER, false>(A, 0, g_shape, st);
 }
 
 int render_pt_frozen_rr(const bvh_state& bs, const vr_material* mats, const vr_material* rr_mats, size_t num_mats,

@@ -322,9 +322,11 @@ constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
 // at different depths stay busy; only the order in which triangles are
 // tested changes (Aila and Laine 2009, speculative traversal).  Returns true
 // when this lane's query is finished (T.r is its result).
+// any_rt: this query is an any-hit query (a shadow ray) although the kernel
+// traces closest hits otherwise.
 template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
 __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
-                                          uint32_t* __restrict__ spill, trace_counters* cnt)
+                                          uint32_t* __restrict__ spill, trace_counters* cnt, bool any_rt = false)
 {
     if (b.num_prims == 0) return true;
     auto push = [&](uint32_t e) {
@@ -399,7 +401,7 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
             T.r.u = u;
             T.r.v = v;
             T.r.tri = (int32_t)k;
-            if (ANY_HIT) return true;
+            if (ANY_HIT || any_rt) return true;
         }
     }
     return false;

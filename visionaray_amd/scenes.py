@@ -244,6 +244,44 @@ def config0_scene(level: int = 5, seed: int = 0) -> np.ndarray:
     return concat(room(1.0), icosphere(level, 0.3, jitter=0.002, rng=rng))
 
 
+# area_light<float, basic_triangle<3, float>> (area_light.h): cl, kl, triangle
+LIGHT_DTYPE = np.dtype([("cl", "<f4", (3,)), ("kl", "<f4"), ("geometry", TRI_DTYPE)])
+assert LIGHT_DTYPE.itemsize == 60
+
+
+def lights_from_emissive(tris: np.ndarray, mats: np.ndarray) -> np.ndarray:
+    """One area light per triangle whose material is emissive, in triangle
+    order, with cl = ce and kl = ls (as the reference's viewer builds its
+    light list from the emissive triangles, src/viewer/viewer.cpp:3028-3083)."""
+    m = mats[np.minimum(tris["geom_id"], mats.size - 1)]
+    sel = m["type"] == 1
+    out = np.zeros(int(sel.sum()), LIGHT_DTYPE)
+    out["geometry"] = tris[sel]
+    out["cl"] = m["ce"][sel]
+    out["kl"] = m["ls"][sel]
+    return out
+
+
+def lit_room_scene(level: int = 4, light_half: float = 0.25, emission: float = 8.0, seed: int = 0):
+    """The config-0 room and icosphere lit by a square ceiling light of half
+    size `light_half` just below the ceiling (two emissive triangles facing
+    down).  Materials: 6 matte walls/sphere colours (geom 0-5), the light
+    (geom 6).  Returns (tris, mats, lights)."""
+    base = config0_scene(level=level, seed=seed)
+    base["geom_id"] = np.arange(base.size) % 6
+    h, y = np.float32(light_half), np.float32(0.98)
+    a = np.array([[-h, y, -h], [-h, y, -h]], np.float32)
+    b = np.array([[h, y, h], [-h, y, h]], np.float32)
+    c = np.array([[h, y, -h], [h, y, h]], np.float32)
+    light = make_triangles(a, b, c, geom_id=6)
+    tris = concat(base, light)
+    mats = lambert_materials(7, seed=seed + 5)
+    mats["type"][6] = 1
+    mats["ce"][6] = (1.0, 0.9, 0.75)
+    mats["ls"][6] = emission
+    return tris, mats, lights_from_emissive(tris, mats)
+
+
 def config1_scene(cells: int = 1448, spheres: int = 2000, sphere_level: int = 3) -> np.ndarray:
     """fBm heightfield (2 * cells^2 triangles, seed 1) plus `spheres` level-3
     icospheres (radius log-U[0.005, 0.05], seed 2) over the heightfield box:
