@@ -193,6 +193,7 @@ def main():
     extras = {}
     if not args.no_extras:
         extras["reference_scheme_frame"] = reference_scheme_frame(V, kernel, sp, stream, kernel_ms)
+        extras["nee_frame"] = nee_frame(V, S, tris, mats, kernel, sp, stream, kernel_ms)
         extras["heightfield_only"] = heightfield_only(args, V, S, stream)
 
     cpu = None
@@ -274,6 +275,31 @@ def reference_scheme_frame(V, kernel, sp, stream, kernel_ms):
            "scheme": "one thread per pixel in 16x16 blocks, binary-tree traversal (cuda_sched + intersect.inl)"}
     log(f"reference-scheme frame: {ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} incoherent Mrays/s); "
         f"this build x{out['speedup_of_this_build']:.2f}")
+    return out
+
+
+def nee_frame(V, S, tris, mats, kernel, sp, stream, kernel_ms):
+    """The same frame with next-event estimation: every emissive triangle of
+    the scene is an area light (vr_render_pt_lights); shadow rays count as
+    incoherent rays."""
+    lights = S.lights_from_emissive(tris, mats)
+    d_lights = V.DeviceArray.from_host(lights)
+    k = V.pathtracing.kernel(kernel.bvh, kernel.materials, kernel.params, d_lights)
+    sched = V.hip_sched()
+    sched.frame(k, sp, stream=stream)
+    stream.synchronize()
+    sched.reset_counters()
+    e0, e1 = V.Event(), V.Event()
+    e0.record(stream)
+    sched.frame(k, sp, stream=stream)
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1)
+    prim, sec = sched.rays()
+    out = {"lights": int(lights.size), "ms": round(ms, 3), "rays_per_frame": {"primary": prim, "secondary": sec},
+           "incoherent_mrays_per_s": round(sec / (ms * 1e-3) / 1e6, 3),
+           "relative_to_plain_frame": round(ms / kernel_ms, 3)}
+    log(f"NEE frame ({lights.size} area lights): {ms:.2f} ms, {out['incoherent_mrays_per_s']:.1f} incoherent "
+        f"Mrays/s (closest + shadow)")
     return out
 
 

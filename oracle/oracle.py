@@ -68,6 +68,8 @@ def lib():
         l.vro_render_ao.argtypes = [P, P, P, SZ, P, P, P, P, C.c_int, C.c_int]
         l.vro_render_pt.argtypes = [P, P, P, SZ, P, P, P, P, P, C.c_int, C.c_int, P]
         l.vro_render_pt_lights.argtypes = [P, P, P, SZ, P, P, SZ, P, P, P, P, C.c_int, C.c_int, P]
+        l.vro_query_log.argtypes = [P, SZ]
+        l.vro_query_log_count.restype = SZ
         l.vro_render_pt_adjoint.argtypes = [P, P, P, SZ, P, P, P, P, P, P, C.c_int, C.c_int]
         _lib = l
     return _lib
@@ -259,6 +261,23 @@ def render_pt(nodes, tris, indices, mats, cam, params, rows=None, lights=None):
         lib().vro_render_pt_lights(_p(nodes), _p(tris), _p(indices), tris.size, _p(mats), _p(lights), lights.size,
                                    _p(cam), _p(params), _p(acc), _p(col), y0, y1, _p(counts))
     return col, counts
+
+
+QUERY_DTYPE = np.dtype([("ray", RAY_DTYPE), ("any", "<i4"), ("hit", "<i4"), ("prim_id", "<i4"), ("t", "<f4"),
+                        ("bounce", "<u4"), ("pixel", "<u4")])
+
+
+def render_pt_queries(nodes, tris, indices, mats, cam, params, rows, lights=None, cap=1 << 20):
+    """render_pt over `rows` that also returns every query it traced
+    (QUERY_DTYPE: ray, any-hit flag, hit, prim_id, t, bounce, pixel)."""
+    buf = np.zeros(cap, QUERY_DTYPE)
+    lib().vro_query_log(_p(buf), cap)
+    try:
+        img, counts = render_pt(nodes, tris, indices, mats, cam, params, rows=rows, lights=lights)
+        n = lib().vro_query_log_count()
+    finally:
+        lib().vro_query_log(None, 0)
+    return img, counts, buf[:n].copy()
 
 
 def render_pt_adjoint(nodes, tris, indices, mats, cam, params, dl_dimg, rows=None):

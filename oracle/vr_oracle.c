@@ -680,6 +680,34 @@ static light_sample sample_area_light(const vro_light* l, v3 ref, rng_t* g)
     return r;
 }
 
+/* Optional log of the queries the path tracer traces (diagnostics of the
+ * tests): ray, any-hit flag, result, bounce and pixel. */
+static vro_query* g_log;
+static size_t g_log_cap, g_log_n;
+static uint32_t g_log_pixel;
+
+void vro_query_log(vro_query* buf, size_t cap)
+{
+    g_log = buf;
+    g_log_cap = cap;
+    g_log_n = 0;
+}
+
+size_t vro_query_log_count(void) { return g_log_n; }
+
+static void log_query(const vro_ray* r, int any, const hitrec* h, uint32_t bounce)
+{
+    if (!g_log || g_log_n >= g_log_cap) return;
+    vro_query* q = &g_log[g_log_n++];
+    q->ray = *r;
+    q->any = any;
+    q->hit = h->hit;
+    q->prim_id = h->hit ? h->prim_id : -1;
+    q->t = h->t;
+    q->bounce = bounce;
+    q->pixel = g_log_pixel;
+}
+
 /* power_heuristic (sampling.h:73-79) */
 static inline float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
 
@@ -713,6 +741,7 @@ static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t
     int result_hit = 0;
     for (uint32_t bounce = 0; bounce < p->num_bounces; ++bounce) {
         hitrec hr = trace_one(nodes, tris, indices, num_prims, &ray, 0, NULL);
+        log_query(&ray, 0, &hr, bounce);
         if (counts) counts[bounce == 0 ? 0 : 1]++;
         if (!hr.hit) {
             for (int k = 0; k < 3; ++k) I[k] += p->ambient[k] * tp[k];
@@ -778,6 +807,7 @@ static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t
             if (active && ldotn > 0.0f && ldotln > 0.0f) {
                 const vro_ray shadow = mkray(add3(pos, muls(L, p->epsilon)), L, p->epsilon, ld - p->epsilon);
                 const hitrec lhr = trace_one(nodes, tris, indices, num_prims, &shadow, 1, NULL);
+                log_query(&shadow, 1, &lhr, bounce);
                 if (counts) counts[1]++;
                 if (!lhr.hit) {
                     /* matte: pdf dot(n, L) / pi; shade ((f pi) E) max(0, dot(n, L)) */
@@ -933,6 +963,7 @@ void vro_render_pt_lights(const vro_node* nodes, const vro_tri* tris, const uint
         for (int x = 0; x < w; ++x) {
             rng_t g;
             rng_seed(&g, vro_make_random_seed(y * w + x, (int32_t)p->frame_id));
+            g_log_pixel = (uint32_t)(y * w + x);
             float sum[4] = {0, 0, 0, 0};
             for (uint32_t s = 0; s < p->spp; ++s) {
                 vro_ray r = primary_ray(cam, &g, p->arg_order, x, y, w, h);

@@ -155,6 +155,20 @@ void vro_render_pt_lights(const vro_node* nodes, const vro_tri* tris, const uint
                           const vro_camera* cam, const vro_render_params* p, float* accum, float* color,
                           int y_begin, int y_end, uint64_t* ray_counts);
 
+/* Diagnostics: log every query the path tracer traces (closest hits and
+ * shadow rays) into buf[cap] until vro_query_log(NULL, 0). */
+typedef struct {
+    vro_ray ray;
+    int32_t any;
+    int32_t hit;
+    int32_t prim_id;
+    float t;
+    uint32_t bounce;
+    uint32_t pixel;
+} vro_query;
+void vro_query_log(vro_query* buf, size_t cap);
+size_t vro_query_log_count(void);
+
 /* Adjoint of render_pt (no reference counterpart): FP64 gradients of a loss
  * with image gradient dl_dimg (RGBA per pixel) w.r.t. matte albedo cd and
  * emission ce, 3 per material each, accumulated into grad_cd / grad_ce. */

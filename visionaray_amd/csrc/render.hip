@@ -472,16 +472,18 @@ __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 //
 // BLK (threads per block) and LDSD (LDS stack entries per lane) are the
 // autotuned launch shape; A.tile orders pixels in tile x tile squares.
-// waves per SIMD the register allocation must leave room for (0: as many
-// registers as the kernel wants)
+// waves per SIMD the register allocation must leave room for: 5 (96 VGPRs)
+// for the plain kernels; 4 (128 VGPRs) with NEE, whose shadow-ray state would
+// otherwise spill (measured: NEE frame 58.9 -> 54.7 ms on the large scene,
+// the plain frame 36.9 -> 38.5 ms at 4)
 #ifndef VR_PT_WAVES_PER_EU
 #define VR_PT_WAVES_PER_EU 5
 #endif
-#if VR_PT_WAVES_PER_EU > 0
-#define VR_PT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_PT_WAVES_PER_EU)))
-#else
-#define VR_PT_OCCUPANCY
+#ifndef VR_PT_NEE_WAVES_PER_EU
+#define VR_PT_NEE_WAVES_PER_EU 4
 #endif
+#define VR_PT_OCCUPANCY \
+    __attribute__((amdgpu_waves_per_eu((MODE & PT_NEE) ? VR_PT_NEE_WAVES_PER_EU : VR_PT_WAVES_PER_EU)))
 
 // lanes that must wait for shading before the wave stops advancing its queries
 #ifndef VR_PT_REGEN_LANES
