@@ -477,6 +477,32 @@ def ref_sah(tris):
         t.close()
 
 
+def ref_sah_cost(nodes, tris, indices) -> float:
+    """The reference's sah_cost(tree) (detail/bvh/statistics.h:38-68) with its
+    default costs (ci 1.2, cl 0, cp 1)."""
+    t = _Tmp()
+    try:
+        _run(["sahcost", t.put("nodes.bin", np.ascontiguousarray(nodes, NODE_DTYPE)), len(nodes),
+              t.put("tris.bin", tris), tris.size, t.put("idx.bin", np.asarray(indices, np.uint32)),
+              t.path("out.bin")])
+        return float(np.fromfile(t.path("out.bin"), np.float32)[0])
+    finally:
+        t.close()
+
+
+def ref_rotate(nodes, tris, indices):
+    """The reference's bvh_optimizer::optimize_tree_rotations
+    (detail/bvh/optimize.h:22-269): (rotated nodes, rotation count)."""
+    t = _Tmp()
+    try:
+        _run(["rotate", t.put("nodes.bin", np.ascontiguousarray(nodes, NODE_DTYPE)), len(nodes),
+              t.put("tris.bin", tris), tris.size, t.put("idx.bin", np.asarray(indices, np.uint32)),
+              t.path("out.bin"), t.path("count.bin")])
+        return np.fromfile(t.path("out.bin"), NODE_DTYPE), int(np.fromfile(t.path("count.bin"), np.int32)[0])
+    finally:
+        t.close()
+
+
 def ref_lbvh_cpu(tris):
     """The reference's CPU lbvh_builder tree (leaf size 4)."""
     t = _Tmp()

@@ -20,6 +20,8 @@
 //   ptl     NODES NN TRIS NP IDX MATS NM LIGHTS NL CAMIN PARAMS OUT
 //                                       pathtracing::kernel with area lights (NEE + MIS)
 //   sah     TRIS N OUTNODES OUTIDX      binned_sah_builder (no spatial splits)
+//   sahcost NODES NN TRIS NP IDX OUT     sah_cost(tree) (statistics.h:38-68), default costs
+//   rotate  NODES NN TRIS NP IDX OUTNODES OUTCOUNT   bvh_optimizer::optimize_tree_rotations
 //   bench   ...                         CPU baseline timing (see below)
 
 #include <algorithm>
@@ -47,6 +49,9 @@
 #include <visionaray/scheduler.h>
 #include <visionaray/traverse.h>
 #include <visionaray/get_normal.h>
+#include <visionaray/detail/bvh/optimize.h>
+#include <visionaray/detail/bvh/statistics.h>
+#include <visionaray/detail/thread_pool.h>
 
 using namespace visionaray;
 
@@ -485,6 +490,21 @@ int main(int argc, char** argv)
         tiled_sched<basic_ray<float>> sched(p.threads);
         for (int f = 0; f <= p.frame_id; ++f) render_ptl<basic_ray<float>>(sched, tree, mats, lights, cam, p, rt);
         save_rt(a[11], rt);
+        return 0;
+    }
+    if (mode == "sahcost" && na == 6) {
+        auto tree = load_tree(a);
+        float c = sah_cost(tree);
+        save(a[5], &c, 1);
+        return 0;
+    }
+    if (mode == "rotate" && na == 7) {
+        auto tree = load_tree(a);
+        thread_pool pool(1);
+        bvh_optimizer opt;
+        int count = opt.optimize_tree_rotations(tree, pool);
+        save(a[5], tree.nodes().data(), tree.nodes().size());
+        save(a[6], &count, 1);
         return 0;
     }
     if (mode == "lbvhcpu" && na == 4) {
