@@ -194,6 +194,7 @@ def main():
     if not args.no_extras:
         extras["reference_scheme_frame"] = reference_scheme_frame(V, kernel, sp, stream, kernel_ms)
         extras["nee_frame"] = nee_frame(V, S, tris, mats, kernel, sp, stream, kernel_ms)
+        extras["lbvh_rotated"] = rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms)
         extras["heightfield_only"] = heightfield_only(args, V, S, stream)
 
     cpu = None
@@ -275,6 +276,41 @@ def reference_scheme_frame(V, kernel, sp, stream, kernel_ms):
            "scheme": "one thread per pixel in 16x16 blocks, binary-tree traversal (cuda_sched + intersect.inl)"}
     log(f"reference-scheme frame: {ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} incoherent Mrays/s); "
         f"this build x{out['speedup_of_this_build']:.2f}")
+    return out
+
+
+def rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms, builds=3):
+    """The LBVH with the optional rotation pass (VR_BVH_ROTATE): build time,
+    sah_cost against the plain LBVH, and the same frame on it."""
+    import time as _t
+    times, rb = [], None
+    for _ in range(builds):
+        if rb is not None:
+            rb.destroy()
+        stream.synchronize()
+        t0 = _t.perf_counter()
+        rb = V.lbvh_builder().build(d_tris, keep_nodes=False, rotate=True, stream=stream)
+        stream.synchronize()
+        times.append((_t.perf_counter() - t0) * 1e3)
+    sah_plain, sah_rot = bvh.sah_cost(stream=stream), rb.sah_cost(stream=stream)
+    k = V.pathtracing.kernel(rb, kernel.materials, kernel.params)
+    sched = V.hip_sched()
+    sched.frame(k, sp, stream=stream)
+    stream.synchronize()
+    sched.reset_counters()
+    e0, e1 = V.Event(), V.Event()
+    e0.record(stream)
+    sched.frame(k, sp, stream=stream)
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1)
+    sec = sched.rays()[1]
+    rb.destroy()
+    out = {"build_ms_wall": round(float(np.median(times)), 3), "sah_cost_plain": round(sah_plain, 3),
+           "sah_cost_rotated": round(sah_rot, 3), "frame_ms": round(ms, 3),
+           "incoherent_mrays_per_s": round(sec / (ms * 1e-3) / 1e6, 3),
+           "frame_speedup": round(kernel_ms / ms, 4)}
+    log(f"rotated LBVH: build {out['build_ms_wall']:.2f} ms wall, SAH {sah_plain:.2f} -> {sah_rot:.2f}, frame "
+        f"{ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} Mrays/s, x{out['frame_speedup']:.3f})")
     return out
 
 

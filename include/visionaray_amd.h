@@ -132,6 +132,13 @@ typedef struct vr_bvh_t* vr_bvh;
 
 /* vr_lbvh_build flags */
 #define VR_BVH_KEEP_NODES 1u   /* also write the reference bvh_node[2n-1] layout */
+/* also run one top-down pass of SAH-driven tree rotations (Kensler, the
+ * exchanges of detail/bvh/optimize.h:22-262 with up-to-date costs) on the
+ * device before the traversal nodes are made: a lower sah_cost, cheaper
+ * traversal, a longer build (two small device read-backs).  Not with
+ * VR_BVH_KEEP_NODES; such a BVH is rebuilt (vr_lbvh_rebuild, which rotates
+ * again), not refitted; vr_bvh_get_info's depth is then an upper bound. */
+#define VR_BVH_ROTATE 2u
 
 /* ---- runtime ---------------------------------------------------------- */
 /* Select device `device` for this thread and create its state. */
@@ -214,6 +221,21 @@ int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, 
                        size_t* num_nodes, uint32_t* h_indices);
 
 /* ---- traversal -------------------------------------------------------- */
+/* ---- tree quality ------------------------------------------------------
+ * sah_cost(tree, ci, cl, cp) (detail/bvh/statistics.h:38-68) over a
+ * reference-layout node array on the host: the float sums in node order, as
+ * the reference computes them. */
+int vr_bvh_sah_cost_host(const vr_bvh_node* h_nodes, size_t num_nodes, float ci, float cl, float cp, float* cost);
+/* The same SAH cost of a BVH handle, evaluated on the device over its binary
+ * nodes (FP64 sums; agrees with the float sums to their rounding).
+ * Synchronizes the stream. */
+int vr_bvh_sah_cost(vr_bvh bvh, float ci, float cl, float cp, double* cost, void* stream);
+/* bvh_optimizer::optimize_tree_rotations (detail/bvh/optimize.h:22-262,
+ * Kensler tree rotations) on a reference-layout node array on the host, in
+ * place; *num_rotations = rotations performed.  The reference's result, node
+ * for node. */
+int vr_bvh_optimize_rotations_host(vr_bvh_node* h_nodes, size_t num_nodes, int* num_rotations);
+
 /* closest_hit / any_hit over one BVH (detail/traverse_linear.inl:149-214 ->
  * intersect<ClosestHit|AnyHit>, detail/bvh/intersect.inl:41-357). */
 int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);

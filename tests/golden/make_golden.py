@@ -90,6 +90,22 @@ def main():
     np.savez_compressed(OUT / "images.npz", cam_in=cam_in, ao_params=ao_p, ao=ao, mats=mats, tris_m=tris_m,
                         pt_params=pt_p, pt=pt)
     nee()
+    quality()
+
+
+def quality():
+    """sah_cost and bvh_optimizer::optimize_tree_rotations of the reference
+    over the GPU-layout LBVH of the small config-0 scene, and sah_cost of the
+    reference's binned-SAH tree."""
+    tris = S.config0_scene(level=3)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    rot, count = O.ref_rotate(nodes, tris, idx)
+    sah_nodes, sah_idx = O.ref_sah(tris)
+    np.savez_compressed(OUT / "quality.npz", tris=tris, lbvh_nodes=nodes, lbvh_indices=idx,
+                        lbvh_sah=np.float32(O.ref_sah_cost(nodes, tris, idx)), rotated=rot,
+                        rotations=np.int32(count), rotated_sah=np.float32(O.ref_sah_cost(rot, tris, idx)),
+                        sah_nodes=sah_nodes, sah_indices=sah_idx,
+                        sah_tree_sah=np.float32(O.ref_sah_cost(sah_nodes, tris, sah_idx)))
 
 
 def nee():

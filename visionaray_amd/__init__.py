@@ -28,7 +28,7 @@ from .scenes import LIGHT_DTYPE, MAT_DTYPE, TRI_DTYPE
 
 __all__ = [
     "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build",
-    "closest_hit", "any_hit", "trace", "pinhole_camera", "make_kernel_params", "make_sched_params",
+    "closest_hit", "any_hit", "trace", "sah_cost_host", "optimize_tree_rotations", "pinhole_camera", "make_kernel_params", "make_sched_params",
     "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
     "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "LIGHT_DTYPE", "synchronize", "model",
 ]
@@ -185,6 +185,13 @@ class BVH:
         check(lib().vr_bvh_sorted_refs(self.handle, c.ptr, i.ptr, None))
         return c.to_host(), i.to_host()
 
+    def sah_cost(self, ci: float = 1.2, cl: float = 0.0, cp: float = 1.0, stream=None) -> float:
+        """sah_cost(tree, ci, cl, cp) (detail/bvh/statistics.h:38-68), on the
+        device over the binary nodes (FP64 sums)."""
+        c = C.c_double()
+        check(lib().vr_bvh_sah_cost(self.handle, ci, cl, cp, C.byref(c), _stream(stream)))
+        return float(c.value)
+
     def rebuild(self, prims, stream=None):
         """vr_lbvh_rebuild: a new LBVH of `prims` into this handle, reusing its
         device arrays when the primitive count is unchanged."""
@@ -213,11 +220,14 @@ class BVH:
 class lbvh_builder:
     """GPU LBVH builder (lbvh_builder::build, detail/bvh/lbvh.h:681-844)."""
 
-    def build(self, prims, n: int | None = None, keep_nodes: bool = True, stream=None) -> BVH:
+    def build(self, prims, n: int | None = None, keep_nodes: bool = True, stream=None, rotate: bool = False) -> BVH:
+        """rotate: SAH-driven tree rotations before the traversal nodes are
+        made (VR_BVH_ROTATE; implies keep_nodes=False)."""
         p = _as_device(prims, TRI_DTYPE)
         n = p.count if n is None else int(n)
         h = C.c_void_p()
-        check(lib().vr_lbvh_build(p.ptr, n, VR_BVH_KEEP_NODES if keep_nodes else 0, _stream(stream), C.byref(h)))
+        flags = (VR_BVH_KEEP_NODES if keep_nodes and not rotate else 0) | (_lib.VR_BVH_ROTATE if rotate else 0)
+        check(lib().vr_lbvh_build(p.ptr, n, flags, _stream(stream), C.byref(h)))
         return BVH(h, p)
 
     @staticmethod
@@ -250,6 +260,23 @@ def sah_build(prims: np.ndarray, max_leaf_size: int = 4):
     nn = C.c_size_t()
     check(lib().vr_sah_build(ptr(prims), n, max_leaf_size, ptr(nodes), C.byref(nn), ptr(idx)))
     return nodes[: nn.value].copy(), idx[:n].copy()
+
+
+def sah_cost_host(nodes: np.ndarray, ci: float = 1.2, cl: float = 0.0, cp: float = 1.0) -> float:
+    """sah_cost over a reference-layout node array, summed as the reference does."""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    c = C.c_float()
+    check(lib().vr_bvh_sah_cost_host(ptr(nodes), nodes.size, ci, cl, cp, C.byref(c)))
+    return float(c.value)
+
+
+def optimize_tree_rotations(nodes: np.ndarray):
+    """bvh_optimizer::optimize_tree_rotations (detail/bvh/optimize.h:22-262) on a
+    copy of a reference-layout node array: (rotated nodes, rotations)."""
+    out = np.array(nodes, NODE_DTYPE, copy=True)
+    n = C.c_int()
+    check(lib().vr_bvh_optimize_rotations_host(ptr(out), out.size, C.byref(n)))
+    return out, int(n.value)
 
 
 def lbvh_build_host(prims: np.ndarray, max_leaf_size: int = 4):

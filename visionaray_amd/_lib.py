@@ -19,6 +19,7 @@ VR_ERR_HIP = 3
 VR_ERR_NOT_INITIALIZED = 4
 VR_ERR_UNSUPPORTED = 5
 VR_BVH_KEEP_NODES = 1
+VR_BVH_ROTATE = 2
 
 # every symbol the C header declares: name -> (restype, argtypes)
 P = C.c_void_p
@@ -86,6 +87,9 @@ SIGNATURES = {
     "vr_destroy_batch": (I, [P]),
     "vr_last_error_string": (C.c_char_p, []),
     "vr_render_pt_lights": (I, [P, P, SZ, P, SZ, P, P, P, P, P, P, I, P]),
+    "vr_bvh_sah_cost_host": (I, [P, SZ, F, F, F, C.POINTER(F)]),
+    "vr_bvh_sah_cost": (I, [P, F, F, F, C.POINTER(C.c_double), P]),
+    "vr_bvh_optimize_rotations_host": (I, [P, SZ, C.POINTER(I)]),
     "vr_mesh_load": (I, [C.POINTER(C.c_char_p), SZ, C.POINTER(P)]),
     "vr_mesh_get_info": (I, [P, P]),
     "vr_mesh_triangles": (I, [P, P]),

@@ -13,7 +13,8 @@
 
 namespace vr {
 
-int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, hipStream_t st);
+int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, bool rotate,
+                    hipStream_t st);
 int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st);
 int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st);
 int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
@@ -289,7 +290,8 @@ int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* st
     if (rc) return rc;
     vr_bvh h = new vr_bvh_t;
     h->s.device = current_device();
-    rc = lbvh_build_into(h->s, d_prims, n, (flags & VR_BVH_KEEP_NODES) != 0, (hipStream_t)stream);
+    rc = lbvh_build_into(h->s, d_prims, n, (flags & VR_BVH_KEEP_NODES) != 0, (flags & VR_BVH_ROTATE) != 0,
+                         (hipStream_t)stream);
     if (rc) {
         vr_bvh_destroy(h);
         return rc;
@@ -307,7 +309,7 @@ int vr_lbvh_rebuild(vr_bvh bvh, const vr_triangle* d_prims, size_t n, void* stre
     VR_CHECK(n == 0 || d_prims, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_rebuild: null primitives");
     int rc = ensure_device();
     if (rc) return rc;
-    return lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, (hipStream_t)stream);
+    return lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, bvh->s.want_rotate, (hipStream_t)stream);
     VR_TRY_END
 }
 
@@ -526,6 +528,33 @@ int vr_lbvh_build_host(const vr_triangle* h_prims, size_t n, int max_leaf_size, 
     VR_CHECK(num_nodes && (n == 0 || (h_prims && h_nodes && h_indices)), VR_ERR_INVALID_ARGUMENT,
              "vr_lbvh_build_host: null argument");
     return lbvh_build_host(h_prims, n, max_leaf_size, h_nodes, num_nodes, h_indices);
+    VR_TRY_END
+}
+
+int vr_bvh_sah_cost_host(const vr_bvh_node* h_nodes, size_t num_nodes, float ci, float cl, float cp, float* cost)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(h_nodes && cost && num_nodes > 0, VR_ERR_INVALID_ARGUMENT, "vr_bvh_sah_cost_host: null argument");
+    return sah_cost_host(h_nodes, num_nodes, ci, cl, cp, cost);
+    VR_TRY_END
+}
+
+int vr_bvh_sah_cost(vr_bvh bvh, float ci, float cl, float cp, double* cost, void* stream)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(bvh && cost, VR_ERR_INVALID_ARGUMENT, "vr_bvh_sah_cost: null argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return sah_cost_device(bvh->s, ci, cl, cp, cost, (hipStream_t)stream);
+    VR_TRY_END
+}
+
+int vr_bvh_optimize_rotations_host(vr_bvh_node* h_nodes, size_t num_nodes, int* num_rotations)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(h_nodes && num_rotations && num_nodes > 0, VR_ERR_INVALID_ARGUMENT,
+             "vr_bvh_optimize_rotations_host: null argument");
+    return optimize_rotations_host(h_nodes, num_nodes, num_rotations);
     VR_TRY_END
 }
 

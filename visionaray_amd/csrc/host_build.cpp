@@ -323,4 +323,201 @@ int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bv
     return VR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// tree statistics and rotations over the reference node layout
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline box node_box(const vr_bvh_node& n)
+{
+    box b;
+    b.lo = f3{n.lo[0], n.lo[1], n.lo[2]};
+    b.hi = f3{n.hi[0], n.hi[1], n.hi[2]};
+    return b;
+}
+
+inline void set_box(vr_bvh_node& n, const box& b)
+{
+    n.lo[0] = b.lo.x, n.lo[1] = b.lo.y, n.lo[2] = b.lo.z;
+    n.hi[0] = b.hi.x, n.hi[1] = b.hi.y, n.hi[2] = b.hi.z;
+}
+
+// surface_area (math/detail/aabb.inl:199-224): 2 (s.x s.y + s.y s.z + s.z s.x), s = max - min
+inline float surface_area(const box& b)
+{
+    const f3 s = b.size();
+    return 2.0f * (s.x * s.y + s.y * s.z + s.z * s.x);
+}
+
+inline box joined(box a, const box& b)
+{
+    a.insert(b);
+    return a;
+}
+
+}  // namespace
+
+// sah_cost(tree, ci, cl, cp) (detail/bvh/statistics.h:38-68): float sums in
+// node order
+int sah_cost_host(const vr_bvh_node* nodes, size_t n, float ci, float cl, float cp, float* out)
+{
+    VR_CHECK(n > 0, VR_ERR_INVALID_ARGUMENT, "sah_cost: empty tree");
+    const float a_r = surface_area(node_box(nodes[0]));
+    float a_l = 0.0f, a_i = 0.0f, a_l_n = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        const float sa = surface_area(node_box(nodes[i]));
+        if (nodes[i].num_prims) {
+            a_l += sa;
+            a_l_n += sa * (float)nodes[i].num_prims;
+        } else {
+            a_i += sa;
+        }
+    }
+    *out = ci * (a_i / a_r) + cl * (a_l / a_r) + cp * (a_l_n / a_r);
+    return VR_OK;
+}
+
+// bvh_optimizer::optimize_tree_rotations (detail/bvh/optimize.h:22-262;
+// Kensler, "Tree rotations for improving bounding volume hierarchies"):
+// per-node costs C_t + (SA_0 C_0 + SA_1 C_1) / SA (leaves C_t + C_i n),
+// assigned bottom up by sweeps over the node array; then one depth-first pass
+// from the root that, at every inner node, tries the four exchanges of a
+// grandchild with its uncle and applies the cheapest one cheaper than the
+// node's cost.  As in the reference, the cost table is not updated after an
+// exchange (entries stay with array positions while node records move), and
+// the two affected boxes are recomputed.  A tree whose root is a leaf is left
+// as it is (the reference's cost sweep would not end); the depth-first stack
+// grows as needed (the reference's holds 64 entries).
+int optimize_rotations_host(vr_bvh_node* nodes, size_t n, int* count)
+{
+    VR_CHECK(n > 0, VR_ERR_INVALID_ARGUMENT, "optimize_rotations: empty tree");
+    *count = 0;
+    if (nodes[0].num_prims) return VR_OK;
+    const float C_t = 1.0f, C_i = 1.0f;
+    auto child = [&](uint32_t i, int k) -> uint32_t { return nodes[i].first + (uint32_t)k; };
+    auto inner = [&](uint32_t i) { return nodes[i].num_prims == 0; };
+    for (size_t i = 0; i < n; ++i)
+        if (inner((uint32_t)i)) VR_CHECK((size_t)nodes[i].first + 1 < n, VR_ERR_INVALID_ARGUMENT,
+                                         "optimize_rotations: child index out of range");
+    std::vector<float> costs(n, FLT_MAX);
+    for (bool root_done = false; !root_done;) {
+        bool progress = false;
+        for (size_t i = 0; i < n; ++i) {
+            if (costs[i] < FLT_MAX) continue;
+            const vr_bvh_node& nd = nodes[i];
+            if (nd.num_prims) {
+                costs[i] = C_t + C_i * (float)nd.num_prims;
+                progress = true;
+            } else {
+                const uint32_t c0 = nd.first, c1 = nd.first + 1;
+                if (costs[c0] < FLT_MAX && costs[c1] < FLT_MAX) {
+                    const float sa0 = surface_area(node_box(nodes[c0]));
+                    const float sa1 = surface_area(node_box(nodes[c1]));
+                    costs[i] = C_t + (sa0 * costs[c0] + sa1 * costs[c1]) / surface_area(node_box(nd));
+                    progress = true;
+                    if (i == 0) root_done = true;
+                }
+            }
+        }
+        VR_CHECK(progress || root_done, VR_ERR_INVALID_ARGUMENT, "optimize_rotations: not a tree (cycle)");
+    }
+    std::vector<uint32_t> st{0u};
+    uint32_t addr = 0;
+    int rotations = 0;
+    while (!st.empty()) {
+        const vr_bvh_node node = nodes[addr];
+        if (!node.num_prims) {
+            const float C = costs[addr];
+            enum { R00, R01, R11, R10 };
+            float c[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+            const uint32_t k0 = child(addr, 0), k1 = child(addr, 1);
+            if (inner(k0)) {
+                {  // 0.0 <-> 1
+                    const box box00 = node_box(nodes[k1]);
+                    const box box01 = node_box(nodes[child(k0, 1)]);
+                    const box box0 = joined(box00, box01);
+                    const box box1 = node_box(nodes[child(k0, 0)]);
+                    const box bx = joined(box0, box1);
+                    const float c00 = costs[k1], c01 = costs[child(k0, 1)];
+                    const float cc0 = C_t + (surface_area(box00) * c00 + surface_area(box01) * c01) / surface_area(box0);
+                    const float cc1 = costs[child(k0, 0)];
+                    c[R00] = C_t + (surface_area(box0) * cc0 + surface_area(box1) * cc1) / surface_area(bx);
+                }
+                {  // 0.1 <-> 1
+                    const box box00 = node_box(nodes[child(k0, 0)]);
+                    const box box01 = node_box(nodes[k1]);
+                    const box box0 = joined(box00, box01);
+                    const box box1 = node_box(nodes[child(k0, 1)]);
+                    const box bx = joined(box0, box1);
+                    const float c00 = costs[child(k0, 0)], c01 = costs[k1];
+                    const float cc0 = C_t + (surface_area(box00) * c00 + surface_area(box01) * c01) / surface_area(box0);
+                    const float cc1 = costs[child(k0, 1)];
+                    c[R01] = C_t + (surface_area(box0) * cc0 + surface_area(box1) * cc1) / surface_area(bx);
+                }
+            }
+            if (inner(k1)) {
+                {  // 1.1 <-> 0
+                    const box box10 = node_box(nodes[child(k1, 0)]);
+                    const box box11 = node_box(nodes[k0]);
+                    const box box1 = joined(box10, box11);
+                    const box box0 = node_box(nodes[child(k1, 1)]);
+                    const box bx = joined(box0, box1);
+                    const float c10 = costs[child(k1, 0)], c11 = costs[k0];
+                    const float cc0 = costs[child(k1, 1)];
+                    const float cc1 = C_t + (surface_area(box10) * c10 + surface_area(box11) * c11) / surface_area(box1);
+                    c[R11] = C_t + (surface_area(box0) * cc0 + surface_area(box1) * cc1) / surface_area(bx);
+                }
+                {  // 1.0 <-> 0
+                    const box box10 = node_box(nodes[k0]);
+                    const box box11 = node_box(nodes[child(k1, 1)]);
+                    const box box1 = joined(box10, box11);
+                    const box box0 = node_box(nodes[child(k1, 0)]);
+                    const box bx = joined(box0, box1);
+                    const float c10 = costs[k0], c11 = costs[child(k1, 1)];
+                    const float cc0 = costs[child(k1, 0)];
+                    const float cc1 = C_t + (surface_area(box10) * c10 + surface_area(box11) * c11) / surface_area(box1);
+                    c[R10] = C_t + (surface_area(box0) * cc0 + surface_area(box1) * cc1) / surface_area(bx);
+                }
+            }
+            int rotation = -1;
+            float c_new = C;
+            for (int i = 0; i < 4; ++i) {
+                if (c[i] < c_new) {
+                    rotation = i;
+                    c_new = c[i];
+                }
+            }
+            uint32_t n0 = ~0u, n1 = ~0u;
+            switch (rotation) {
+            case R00: n0 = child(k0, 0); n1 = k1; break;
+            case R01: n0 = child(k0, 1); n1 = k1; break;
+            case R11: n0 = k0; n1 = child(k1, 1); break;
+            case R10: n0 = k0; n1 = child(k1, 0); break;
+            default: break;
+            }
+            if (n0 != ~0u && n1 != ~0u) {
+                std::swap(nodes[n0], nodes[n1]);
+                if (rotation == R00 || rotation == R01) {
+                    const box b0 = joined(node_box(nodes[child(k0, 0)]), node_box(nodes[child(k0, 1)]));
+                    set_box(nodes[k0], b0);
+                    set_box(nodes[addr], joined(b0, node_box(nodes[k1])));
+                } else {
+                    const box b1 = joined(node_box(nodes[child(k1, 0)]), node_box(nodes[child(k1, 1)]));
+                    set_box(nodes[k1], b1);
+                    set_box(nodes[addr], joined(node_box(nodes[k0]), b1));
+                }
+                ++rotations;
+            }
+            addr = child(addr, 0);
+            st.push_back(k1);
+        } else {
+            addr = st.back();
+            st.pop_back();
+        }
+    }
+    *count = rotations;
+    return VR_OK;
+}
+
 }  // namespace vr

@@ -485,6 +485,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
         if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
         store_box(&s_box[k * 12 + side * 6], lo, hi);
         s_h[2 * k + side] = h;
+        heights[2 * p + side] = h;  // every child slot's height (the rotation pass reads them)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         if (atomicAdd(&s_flag[k], 1u) == 0u) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -703,9 +704,12 @@ void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* 
 }  // namespace
 
 // Build (or rebuild in place) the LBVH of `bs` from n primitives.
-int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, hipStream_t st)
+int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool keep_nodes, bool rotate,
+                    hipStream_t st)
 {
     VR_CHECK(n <= (size_t)MAX_LEAF_FIRST, VR_ERR_UNSUPPORTED, "lbvh: more than 2^26 - 1 primitives");
+    VR_CHECK(!(keep_nodes && rotate), VR_ERR_UNSUPPORTED,
+             "lbvh: the reference node layout is not kept for a rotated tree");
     const size_t npairs = n > 1 ? n - 1 : 0;
     // (re)allocate owned arrays when the size changes
     if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
@@ -719,6 +723,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         bs.num_prims = n;
         bs.num_pairs = npairs;
         bs.is_lbvh = true;
+        bs.want_rotate = rotate;
         if (n == 0) return VR_OK;
         VR_HIP_CHECK(hipMalloc(&bs.pairs, std::max<size_t>(npairs, 1) * sizeof(pair_node)));
         VR_HIP_CHECK(hipMalloc(&bs.tris, n * sizeof(tri_node)));
@@ -788,6 +793,15 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     // (vr_bvh_get_info reads it); 65 bounds every Karras tree, which the
     // traversal stack covers, so the build never waits on the device.
     bs.depth = n <= 1 ? (uint32_t)n : 65u;
+    bs.rotated = false;
+    bs.want_rotate = rotate;
+    if (rotate && n > 1) {
+        // optional: SAH-driven tree rotations before the collapse
+        uint32_t count = 0;
+        rc = lbvh_rotate(bs, LBVH_LEAF_MAX, &count, st);
+        if (rc) return rc;
+        bs.rotated = count > 0;
+    }
     return collapse_quads(bs, LBVH_LEAF_MAX, st);
 }
 
@@ -796,6 +810,7 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
 {
     const size_t n = bs.num_prims;
     if (n == 0) return VR_OK;
+    VR_CHECK(!bs.rotated, VR_ERR_UNSUPPORTED, "refit: a rotated LBVH is rebuilt (vr_lbvh_rebuild), not refitted");
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
     VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
     // the hierarchy (child entries, parent links, ranges, leaf parents) is

@@ -45,6 +45,15 @@ int fail(int code, const std::string& msg);
 // create the current device's library state on first use
 int ensure_device();
 
+struct bvh_state;
+// sah_cost of a BVH over its binary nodes (FP64 sums); synchronizes st
+int sah_cost_device(const bvh_state& s, float ci, float cl, float cp, double* out, hipStream_t st);
+// reference-layout node arrays on the host: sah_cost and tree rotations
+int sah_cost_host(const vr_bvh_node* nodes, size_t n, float ci, float cl, float cp, float* out);
+int optimize_rotations_host(vr_bvh_node* nodes, size_t n, int* count);
+// SAH-driven tree rotations of a freshly built LBVH (before its collapse)
+int lbvh_rotate(bvh_state& s, uint32_t leaf_max, uint32_t* rotations, hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // device layouts
 // ---------------------------------------------------------------------------
@@ -133,6 +142,8 @@ struct bvh_state {
     uint32_t* indices = nullptr;        // tree.indices()
     size_t num_ref_nodes = 0;
     float* root_box = nullptr;          // 6 floats
+    bool rotated = false;               // LBVH after tree rotations (no refit, no reference layout)
+    bool want_rotate = false;           // LBVH builds of this handle run the rotation pass
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
