@@ -123,6 +123,15 @@ int vr::ensure_device()
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!ds->ready) {
         VR_HIP_CHECK(hipMalloc(&ds->work, (size_t)WORK_SLOTS * 128));
+        // LBVH arrays come from the device's stream-ordered pool; freed blocks
+        // stay in the pool, so the next build of a similar size allocates
+        // without a driver call
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        VR_HIP_CHECK(hipGetDevice(&dev));
+        VR_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+        uint64_t keep = UINT64_MAX;
+        VR_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
         ds->ready = true;
     }
     return VR_OK;
@@ -505,8 +514,20 @@ int vr_bvh_destroy(vr_bvh bvh)
     bvh_state& s = bvh->s;
     void* ptrs[] = {s.pairs, s.quads, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents,
                     s.flags, s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+    if (s.pooled) {
+        // back to the pool once the device is done with them: the caller may
+        // still have work on the BVH in flight on any stream
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(s.device);
+        (void)hipDeviceSynchronize();
+        for (void* p : ptrs)
+            if (p) (void)hipFreeAsync(p, nullptr);
+        (void)hipSetDevice(cur);
+    } else {
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
     delete bvh;
     return VR_OK;
 }

@@ -22,8 +22,9 @@ namespace {
 constexpr int BLOCK = 256;
 
 struct child_ref {
-    uint32_t e;  // pair entry
-    uint32_t n;  // LBVH: triangles below (leaf_max > 0)
+    uint32_t e;      // pair entry
+    uint32_t n;      // LBVH: triangles below (leaf_max > 0)
+    uint32_t first;  // LBVH: first triangle of an inner child's range
     f3 lo, hi;
 };
 
@@ -33,15 +34,16 @@ __device__ inline float half_area(f3 lo, f3 hi)
     return dx * dy + dy * dz + dz * dx;
 }
 
-// triangles under an LBVH pair: its range [min(p, j), max(p, j)], j in e[3]
-__device__ inline uint32_t lbvh_count(const pair_node* __restrict__ pairs, uint32_t p)
-{
-    const uint32_t j = pairs[p].e[3];
-    return max(p, j) - min(p, j) + 1u;
-}
-
-// the two children of pair p (their entries, boxes, box locations and, for an
-// LBVH, triangle counts)
+// the two children of pair p: entries and boxes, and for an LBVH their
+// triangle counts and range starts.  A Karras pair p covers [first, last] =
+// [min(p, j), max(p, j)], j in e[3]; its left child (inner: pair `split`)
+// covers [first, split] and its right child (inner: pair split + 1) covers
+// [split + 1, last], so an unchanged Karras tree (RANGES) gives both counts
+// from p's own record.  After tree rotations the ranges only hold for the
+// untouched subtrees of at most leaf_max triangles, and the counts are read
+// from each child's record: a changed pair keeps a count above leaf_max
+// either way, which is all the collapse asks of it.
+template <bool RANGES>
 __device__ inline void load_pair(const pair_node* __restrict__ pairs, uint32_t num_pairs, uint32_t p,
                                  bool lbvh, child_ref& a, child_ref& b)
 {
@@ -56,12 +58,36 @@ __device__ inline void load_pair(const pair_node* __restrict__ pairs, uint32_t n
     b.hi = mk3(q2.y, q2.z, q2.w);
     a.n = 1u;
     b.n = 1u;
-    if (lbvh) {
-        if (!(a.e & LEAF_BIT) && a.e < num_pairs) a.n = lbvh_count(pairs, a.e);
-        if (!(b.e & LEAF_BIT) && b.e < num_pairs) b.n = lbvh_count(pairs, b.e);
+    a.first = 0u;
+    b.first = 0u;
+    if (!lbvh) return;
+    const bool ai = !(a.e & LEAF_BIT) && a.e < num_pairs;
+    const bool bi = !(b.e & LEAF_BIT) && b.e < num_pairs;
+    if constexpr (RANGES) {
+        const uint32_t first = min(p, e.w), last = max(p, e.w);
+        if (ai) {
+            a.n = a.e - first + 1u;
+            a.first = first;
+        }
+        if (bi) {
+            b.n = last - b.e + 1u;
+            b.first = b.e;
+        }
+    } else {
+        if (ai) {
+            const uint32_t j = pairs[a.e].e[3];
+            a.n = max(a.e, j) - min(a.e, j) + 1u;
+            a.first = min(a.e, j);
+        }
+        if (bi) {
+            const uint32_t j = pairs[b.e].e[3];
+            b.n = max(b.e, j) - min(b.e, j) + 1u;
+            b.first = min(b.e, j);
+        }
     }
 }
 
+template <bool RANGES>
 __global__ __launch_bounds__(BLOCK) void k_collapse(const pair_node* __restrict__ pairs, uint32_t num_pairs,
                                                     uint32_t leaf_max, quad_node* __restrict__ quads)
 {
@@ -74,14 +100,22 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(const pair_node* __restrict_
         // a subtree of at most leaf_max triangles is always a leaf of its
         // parent's quad, never a quad itself (the root pair is kept: a tree
         // that small still starts at quad 0)
-        if (lbvh && p != 0 && lbvh_count(pairs, p) <= leaf_max) continue;
+        if (lbvh && p != 0) {
+            const uint32_t j = pairs[p].e[3];
+            if (max(p, j) - min(p, j) + 1u <= leaf_max) continue;
+        }
+        // every index into c[] is a compile-time constant once the loops
+        // are unrolled, so the four children stay in registers
         child_ref c[4];
-        load_pair(pairs, num_pairs, p, lbvh, c[0], c[1]);
+        load_pair<RANGES>(pairs, num_pairs, p, lbvh, c[0], c[1]);
         int nc = 2;
-        while (nc < 4) {
+#pragma unroll
+        for (int step = 0; step < 2; ++step) {
+            if (nc != 2 + step) break;
             int best = -1;
             float best_a = -1.0f;
-            for (int k = 0; k < nc; ++k) {
+#pragma unroll
+            for (int k = 0; k < 2 + step; ++k) {
                 if (!openable(c[k])) continue;
                 const float a = half_area(c[k].lo, c[k].hi);
                 if (a > best_a) {
@@ -90,29 +124,37 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(const pair_node* __restrict_
                 }
             }
             if (best < 0) break;
+            uint32_t eb = c[0].e;
+#pragma unroll
+            for (int k = 1; k < 2 + step; ++k)
+                if (k == best) eb = c[k].e;
             child_ref a, b;
-            load_pair(pairs, num_pairs, c[best].e, lbvh, a, b);
-            c[best] = a;
-            c[nc] = b;
+            load_pair<RANGES>(pairs, num_pairs, eb, lbvh, a, b);
+#pragma unroll
+            for (int k = 0; k < 2 + step; ++k)
+                if (k == best) c[k] = a;
+            c[2 + step] = b;
             ++nc;
         }
         quad_node qn;
         const float inf = __builtin_inff();
         // unused slots: lo = hi = +inf, a box the slab test never enters
-        float lx[4] = {inf, inf, inf, inf}, ly[4] = {inf, inf, inf, inf}, lz[4] = {inf, inf, inf, inf};
-        float hx[4] = {inf, inf, inf, inf}, hy[4] = {inf, inf, inf, inf}, hz[4] = {inf, inf, inf, inf};
-        uint32_t ce[4] = {LEAF_BIT, LEAF_BIT, LEAF_BIT, LEAF_BIT};
-        for (int k = 0; k < nc; ++k) {
-            lx[k] = c[k].lo.x; ly[k] = c[k].lo.y; lz[k] = c[k].lo.z;
-            hx[k] = c[k].hi.x; hy[k] = c[k].hi.y; hz[k] = c[k].hi.z;
+        float lx[4], ly[4], lz[4], hx[4], hy[4], hz[4];
+        uint32_t ce[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < nc;
+            lx[k] = used ? c[k].lo.x : inf;
+            ly[k] = used ? c[k].lo.y : inf;
+            lz[k] = used ? c[k].lo.z : inf;
+            hx[k] = used ? c[k].hi.x : inf;
+            hy[k] = used ? c[k].hi.y : inf;
+            hz[k] = used ? c[k].hi.z : inf;
             // leaves as they are, small LBVH subtrees as leaf ranges, other
             // inner children as the quad of their own pair
             uint32_t e = c[k].e;
-            if (!(e & LEAF_BIT) && lbvh && e < num_pairs && c[k].n <= leaf_max) {
-                const uint32_t j = pairs[e].e[3];
-                e = leaf_entry(min(e, j), c[k].n);
-            }
-            ce[k] = e;
+            if (!(e & LEAF_BIT) && lbvh && e < num_pairs && c[k].n <= leaf_max) e = leaf_entry(c[k].first, c[k].n);
+            ce[k] = used ? e : LEAF_BIT;
         }
         qn.lo_x = make_float4(lx[0], lx[1], lx[2], lx[3]);
         qn.lo_y = make_float4(ly[0], ly[1], ly[2], ly[3]);
@@ -136,7 +178,12 @@ int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st)
     if (!s.quads) VR_HIP_CHECK(hipMalloc(&s.quads, np * sizeof(quad_node)));
     s.collapse_leaf_max = leaf_max;
     const unsigned grid = std::max(1u, std::min(div_up(np, BLOCK), 16384u));
-    hipLaunchKernelGGL(k_collapse, dim3(grid), dim3(BLOCK), 0, st, s.pairs, (uint32_t)np, leaf_max, s.quads);
+    // an LBVH's own Karras ranges give every count, unless rotations moved them
+    if (leaf_max > 0 && s.is_lbvh && !s.rotated)
+        hipLaunchKernelGGL(k_collapse<true>, dim3(grid), dim3(BLOCK), 0, st, s.pairs, (uint32_t)np, leaf_max, s.quads);
+    else
+        hipLaunchKernelGGL(k_collapse<false>, dim3(grid), dim3(BLOCK), 0, st, s.pairs, (uint32_t)np, leaf_max,
+                           s.quads);
     VR_HIP_CHECK(hipGetLastError());
     s.root = 0u;
     return VR_OK;

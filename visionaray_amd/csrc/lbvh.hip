@@ -60,9 +60,8 @@ struct tri_in {
     uint32_t geom_id, prim_id;
     f3 v1, e1, e2;
 };
-__device__ inline tri_in load_tri(const vr_triangle* __restrict__ p, size_t i)
+__device__ inline tri_in tri_from_words(const float* q)
 {
-    const float* q = reinterpret_cast<const float*>(p + i);
     tri_in t;
     t.geom_id = __float_as_uint(q[0]);
     t.prim_id = __float_as_uint(q[1]);
@@ -70,6 +69,10 @@ __device__ inline tri_in load_tri(const vr_triangle* __restrict__ p, size_t i)
     t.e1 = mk3(q[5], q[6], q[7]);
     t.e2 = mk3(q[8], q[9], q[10]);
     return t;
+}
+__device__ inline tri_in load_tri(const vr_triangle* __restrict__ p, size_t i)
+{
+    return tri_from_words(reinterpret_cast<const float*>(p + i));
 }
 
 // get_bounds(triangle) (math/detail/triangle.inl:41-53)
@@ -91,22 +94,19 @@ __device__ inline f3 tri_centroid(const tri_in& t)
     return muls3(add3(hi, lo), 0.5f);
 }
 
-__global__ void k_init_bounds(uint32_t* cb)
-{
-    if (threadIdx.x < 3) cb[threadIdx.x] = f2o(FLT_MAX);
-    else if (threadIdx.x < 6) cb[threadIdx.x] = f2o(-FLT_MAX);
-}
+// The primitive copy (index_bvh_t(P*, n), bvh.h:891-896) and the centroid
+// box in one pass over the triangles: a block stages CB_BLOCK triangles
+// (11 words each) in LDS with 16-byte loads when the source is 16-byte
+// aligned, writes them to the copy with 16-byte stores and reduces their
+// centroids; every block leaves its six extremes in `part` (no contended
+// atomics), and k_bounds_final folds them.  min / max are exact, so any
+// reduction order gives the same box.
+constexpr int CB_BLOCK = 256;
+constexpr int CB_GRID = 2048;
+constexpr int TRI_WORDS = 11;
 
-__global__ __launch_bounds__(BLOCK) void k_centroid_bounds(const vr_triangle* __restrict__ prims, size_t n,
-                                                           uint32_t* __restrict__ cb)
+__device__ inline void block_minmax6(float v[6], float* __restrict__ out)
 {
-    float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        f3 c = tri_centroid(load_tri(prims, i));
-        v[0] = fminf(v[0], c.x); v[1] = fminf(v[1], c.y); v[2] = fminf(v[2], c.z);
-        v[3] = fmaxf(v[3], c.x); v[4] = fmaxf(v[4], c.y); v[5] = fmaxf(v[5], c.z);
-    }
-    // min/max are exact, so any reduction order gives the same values
     for (int off = 32; off >= 1; off >>= 1) {
         for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], __shfl_xor(v[k], off));
         for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], __shfl_xor(v[k], off));
@@ -120,9 +120,60 @@ __global__ __launch_bounds__(BLOCK) void k_centroid_bounds(const vr_triangle* __
         const int k = threadIdx.x;
         float r = s[0][k];
         for (int q = 1; q < WAVES; ++q) r = k < 3 ? fminf(r, s[q][k]) : fmaxf(r, s[q][k]);
-        if (k < 3) atomicMin(&cb[k], f2o(r));
-        else atomicMax(&cb[k], f2o(r));
+        out[k] = r;
     }
+}
+
+__global__ __launch_bounds__(CB_BLOCK) void k_copy_bounds(const vr_triangle* __restrict__ src, size_t n,
+                                                          vr_triangle* __restrict__ dst, int vec,
+                                                          float* __restrict__ part)
+{
+    __shared__ float s_tri[CB_BLOCK * TRI_WORDS];
+    float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+    const size_t chunks = (n + CB_BLOCK - 1) / CB_BLOCK;
+    for (size_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const size_t t0 = c * CB_BLOCK;
+        const uint32_t cnt = (uint32_t)min((size_t)CB_BLOCK, n - t0);
+        const float* sp = reinterpret_cast<const float*>(src + t0);
+        float* dp = dst ? reinterpret_cast<float*>(dst + t0) : nullptr;
+        if (vec && cnt == CB_BLOCK) {
+            // 256 triangles = 704 16-byte words; the chunk starts on 16 bytes
+            for (uint32_t q = threadIdx.x; q < CB_BLOCK * TRI_WORDS / 4; q += CB_BLOCK) {
+                const float4 x = reinterpret_cast<const float4*>(sp)[q];
+                if (dp) reinterpret_cast<float4*>(dp)[q] = x;
+                reinterpret_cast<float4*>(s_tri)[q] = x;
+            }
+        } else {
+            for (uint32_t q = threadIdx.x; q < cnt * TRI_WORDS; q += CB_BLOCK) {
+                const float x = sp[q];
+                if (dp) dp[q] = x;
+                s_tri[q] = x;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const f3 m = tri_centroid(tri_from_words(s_tri + threadIdx.x * TRI_WORDS));
+            v[0] = fminf(v[0], m.x); v[1] = fminf(v[1], m.y); v[2] = fminf(v[2], m.z);
+            v[3] = fmaxf(v[3], m.x); v[4] = fmaxf(v[4], m.y); v[5] = fmaxf(v[5], m.z);
+        }
+        __syncthreads();
+    }
+    block_minmax6(v, part + (size_t)blockIdx.x * 6);
+}
+
+// the centroid box from the blocks' extremes, as order-preserving words
+__global__ __launch_bounds__(CB_BLOCK) void k_bounds_final(const float* __restrict__ part, uint32_t parts,
+                                                           uint32_t* __restrict__ cb)
+{
+    float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t b = threadIdx.x; b < parts; b += CB_BLOCK) {
+        for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], part[(size_t)b * 6 + k]);
+        for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], part[(size_t)b * 6 + k]);
+    }
+    __shared__ float r[6];
+    block_minmax6(v, r);
+    __syncthreads();
+    if (threadIdx.x < 6) cb[threadIdx.x] = f2o(r[threadIdx.x]);
 }
 
 // morton_encode3D (morton.h:33-47)
@@ -314,29 +365,40 @@ __global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t* __restrict__ out, 
 // Karras hierarchy (detail/bvh/lbvh.h:120-185, 281-330)
 // ---------------------------------------------------------------------------
 
-__device__ inline int kdelta(const uint32_t* __restrict__ k, int n, int i, int j)
-{
-    if (j < 0 || j >= n) return -1;
-    const uint32_t x = k[i] ^ k[j];
-    if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
-    return (int)__clz((int)x);
-}
+// Karras' delta over a block's window of the sorted keys: most ranges and
+// splits are short, so nearly every probe is an LDS read and only the long
+// ranges of the upper levels go to global memory
+constexpr int KWIN = 256;
 
-// parent link of a node: (parent pair << 1) | side
 __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ codes, int n,
                                                   pair_node* __restrict__ pairs, uint32_t* __restrict__ leaf_parent)
 {
+    __shared__ uint32_t s_codes[BLOCK + 2 * KWIN];
+    const int w0 = (int)(blockIdx.x * BLOCK) - KWIN;
+    for (int t = threadIdx.x; t < BLOCK + 2 * KWIN; t += BLOCK) {
+        const int g = w0 + t;
+        s_codes[t] = g >= 0 && g < n ? codes[g] : 0u;
+    }
+    __syncthreads();
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n - 1) return;
-    const int d = kdelta(codes, n, i, i + 1) >= kdelta(codes, n, i, i - 1) ? 1 : -1;
-    const int dmin = kdelta(codes, n, i, i - d);
+    const uint32_t ki = s_codes[i - w0];
+    auto kdelta = [&](int j) -> int {
+        if (j < 0 || j >= n) return -1;
+        const int t = j - w0;
+        const uint32_t x = ki ^ (t >= 0 && t < BLOCK + 2 * KWIN ? s_codes[t] : codes[j]);
+        if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
+        return (int)__clz((int)x);
+    };
+    const int d = kdelta(i + 1) >= kdelta(i - 1) ? 1 : -1;
+    const int dmin = kdelta(i - d);
     int lmax = 2;
-    while (kdelta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    while (kdelta(i + lmax * d) > dmin) lmax *= 2;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (kdelta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+        if (kdelta(i + (l + t) * d) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = kdelta(codes, n, i, j);
+    const int dnode = kdelta(j);
     // split search with steps t = ceil(l / 2^k): the reference evaluates
     // ceil(l / divf) in float (exact for l < 2^24) and keeps probing with
     // t = 1 afterwards; those extra probes cannot move s (the prefix length
@@ -345,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ c
     int s = 0;
     for (int k = 1;; ++k) {
         const int t = (l + (1 << k) - 1) >> k;
-        if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+        if (kdelta(i + (s + t) * d) > dnode) s += t;
         if (t <= 1) break;
     }
     const int split = i + s * d + (d < 0 ? d : 0);
@@ -442,10 +504,12 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     __shared__ float s_box[CHUNK * 12];
     __shared__ uint32_t s_h[CHUNK * 2];
     __shared__ uint32_t s_flag[CHUNK];
+    __shared__ uint2 s_link[CHUNK];  // e[2], e[3] of the block's pairs: the climb reads no global links
     const uint32_t tid = threadIdx.x;
     const size_t c0 = (size_t)blockIdx.x * CHUNK;
     const size_t c1 = min(n, c0 + (size_t)CHUNK);
     s_flag[tid] = 0;
+    if (c0 + tid + 1 < n) s_link[tid] = *reinterpret_cast<const uint2*>(&pairs[c0 + tid].e[2]);
     __syncthreads();
     const size_t j = c0 + tid;
     if (j >= n) return;
@@ -472,8 +536,9 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     for (int level = 0; level < 128; ++level) {
         const uint32_t p = link >> 1, side = link & 1u;
         if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
-        const uint32_t other = pairs[p].e[3];
-        if (!(p >= c0 && p < c1 && other >= c0 && other < c1)) {
+        const bool inside = p >= c0 && p < c1;
+        const uint32_t other = inside ? s_link[p - c0].y : pairs[p].e[3];
+        if (!(inside && other >= c0 && other < c1)) {
             publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
             if (atomicAdd(&flags[p], 1u) == 1u) {
                 const uint32_t b = p / SUPER;
@@ -504,7 +569,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
             finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
             return;
         }
-        link = pairs[p].e[2];
+        link = s_link[k].x;
     }
 }
 
@@ -618,6 +683,7 @@ struct build_scratch {
     uint32_t* sums1;  // level-2 block sums
     uint32_t* sums2;
     uint32_t* cb;     // 6 words
+    float* part;      // CB_GRID x 6 per-block centroid extremes
     uint32_t* leaf_parent;
     uint32_t* counts;  // bottom-up ready-list counters: one per SUPER block, then the top list's
                        // (the ready lists reuse keys0 and vals0 after the sort)
@@ -647,6 +713,7 @@ build_scratch plan_scratch(size_t n, char* base)
     s.sums1 = take(l2);
     s.sums2 = take(1);
     s.cb = take(8);
+    s.part = reinterpret_cast<float*>(take((size_t)CB_GRID * 6));
     s.leaf_parent = take(n);
     s.counts = take((n + SUPER - 1) / SUPER + 1);
     s.bytes = off;
@@ -674,11 +741,14 @@ int scan_exclusive(const uint32_t* in, uint32_t* out, size_t count, uint32_t* su
     return VR_OK;
 }
 
-int launch_morton(const vr_triangle* prims, size_t n, uint32_t* cb, uint32_t* codes, hipStream_t st)
+// centroid box (and, when dst is set, the primitive copy), then the codes
+int launch_morton(const vr_triangle* prims, size_t n, vr_triangle* dst, float* part, uint32_t* cb, uint32_t* codes,
+                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, cb);
-    const unsigned grid = std::max(1u, std::min(div_up(n, BLOCK), 4096u));
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(grid), dim3(BLOCK), 0, st, prims, n, cb);
+    const unsigned grid = std::max(1u, std::min(div_up(n, CB_BLOCK), (unsigned)CB_GRID));
+    const int vec = (reinterpret_cast<uintptr_t>(prims) % 16 == 0) && (!dst || reinterpret_cast<uintptr_t>(dst) % 16 == 0);
+    hipLaunchKernelGGL(k_copy_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, dst, vec, part);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(CB_BLOCK), 0, st, part, grid, cb);
     hipLaunchKernelGGL(k_morton, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, prims, n, cb, codes);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
@@ -713,10 +783,12 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const size_t npairs = n > 1 ? n - 1 : 0;
     // (re)allocate owned arrays when the size changes
     if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
+        // the handle's arrays, stream ordered from the device's pool (a
+        // build after a destroyed one of similar size reuses its memory)
         void* owned[] = {bs.pairs, bs.quads, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
                          bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
         for (void* q : owned)
-            if (q) (void)hipFree(q);
+            if (q) VR_HIP_CHECK(bs.pooled ? hipFreeAsync(q, st) : hipFree(q));
         const int device = bs.device;
         bs = bvh_state{};
         bs.device = device;
@@ -724,32 +796,32 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         bs.num_pairs = npairs;
         bs.is_lbvh = true;
         bs.want_rotate = rotate;
+        bs.pooled = true;
         if (n == 0) return VR_OK;
-        VR_HIP_CHECK(hipMalloc(&bs.pairs, std::max<size_t>(npairs, 1) * sizeof(pair_node)));
-        VR_HIP_CHECK(hipMalloc(&bs.tris, n * sizeof(tri_node)));
-        VR_HIP_CHECK(hipMalloc(&bs.prims, n * sizeof(vr_triangle)));
-        VR_HIP_CHECK(hipMalloc(&bs.sorted_codes, n * 4));
-        VR_HIP_CHECK(hipMalloc(&bs.sorted_ids, n * 4));
-        VR_HIP_CHECK(hipMalloc(&bs.flags, std::max<size_t>(npairs, 1) * 4));
-        VR_HIP_CHECK(hipMalloc(&bs.heights, std::max<size_t>(2 * npairs, 1) * 4));
-        VR_HIP_CHECK(hipMalloc(&bs.root_box, 8 * sizeof(float)));
-        VR_HIP_CHECK(hipMalloc(&bs.indices, n * 4));
+        auto alloc = [&](auto** p, size_t bytes) { return hipMallocAsync(reinterpret_cast<void**>(p), bytes, st); };
+        VR_HIP_CHECK(alloc(&bs.pairs, std::max<size_t>(npairs, 1) * sizeof(pair_node)));
+        VR_HIP_CHECK(alloc(&bs.quads, std::max<size_t>(npairs, 1) * sizeof(quad_node)));
+        VR_HIP_CHECK(alloc(&bs.tris, n * sizeof(tri_node)));
+        VR_HIP_CHECK(alloc(&bs.prims, n * sizeof(vr_triangle)));
+        VR_HIP_CHECK(alloc(&bs.sorted_codes, n * 4));
+        VR_HIP_CHECK(alloc(&bs.sorted_ids, n * 4));
+        VR_HIP_CHECK(alloc(&bs.flags, std::max<size_t>(npairs, 1) * 4));
+        VR_HIP_CHECK(alloc(&bs.heights, std::max<size_t>(2 * npairs, 1) * 4));
+        VR_HIP_CHECK(alloc(&bs.root_box, 8 * sizeof(float)));
+        VR_HIP_CHECK(alloc(&bs.indices, n * 4));
         if (keep_nodes) {
             bs.num_ref_nodes = 2 * n - 1;
-            VR_HIP_CHECK(hipMalloc(&bs.ref_nodes, bs.num_ref_nodes * sizeof(vr_bvh_node)));
+            VR_HIP_CHECK(alloc(&bs.ref_nodes, bs.num_ref_nodes * sizeof(vr_bvh_node)));
         }
         build_scratch plan = plan_scratch(n, nullptr);
         bs.scratch_bytes = plan.bytes;
-        VR_HIP_CHECK(hipMalloc(&bs.scratch, plan.bytes));
+        VR_HIP_CHECK(alloc(&bs.scratch, plan.bytes));
     }
     if (n == 0) return VR_OK;
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
 
-    // copy of the primitives (index_bvh_t(P*, n), bvh.h:891-896)
-    VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
-
-    // 1-2: centroid bounds and Morton codes
-    int rc = launch_morton(d_prims, n, s.cb, s.keys0, st);
+    // 1-2: copy of the primitives with the centroid bounds, Morton codes
+    int rc = launch_morton(d_prims, n, bs.prims, s.part, s.cb, s.keys0, st);
     if (rc) return rc;
 
     // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
@@ -839,10 +911,12 @@ int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st)
 {
     if (n == 0) return VR_OK;
-    uint32_t* cb = nullptr;
-    VR_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&cb), 64, st));
-    int rc = launch_morton(d_prims, n, cb, d_codes, st);
-    (void)hipFreeAsync(cb, st);
+    void* tmp = nullptr;
+    VR_HIP_CHECK(hipMallocAsync(&tmp, 256 + (size_t)CB_GRID * 6 * sizeof(float), st));
+    uint32_t* cb = static_cast<uint32_t*>(tmp);
+    float* part = reinterpret_cast<float*>(static_cast<char*>(tmp) + 256);
+    int rc = launch_morton(d_prims, n, nullptr, part, cb, d_codes, st);
+    (void)hipFreeAsync(tmp, st);
     return rc;
 }
 

@@ -144,6 +144,7 @@ struct bvh_state {
     float* root_box = nullptr;          // 6 floats
     bool rotated = false;               // LBVH after tree rotations (no refit, no reference layout)
     bool want_rotate = false;           // LBVH builds of this handle run the rotation pass
+    bool pooled = false;                // arrays from the stream-ordered pool (hipMallocAsync)
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
