@@ -521,8 +521,12 @@ int vr_bvh_destroy(vr_bvh bvh)
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(s.device);
         (void)hipDeviceSynchronize();
-        for (void* p : ptrs)
-            if (p) (void)hipFreeAsync(p, nullptr);
+        if (s.arena) {
+            (void)hipFreeAsync(s.arena, nullptr);
+        } else {
+            for (void* p : ptrs)
+                if (p) (void)hipFreeAsync(p, nullptr);
+        }
         (void)hipSetDevice(cur);
     } else {
         for (void* p : ptrs)

@@ -23,6 +23,7 @@
 #include "vr_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace vr {
@@ -187,28 +188,6 @@ __device__ inline uint32_t separate_bits3(uint32_t n)
     return n;
 }
 
-// assign_morton_codes (detail/bvh/lbvh.h:242-269): centroid relative to the
-// centroid box, scaled by 1024, clamped to [0, 1023] with the ternary min/max
-// (a degenerate axis gives 0/0 = NaN, which the clamp maps to 1023), then
-// truncated and interleaved.
-__global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict__ prims, size_t n,
-                                                  const uint32_t* __restrict__ cb, uint32_t* __restrict__ codes)
-{
-    const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const f3 lo = mk3(o2f(cb[0]), o2f(cb[1]), o2f(cb[2]));
-    const f3 hi = mk3(o2f(cb[3]), o2f(cb[4]), o2f(cb[5]));
-    const f3 center = muls3(add3(hi, lo), 0.5f);
-    const f3 size = sub3(hi, lo);
-    f3 c = tri_centroid(load_tri(prims, i));
-    c = sub3(c, center);
-    c = add3(c, muls3(size, 0.5f));
-    c = f3{c.x / size.x, c.y / size.y, c.z / size.z};
-    c = min3v(max3v(muls3(c, 1024.0f), mk3(0.0f, 0.0f, 0.0f)), mk3(1023.0f, 1023.0f, 1023.0f));
-    codes[i] = separate_bits3((uint32_t)c.x) | (separate_bits3((uint32_t)c.y) << 1) |
-               (separate_bits3((uint32_t)c.z) << 2);
-}
-
 // ---------------------------------------------------------------------------
 // stable LSD radix sort, 8-bit digits
 // ---------------------------------------------------------------------------
@@ -271,7 +250,8 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp /* W
 __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in, int first_pass,
                                                          size_t n, int shift, const uint32_t* __restrict__ hist,
-                                                         const uint32_t* __restrict__ offsets, uint32_t num_tiles,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ totals, uint32_t num_tiles,
                                                          uint32_t* __restrict__ keys_out,
                                                          uint32_t* __restrict__ vals_out)
 {
@@ -286,8 +266,11 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     const uint32_t tid = threadIdx.x, w = tid / 64;
     const size_t base = (size_t)blockIdx.x * TILE;
     const uint32_t cnt = hist[(size_t)tid * num_tiles + blockIdx.x];
-    s_goff[tid] = offsets[(size_t)tid * num_tiles + blockIdx.x];
     uint32_t total;
+    // digit tid's first output slot: all smaller digits, then this digit in
+    // the tiles before this one
+    const uint32_t dbase = block_exclusive_scan(totals[tid], s_tmp, total);
+    s_goff[tid] = dbase + offsets[(size_t)tid * num_tiles + blockIdx.x];
     s_dstart[tid] = block_exclusive_scan(cnt, s_tmp, total);
     s_run[tid] = 0;
     for (int q = 0; q < WAVES; ++q) s_wcnt[q][tid] = 0;
@@ -331,34 +314,104 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     }
 }
 
-// exclusive scan of `count` uint32 values, 2048 per block
-__global__ __launch_bounds__(BLOCK) void k_scan_block(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                      size_t count, uint32_t* __restrict__ block_sums)
+// The spine of one radix pass: block d scans digit d's row of per-tile
+// counts (hist[d * tiles + t]) into offs and leaves the row's total; the
+// scatter adds the exclusive scan of the 256 totals (one per thread).
+__global__ __launch_bounds__(BLOCK) void k_radix_spine(const uint32_t* __restrict__ hist, uint32_t tiles,
+                                                       uint32_t* __restrict__ offs, uint32_t* __restrict__ totals)
 {
     __shared__ uint32_t s_tmp[WAVES + 1];
-    const size_t base = (size_t)blockIdx.x * TILE + (size_t)threadIdx.x * ITEMS;
-    uint32_t v[ITEMS];
-    uint32_t sum = 0;
-    for (int k = 0; k < ITEMS; ++k) {
-        v[k] = base + k < count ? in[base + k] : 0u;
-        sum += v[k];
+    const uint32_t* row = hist + (size_t)blockIdx.x * tiles;
+    uint32_t* orow = offs + (size_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < tiles; base += TILE) {
+        const uint32_t b = base + threadIdx.x * ITEMS;
+        uint32_t v[ITEMS];
+        uint32_t sum = 0;
+        for (int k = 0; k < ITEMS; ++k) {
+            v[k] = b + k < tiles ? row[b + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = block_exclusive_scan(sum, s_tmp, total) + carry;
+        for (int k = 0; k < ITEMS; ++k) {
+            if (b + k < tiles) orow[b + k] = run;
+            run += v[k];
+        }
+        carry += total;
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(sum, s_tmp, total);
-    for (int k = 0; k < ITEMS; ++k) {
-        if (base + k < count) out[base + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t* __restrict__ out, size_t count,
-                                                    const uint32_t* __restrict__ block_offsets)
+// assign_morton_codes (detail/bvh/lbvh.h:242-269): centroid relative to the
+// centroid box, scaled by 1024, clamped to [0, 1023] with the ternary min/max
+// (a degenerate axis gives 0/0 = NaN, which the clamp maps to 1023), then
+// truncated and interleaved.
+//
+// One block per sort tile (TILE keys), staged through LDS in BLOCK-triangle
+// chunks (16-byte loads when the triangles are 16-byte aligned).  In a build
+// the same kernel leaves the first radix pass's per-tile digit counts and
+// clears the hierarchy's link and counter arrays.
+struct morton_extras {
+    uint32_t* hist;         // digit-0 counts per tile (first radix pass), or null
+    uint32_t tiles;
+    uint32_t* leaf_parent;  // n words set to ~0 (no parent yet), or null
+    uint32_t* flags;        // n - 1 words cleared
+    uint32_t* counts;       // ncounts words cleared
+    uint32_t ncounts;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict__ prims, size_t n, int vec,
+                                                  const uint32_t* __restrict__ cb, uint32_t* __restrict__ codes,
+                                                  morton_extras x)
 {
-    const size_t base = (size_t)blockIdx.x * TILE + (size_t)threadIdx.x * ITEMS;
-    const uint32_t add = block_offsets[blockIdx.x];
-    for (int k = 0; k < ITEMS; ++k)
-        if (base + k < count) out[base + k] += add;
+    __shared__ float s_tri[BLOCK * TRI_WORDS];
+    __shared__ uint32_t h[RADIX];
+    const uint32_t tid = threadIdx.x;
+    h[tid] = 0u;
+    const f3 lo = mk3(o2f(cb[0]), o2f(cb[1]), o2f(cb[2]));
+    const f3 hi = mk3(o2f(cb[3]), o2f(cb[4]), o2f(cb[5]));
+    const f3 center = muls3(add3(hi, lo), 0.5f);
+    const f3 size = sub3(hi, lo);
+    const size_t base = (size_t)blockIdx.x * TILE;
+    for (int k = 0; k < ITEMS; ++k) {
+        const size_t t0 = base + (size_t)k * BLOCK;
+        if (t0 >= n) break;  // uniform over the block
+        const uint32_t cnt = (uint32_t)min((size_t)BLOCK, n - t0);
+        const float* sp = reinterpret_cast<const float*>(prims + t0);
+        if (vec && cnt == BLOCK) {
+            for (uint32_t q = tid; q < BLOCK * TRI_WORDS / 4; q += BLOCK)
+                reinterpret_cast<float4*>(s_tri)[q] = reinterpret_cast<const float4*>(sp)[q];
+        } else {
+            for (uint32_t q = tid; q < cnt * TRI_WORDS; q += BLOCK) s_tri[q] = sp[q];
+        }
+        __syncthreads();
+        const bool valid = tid < cnt;
+        const size_t i = t0 + tid;
+        uint32_t code = 0u;
+        if (valid) {
+            f3 c = tri_centroid(tri_from_words(s_tri + tid * TRI_WORDS));
+            c = sub3(c, center);
+            c = add3(c, muls3(size, 0.5f));
+            c = f3{c.x / size.x, c.y / size.y, c.z / size.z};
+            c = min3v(max3v(muls3(c, 1024.0f), mk3(0.0f, 0.0f, 0.0f)), mk3(1023.0f, 1023.0f, 1023.0f));
+            code = separate_bits3((uint32_t)c.x) | (separate_bits3((uint32_t)c.y) << 1) |
+                   (separate_bits3((uint32_t)c.z) << 2);
+            codes[i] = code;
+            if (x.leaf_parent) {
+                x.leaf_parent[i] = 0xFFFFFFFFu;
+                if (i + 1 < n) x.flags[i] = 0u;
+            }
+        }
+        if (x.hist) {
+            const uint32_t d = code & 0xFFu;
+            const uint64_t m = match_digit(d, valid);
+            if (valid && mbcnt64(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+        }
+        __syncthreads();
+    }
+    if (x.hist) x.hist[(size_t)tid * x.tiles + blockIdx.x] = h[tid];
+    for (size_t c = (size_t)blockIdx.x * BLOCK + tid; c < x.ncounts; c += (size_t)gridDim.x * BLOCK) x.counts[c] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -499,7 +552,8 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
                                                            uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
                                                            vr_bvh_node* __restrict__ ref_nodes,
                                                            float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
-                                                           uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count)
+                                                           uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count,
+                                                           uint32_t* __restrict__ indices)
 {
     __shared__ float s_box[CHUNK * 12];
     __shared__ uint32_t s_h[CHUNK * 2];
@@ -513,6 +567,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     __syncthreads();
     const size_t j = c0 + tid;
     if (j >= n) return;
+    if (indices) indices[j] = (uint32_t)j;  // index_bvh_t's primitive list is the identity
     const int32_t id = sorted_ids[j];
     const tri_in t = load_tri(prims, (size_t)id);
     tri_node tn;
@@ -654,11 +709,6 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
-__global__ void k_iota_u32(uint32_t* p, size_t n)
-{
-    const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
-    if (i < n) p[i] = (uint32_t)i;
-}
 
 __global__ __launch_bounds__(BLOCK) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n)
 {
@@ -677,11 +727,9 @@ struct build_scratch {
     uint32_t* vals0;
     uint32_t* keys1;
     uint32_t* vals1;
-    uint32_t* hist;
-    uint32_t* offs;
-    uint32_t* sums0;  // level-1 block sums
-    uint32_t* sums1;  // level-2 block sums
-    uint32_t* sums2;
+    uint32_t* hist;   // per-tile digit counts, digit-major
+    uint32_t* offs;   // their exclusive scan along each digit's row
+    uint32_t* totals; // per-digit totals
     uint32_t* cb;     // 6 words
     float* part;      // CB_GRID x 6 per-block centroid extremes
     uint32_t* leaf_parent;
@@ -694,8 +742,6 @@ build_scratch plan_scratch(size_t n, char* base)
 {
     const size_t tiles = std::max<size_t>(1, (n + TILE - 1) / TILE);
     const size_t hist_n = tiles * RADIX;
-    const size_t l1 = (hist_n + TILE - 1) / TILE;
-    const size_t l2 = (l1 + TILE - 1) / TILE;
     size_t off = 0;
     auto take = [&](size_t elems) {
         char* p = base ? base + off : nullptr;
@@ -709,9 +755,7 @@ build_scratch plan_scratch(size_t n, char* base)
     s.vals1 = take(n);
     s.hist = take(hist_n);
     s.offs = take(hist_n);
-    s.sums0 = take(l1);
-    s.sums1 = take(l2);
-    s.sums2 = take(1);
+    s.totals = take(RADIX);
     s.cb = take(8);
     s.part = reinterpret_cast<float*>(take((size_t)CB_GRID * 6));
     s.leaf_parent = take(n);
@@ -720,49 +764,35 @@ build_scratch plan_scratch(size_t n, char* base)
     return s;
 }
 
-// exclusive scan of `count` values in `in` into `out` (may alias); up to
-// 2048^3 values through three levels
-int scan_exclusive(const uint32_t* in, uint32_t* out, size_t count, uint32_t* sums0, uint32_t* sums1,
-                   uint32_t* sums2, hipStream_t st)
-{
-    const size_t b0 = (count + TILE - 1) / TILE;
-    VR_CHECK(b0 <= (size_t)TILE * TILE, VR_ERR_UNSUPPORTED, "scan: too many elements");
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)b0), dim3(BLOCK), 0, st, in, out, count, sums0);
-    if (b0 > 1) {
-        const size_t b1 = (b0 + TILE - 1) / TILE;
-        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)b1), dim3(BLOCK), 0, st, sums0, sums0, b0, sums1);
-        if (b1 > 1) {
-            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(BLOCK), 0, st, sums1, sums1, b1, sums2);
-            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)b1), dim3(BLOCK), 0, st, sums0, b0, sums1);
-        }
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)b0), dim3(BLOCK), 0, st, out, count, sums0);
-    }
-    VR_HIP_CHECK(hipGetLastError());
-    return VR_OK;
-}
-
 // centroid box (and, when dst is set, the primitive copy), then the codes
+// (read from the copy when there is one) with the kernel's extras
 int launch_morton(const vr_triangle* prims, size_t n, vr_triangle* dst, float* part, uint32_t* cb, uint32_t* codes,
-                  hipStream_t st)
+                  const morton_extras& x, hipStream_t st)
 {
     const unsigned grid = std::max(1u, std::min(div_up(n, CB_BLOCK), (unsigned)CB_GRID));
-    const int vec = (reinterpret_cast<uintptr_t>(prims) % 16 == 0) && (!dst || reinterpret_cast<uintptr_t>(dst) % 16 == 0);
+    auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+    const int vec = aligned(prims) && (!dst || aligned(dst));
     hipLaunchKernelGGL(k_copy_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, dst, vec, part);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(CB_BLOCK), 0, st, part, grid, cb);
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, prims, n, cb, codes);
+    const vr_triangle* src = dst ? dst : prims;
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, TILE)), dim3(BLOCK), 0, st, src, n, (int)aligned(src), cb, codes, x);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
 
-void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* ref_nodes, hipStream_t st)
+// the bottom-up passes; a build has cleared the ready-list counters already
+// (k_morton) and writes the identity index list on the way (a refit keeps it)
+void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* ref_nodes, bool build,
+                      hipStream_t st)
 {
     const size_t n = bs.num_prims;
     const size_t nb = (n + SUPER - 1) / SUPER;
     uint32_t* depth = reinterpret_cast<uint32_t*>(bs.root_box + 6);
-    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
+    if (!build)
+        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
     hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, bs.prims, n, bs.sorted_ids,
                        s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
-                       s.keys0, s.counts);
+                       s.keys0, s.counts, build ? reinterpret_cast<uint32_t*>(bs.indices) : nullptr);
     if (n > (size_t)CHUNK)
         hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags,
                            bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb);
@@ -783,12 +813,16 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const size_t npairs = n > 1 ? n - 1 : 0;
     // (re)allocate owned arrays when the size changes
     if (bs.num_prims != n || !bs.is_lbvh || (keep_nodes && !bs.ref_nodes)) {
-        // the handle's arrays, stream ordered from the device's pool (a
-        // build after a destroyed one of similar size reuses its memory)
-        void* owned[] = {bs.pairs, bs.quads, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
-                         bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
-        for (void* q : owned)
-            if (q) VR_HIP_CHECK(bs.pooled ? hipFreeAsync(q, st) : hipFree(q));
+        // the handle's arrays: one block, stream ordered from the device's
+        // pool (a build after a destroyed one of similar size reuses it)
+        if (bs.arena) {
+            VR_HIP_CHECK(hipFreeAsync(bs.arena, st));
+        } else {
+            void* owned[] = {bs.pairs, bs.quads, bs.tris, bs.prims, bs.sorted_codes, bs.sorted_ids,
+                             bs.flags, bs.heights, bs.ref_nodes, bs.indices, bs.root_box, bs.scratch};
+            for (void* q : owned)
+                if (q) VR_HIP_CHECK(bs.pooled ? hipFreeAsync(q, st) : hipFree(q));
+        }
         const int device = bs.device;
         bs = bvh_state{};
         bs.device = device;
@@ -798,35 +832,46 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         bs.want_rotate = rotate;
         bs.pooled = true;
         if (n == 0) return VR_OK;
-        auto alloc = [&](auto** p, size_t bytes) { return hipMallocAsync(reinterpret_cast<void**>(p), bytes, st); };
-        VR_HIP_CHECK(alloc(&bs.pairs, std::max<size_t>(npairs, 1) * sizeof(pair_node)));
-        VR_HIP_CHECK(alloc(&bs.quads, std::max<size_t>(npairs, 1) * sizeof(quad_node)));
-        VR_HIP_CHECK(alloc(&bs.tris, n * sizeof(tri_node)));
-        VR_HIP_CHECK(alloc(&bs.prims, n * sizeof(vr_triangle)));
-        VR_HIP_CHECK(alloc(&bs.sorted_codes, n * 4));
-        VR_HIP_CHECK(alloc(&bs.sorted_ids, n * 4));
-        VR_HIP_CHECK(alloc(&bs.flags, std::max<size_t>(npairs, 1) * 4));
-        VR_HIP_CHECK(alloc(&bs.heights, std::max<size_t>(2 * npairs, 1) * 4));
-        VR_HIP_CHECK(alloc(&bs.root_box, 8 * sizeof(float)));
-        VR_HIP_CHECK(alloc(&bs.indices, n * 4));
-        if (keep_nodes) {
-            bs.num_ref_nodes = 2 * n - 1;
-            VR_HIP_CHECK(alloc(&bs.ref_nodes, bs.num_ref_nodes * sizeof(vr_bvh_node)));
-        }
+        const size_t np1 = std::max<size_t>(npairs, 1);
+        if (keep_nodes) bs.num_ref_nodes = 2 * n - 1;
         build_scratch plan = plan_scratch(n, nullptr);
         bs.scratch_bytes = plan.bytes;
-        VR_HIP_CHECK(alloc(&bs.scratch, plan.bytes));
+        const size_t sizes[] = {np1 * sizeof(pair_node), np1 * sizeof(quad_node), n * sizeof(tri_node),
+                                n * sizeof(vr_triangle), n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
+                                n * 4, bs.num_ref_nodes * sizeof(vr_bvh_node), plan.bytes};
+        size_t total = 0;
+        for (size_t b : sizes) total += align_up(b, 256);
+        VR_HIP_CHECK(hipMallocAsync(&bs.arena, total, st));
+        char* at = static_cast<char*>(bs.arena);
+        auto carve = [&](auto*& ptr, size_t bytes) {
+            ptr = bytes ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(at) : nullptr;
+            at += align_up(bytes, 256);
+        };
+        carve(bs.pairs, sizes[0]);
+        carve(bs.quads, sizes[1]);
+        carve(bs.tris, sizes[2]);
+        carve(bs.prims, sizes[3]);
+        carve(bs.sorted_codes, sizes[4]);
+        carve(bs.sorted_ids, sizes[5]);
+        carve(bs.flags, sizes[6]);
+        carve(bs.heights, sizes[7]);
+        carve(bs.root_box, sizes[8]);
+        carve(bs.indices, sizes[9]);
+        carve(bs.ref_nodes, sizes[10]);
+        carve(bs.scratch, sizes[11]);
     }
     if (n == 0) return VR_OK;
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
 
     // 1-2: copy of the primitives with the centroid bounds, Morton codes
-    int rc = launch_morton(d_prims, n, bs.prims, s.part, s.cb, s.keys0, st);
+    // (with the first sort pass's digit counts and the cleared link arrays)
+    const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
+    const size_t nb = (n + SUPER - 1) / SUPER;
+    const morton_extras extras{s.hist, tiles, s.leaf_parent, bs.flags, s.counts, (uint32_t)(nb + 1)};
+    int rc = launch_morton(d_prims, n, bs.prims, s.part, s.cb, s.keys0, extras, st);
     if (rc) return rc;
 
     // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
-    const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
-    const size_t hist_n = (size_t)tiles * RADIX;
     uint32_t* kin = s.keys0;
     uint32_t* vin = s.vals0;
     uint32_t* kout = s.keys1;
@@ -837,26 +882,22 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
             kout = bs.sorted_codes;
             vout = reinterpret_cast<uint32_t*>(bs.sorted_ids);
         }
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(BLOCK), 0, st, kin, n, shift, s.hist, tiles);
-        rc = scan_exclusive(s.hist, s.offs, hist_n, s.sums0, s.sums1, s.sums2, st);
-        if (rc) return rc;
+        if (pass > 0)
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(BLOCK), 0, st, kin, n, shift, s.hist, tiles);
+        hipLaunchKernelGGL(k_radix_spine, dim3(RADIX), dim3(BLOCK), 0, st, s.hist, tiles, s.offs, s.totals);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(BLOCK), 0, st, kin, vin, pass == 0 ? 1 : 0, n, shift,
-                           s.hist, s.offs, tiles, kout, vout);
+                           s.hist, s.offs, s.totals, tiles, kout, vout);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     VR_HIP_CHECK(hipGetLastError());
 
     // 4: hierarchy
-    if (n > 1) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, s.leaf_parent, n, 0xFFFFFFFFu);
+    if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
                            bs.pairs, s.leaf_parent);
-        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags, npairs, 0u);
-    }
-    // 5: boxes, traversal-order triangles, reference layout
-    launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, st);
-    hipLaunchKernelGGL(k_iota_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, st, bs.indices, n);
+    // 5: boxes, traversal-order triangles, reference layout, index list
+    launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, true, st);
     VR_HIP_CHECK(hipGetLastError());
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
     bs.max_leaf = 1;
@@ -890,7 +931,7 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     if (n > 1)
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags,
                            bs.num_pairs, 0u);
-    launch_bottom_up(bs, s, bs.ref_nodes, st);
+    launch_bottom_up(bs, s, bs.ref_nodes, false, st);
     VR_HIP_CHECK(hipGetLastError());
     // same topology: the traversal nodes take the new boxes
     return refit_quads(bs, st);
@@ -915,7 +956,7 @@ int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hi
     VR_HIP_CHECK(hipMallocAsync(&tmp, 256 + (size_t)CB_GRID * 6 * sizeof(float), st));
     uint32_t* cb = static_cast<uint32_t*>(tmp);
     float* part = reinterpret_cast<float*>(static_cast<char*>(tmp) + 256);
-    int rc = launch_morton(d_prims, n, nullptr, part, cb, d_codes, st);
+    int rc = launch_morton(d_prims, n, nullptr, part, cb, d_codes, morton_extras{}, st);
     (void)hipFreeAsync(tmp, st);
     return rc;
 }

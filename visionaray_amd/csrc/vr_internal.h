@@ -145,6 +145,7 @@ struct bvh_state {
     bool rotated = false;               // LBVH after tree rotations (no refit, no reference layout)
     bool want_rotate = false;           // LBVH builds of this handle run the rotation pass
     bool pooled = false;                // arrays from the stream-ordered pool (hipMallocAsync)
+    void* arena = nullptr;              // LBVH: one pooled block holding all the arrays above
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
