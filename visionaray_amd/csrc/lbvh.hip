@@ -439,7 +439,16 @@ __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ c
     auto kdelta = [&](int j) -> int {
         if (j < 0 || j >= n) return -1;
         const int t = j - w0;
-        const uint32_t x = ki ^ (t >= 0 && t < BLOCK + 2 * KWIN ? s_codes[t] : codes[j]);
+        // the empty asm keeps the two reads an LDS read and a global read
+        // (merged, they become one flat load through a selected pointer)
+        uint32_t kj;
+        if (t >= 0 && t < BLOCK + 2 * KWIN) {
+            kj = s_codes[t];
+            asm volatile("" : "+v"(kj));
+        } else {
+            kj = codes[j];
+        }
+        const uint32_t x = ki ^ kj;
         if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
         return (int)__clz((int)x);
     };
@@ -519,7 +528,10 @@ __device__ inline void store_ref_node(vr_bvh_node* o, f3 lo, f3 hi, uint32_t fir
 // second arrival appends the node to the next pass's ready list, bucketed by
 // the block of the node's index.  Kernel boundaries make every earlier write
 // visible to the next pass.
-constexpr int CHUNK = 1024;
+#ifndef VR_LBVH_CHUNK
+#define VR_LBVH_CHUNK 1024
+#endif
+constexpr int CHUNK = VR_LBVH_CHUNK;
 constexpr int TOP_BLOCK = 1024;
 // subtrees of at most this many triangles become one traversal leaf
 #ifndef VR_LBVH_LEAF_MAX
@@ -568,7 +580,10 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     const size_t j = c0 + tid;
     if (j >= n) return;
     if (indices) indices[j] = (uint32_t)j;  // index_bvh_t's primitive list is the identity
+    // both per-leaf words before the triangle gather and its stores: a wait
+    // for a load issued after the stores would wait for the stores too
     const int32_t id = sorted_ids[j];
+    uint32_t link = n > 1 ? leaf_parent[j] : 0u;
     const tri_in t = load_tri(prims, (size_t)id);
     tri_node tn;
     tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
@@ -584,7 +599,6 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
         return;
     }
     const uint32_t npairs = (uint32_t)(n - 1);
-    uint32_t link = leaf_parent[j];
     uint32_t h = 1, ref_first = (uint32_t)id, ref_num = 1;
     // a Karras tree has at most 65 levels; the bound only guards against a
     // corrupt hierarchy
