@@ -439,16 +439,7 @@ __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ c
     auto kdelta = [&](int j) -> int {
         if (j < 0 || j >= n) return -1;
         const int t = j - w0;
-        // the empty asm keeps the two reads an LDS read and a global read
-        // (merged, they become one flat load through a selected pointer)
-        uint32_t kj;
-        if (t >= 0 && t < BLOCK + 2 * KWIN) {
-            kj = s_codes[t];
-            asm volatile("" : "+v"(kj));
-        } else {
-            kj = codes[j];
-        }
-        const uint32_t x = ki ^ kj;
+        const uint32_t x = ki ^ (t >= 0 && t < BLOCK + 2 * KWIN ? s_codes[t] : codes[j]);
         if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
         return (int)__clz((int)x);
     };

@@ -271,7 +271,7 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
         P.I = add3(P.I, e);
         if (ADJ) {
             for (uint32_t j = 0; j < P.nmid; ++j) {
-                const auto* mj = as_global(S.mats + P.mids[j]);
+                const vr_material* mj = S.mats + P.mids[j];
                 const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
                 atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
                 atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);
@@ -283,11 +283,12 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
     if (P.bounce == 0) P.rhit = true;
     const f3 view = neg3(d);
     const f3 pos = add3(o, muls3(d, hr.t));
-    const float4 tb = ldg4(S.b.tris + hr.tri, 1), tcq = ldg4(S.b.tris + hr.tri, 2);
+    const float4* tq = reinterpret_cast<const float4*>(S.b.tris + hr.tri);
+    const float4 tb = tq[1], tcq = tq[2];
     const f3 gn = normalize3(cross3(ldtri3(tb), ldtri3(tcq)));
     uint32_t mid = __float_as_uint(tb.w);
     mid = mid < S.num_mats ? mid : S.num_mats - 1u;
-    const auto* m = as_global(S.mats + mid);
+    const vr_material* m = S.mats + mid;
     f3 refl = mk3(0.0f, 0.0f, 0.0f);
     float pdf;
     f3 src, src0 = mk3(0.0f, 0.0f, 0.0f);
@@ -344,7 +345,7 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
             atomicAdd(&s_grad[es + 4], P.wgt.y * P.tp.y * ls);
             atomicAdd(&s_grad[es + 5], P.wgt.z * P.tp.z * ls);
             for (uint32_t j = 0; j < P.nmid; ++j) {
-                const auto* mj = as_global(S.mats + P.mids[j]);
+                const vr_material* mj = S.mats + P.mids[j];
                 const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
                 atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
                 atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);

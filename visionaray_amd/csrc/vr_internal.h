@@ -74,25 +74,6 @@ struct alignas(64) pair_node {
 };
 static_assert(sizeof(pair_node) == 64, "pair_node is one 64-byte line");
 
-// BVH arrays are device (global) memory.  Kernels that take the arrays from
-// a descriptor in memory (the batched frames) see generic pointers; reading
-// through a global-space pointer keeps those loads global loads instead of
-// flat loads, which also wait on the LDS counter.
-typedef float vr_f4v __attribute__((ext_vector_type(4)));
-template <typename T>
-__device__ __forceinline__ float4 ldg4(const T* p, int i)
-{
-    const __attribute__((address_space(1))) vr_f4v* q =
-        (const __attribute__((address_space(1))) vr_f4v*)(reinterpret_cast<const vr_f4v*>(p));
-    const vr_f4v v = q[i];
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p)
-{
-    return (const __attribute__((address_space(1))) T*)p;
-}
-
 constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t LEAF_FIRST_MASK = 0x03FFFFFFu;
 constexpr uint32_t MAX_LEAF_FIRST = LEAF_FIRST_MASK;
@@ -361,29 +342,15 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
                                           uint32_t* __restrict__ spill, trace_counters* cnt, bool any_rt = false)
 {
     if (b.num_prims == 0) return true;
-    // the empty asm statements keep the LDS and the spill accesses apart:
-    // merged through a selected pointer they become flat accesses, which
-    // wait on both memory counters
     auto push = [&](uint32_t e) {
-        if (T.sp < LDS_DEPTH) {
-            stack[T.sp * STRIDE] = e;
-        } else {
-            asm volatile("" : "+v"(e));
-            spill[T.sp - LDS_DEPTH] = e;
-        }
+        if (T.sp < LDS_DEPTH) stack[T.sp * STRIDE] = e;
+        else spill[T.sp - LDS_DEPTH] = e;
         ++T.sp;
     };
     auto pop = [&]() -> uint32_t {
         if (T.sp == 0) return TRAV_DONE;
         --T.sp;
-        uint32_t e;
-        if (T.sp < LDS_DEPTH) {
-            e = stack[T.sp * STRIDE];
-            asm volatile("" : "+v"(e));
-        } else {
-            e = spill[T.sp - LDS_DEPTH];
-        }
-        return e;
+        return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
     };
     // a leaf at cur goes to the free triangle range
     if (T.ln == 0 && (T.cur & LEAF_BIT) && T.cur != TRAV_DONE) {
@@ -399,12 +366,9 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     if ((float)n_node >= VR_TRAV_NODE_BIAS * (float)n_tri) {
         if (!node_work) return false;
         if (--T.budget == 0 || T.cur >= b.num_quads) return true;
-        const quad_node* p = b.quads + T.cur;
-        const float4 lx = ldg4(p, 0), ly = ldg4(p, 1), lz = ldg4(p, 2), hx = ldg4(p, 3), hy = ldg4(p, 4),
-                     hz = ldg4(p, 5);
-        const float4 chf = ldg4(p, 6);
-        const uint4 ch = make_uint4(__float_as_uint(chf.x), __float_as_uint(chf.y), __float_as_uint(chf.z),
-                                    __float_as_uint(chf.w));
+        const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
+        const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+        const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
         if (COUNT) cnt->nodes++;
         const float tc = T.r.t;
         float k0 = slab_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T.ori, T.inv, T.tmin, T.tmax, tc);
@@ -440,7 +404,8 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     const uint32_t k = T.lf;
     ++T.lf;
     --T.ln;
-    const float4 ta = ldg4(b.tris + k, 0), tb = ldg4(b.tris + k, 1), tq = ldg4(b.tris + k, 2);
+    const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
+    const float4 ta = tp[0], tb = tp[1], tq = tp[2];
     if (COUNT) cnt->tris++;
     float t, u, v;
     if (intersect_tri(T.ori, T.dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
@@ -489,24 +454,13 @@ __device__ __forceinline__ hit_t traverse_binary(const bvh_view& b, f3 ori, f3 d
     uint32_t cur = b.root;
     uint32_t spill[MAX_STACK - LDS_DEPTH];
     auto push = [&](uint32_t e) {
-        if (sp < LDS_DEPTH) {
-            stack[sp * STRIDE] = e;
-        } else {
-            asm volatile("" : "+v"(e));  // an LDS store and a scratch store, not one flat store
-            spill[sp - LDS_DEPTH] = e;
-        }
+        if (sp < LDS_DEPTH) stack[sp * STRIDE] = e;
+        else spill[sp - LDS_DEPTH] = e;
         ++sp;
     };
     auto pop = [&]() -> uint32_t {
         --sp;
-        uint32_t e;
-        if (sp < LDS_DEPTH) {
-            e = stack[sp * STRIDE];
-            asm volatile("" : "+v"(e));  // an LDS read and a scratch read, not one flat read
-        } else {
-            e = spill[sp - LDS_DEPTH];
-        }
-        return e;
+        return sp < LDS_DEPTH ? stack[sp * STRIDE] : spill[sp - LDS_DEPTH];
     };
     // a ray visits each node at most once; a longer walk means a corrupt tree
     uint32_t budget = 2u * b.num_prims + 2u;
