@@ -20,6 +20,7 @@
 //      Nodes inside a 1024-leaf chunk meet in LDS, nodes inside a 65536-leaf
 //      block in one workgroup, the rest in a single workgroup: no
 //      device-scope fence is needed.
+#include "collapse.h"
 #include "vr_internal.h"
 
 #include <algorithm>
@@ -530,6 +531,21 @@ constexpr int TOP_BLOCK = 1024;
 #endif
 constexpr uint32_t LBVH_LEAF_MAX = VR_LBVH_LEAF_MAX;
 constexpr uint32_t SUPER = 64u * CHUNK;
+// pairs completed by one workgroup of the mid or top pass whose quads are
+// made after its climbs, one per thread
+constexpr uint32_t DONE_CAP = 4096;
+
+// quads of the pairs a workgroup completed, in parallel after its climbs
+__device__ inline void quads_after_climbs(const pair_node* __restrict__ pairs, uint32_t npairs, uint32_t leaf_max,
+                                          quad_node* __restrict__ quads, const uint32_t* s_done,
+                                          const uint32_t* s_ndone)
+{
+    __syncthreads();
+    if (!quads) return;
+    const uint32_t count = min(*s_ndone, DONE_CAP);
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
+        make_quad<true>(global_pairs{pairs}, s_done[i], npairs, leaf_max, quads);
+}
 
 __device__ inline void publish_child(pair_node* pairs, uint32_t* heights, vr_bvh_node* ref_nodes, uint32_t p,
                                      uint32_t side, f3 lo, f3 hi, uint32_t h, uint32_t ref_first,
@@ -548,6 +564,20 @@ __device__ inline void finish_root(float* root_box, vr_bvh_node* ref_nodes, uint
     *depth_out = h;
 }
 
+// the block's pair records in LDS: child boxes (12 floats) and entries
+struct lds_pairs {
+    const float* box_;
+    const uint4* ent_;
+    uint32_t c0;
+    __device__ uint4 ent(uint32_t q) const { return ent_[q - c0]; }
+    __device__ void box(uint32_t loc, f3& lo, f3& hi) const
+    {
+        const float* b = box_ + (size_t)((loc >> 1) - c0) * 12 + 6 * (loc & 1u);
+        lo = mk3(b[0], b[1], b[2]);
+        hi = mk3(b[3], b[4], b[5]);
+    }
+};
+
 __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __restrict__ prims, size_t n,
                                                            const int32_t* __restrict__ sorted_ids,
                                                            const uint32_t* __restrict__ leaf_parent,
@@ -556,81 +586,94 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
                                                            vr_bvh_node* __restrict__ ref_nodes,
                                                            float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
                                                            uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count,
-                                                           uint32_t* __restrict__ indices)
+                                                           uint32_t* __restrict__ indices,
+                                                           quad_node* __restrict__ quads, uint32_t leaf_max)
 {
     __shared__ float s_box[CHUNK * 12];
     __shared__ uint32_t s_h[CHUNK * 2];
     __shared__ uint32_t s_flag[CHUNK];
-    __shared__ uint2 s_link[CHUNK];  // e[2], e[3] of the block's pairs: the climb reads no global links
+    // the block's pair entries (children, parent link, range end): the climb
+    // and the quads of the block's pairs read no global records
+    __shared__ uint4 s_ent[CHUNK];
     const uint32_t tid = threadIdx.x;
     const size_t c0 = (size_t)blockIdx.x * CHUNK;
     const size_t c1 = min(n, c0 + (size_t)CHUNK);
     s_flag[tid] = 0;
-    if (c0 + tid + 1 < n) s_link[tid] = *reinterpret_cast<const uint2*>(&pairs[c0 + tid].e[2]);
+    if (c0 + tid + 1 < n) s_ent[tid] = *reinterpret_cast<const uint4*>(pairs[c0 + tid].e);
     __syncthreads();
     const size_t j = c0 + tid;
-    if (j >= n) return;
-    if (indices) indices[j] = (uint32_t)j;  // index_bvh_t's primitive list is the identity
-    // both per-leaf words before the triangle gather and its stores: a wait
-    // for a load issued after the stores would wait for the stores too
-    const int32_t id = sorted_ids[j];
-    uint32_t link = n > 1 ? leaf_parent[j] : 0u;
-    const tri_in t = load_tri(prims, (size_t)id);
-    tri_node tn;
-    tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
-    tn.b = make_float4(t.e1.x, t.e1.y, t.e1.z, __uint_as_float(t.geom_id));
-    tn.c = make_float4(t.e2.x, t.e2.y, t.e2.z, __uint_as_float((uint32_t)id));
-    tris[j] = tn;
-    f3 lo, hi;
-    tri_box(t, lo, hi);
-    if (n == 1) {
-        store_box(root_box, lo, hi);
-        if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, (uint32_t)id, 1u);
-        *depth_out = 1;
-        return;
-    }
-    const uint32_t npairs = (uint32_t)(n - 1);
-    uint32_t h = 1, ref_first = (uint32_t)id, ref_num = 1;
-    // a Karras tree has at most 65 levels; the bound only guards against a
-    // corrupt hierarchy
-    for (int level = 0; level < 128; ++level) {
-        const uint32_t p = link >> 1, side = link & 1u;
-        if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
-        const bool inside = p >= c0 && p < c1;
-        const uint32_t other = inside ? s_link[p - c0].y : pairs[p].e[3];
-        if (!(inside && other >= c0 && other < c1)) {
-            publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
-            if (atomicAdd(&flags[p], 1u) == 1u) {
-                const uint32_t b = p / SUPER;
-                ready[(size_t)b * SUPER + atomicAdd(&bucket_count[b], 1u)] = p;
+    // the leaf and its climb; every thread then meets at the barrier below
+    auto leaf_and_climb = [&]() {
+        if (indices) indices[j] = (uint32_t)j;  // index_bvh_t's primitive list is the identity
+        // both per-leaf words before the triangle gather and its stores: a
+        // wait for a load issued after the stores would wait for the stores too
+        const int32_t id = sorted_ids[j];
+        uint32_t link = n > 1 ? leaf_parent[j] : 0u;
+        const tri_in t = load_tri(prims, (size_t)id);
+        tri_node tn;
+        tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
+        tn.b = make_float4(t.e1.x, t.e1.y, t.e1.z, __uint_as_float(t.geom_id));
+        tn.c = make_float4(t.e2.x, t.e2.y, t.e2.z, __uint_as_float((uint32_t)id));
+        tris[j] = tn;
+        f3 lo, hi;
+        tri_box(t, lo, hi);
+        if (n == 1) {
+            store_box(root_box, lo, hi);
+            if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, (uint32_t)id, 1u);
+            *depth_out = 1;
+            return;
+        }
+        const uint32_t npairs = (uint32_t)(n - 1);
+        uint32_t h = 1, ref_first = (uint32_t)id, ref_num = 1;
+        // a Karras tree has at most 65 levels; the bound only guards against
+        // a corrupt hierarchy
+        for (int level = 0; level < 128; ++level) {
+            const uint32_t p = link >> 1, side = link & 1u;
+            if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
+            const bool inside = p >= c0 && p < c1;
+            const uint32_t other = inside ? s_ent[p - c0].w : pairs[p].e[3];
+            if (!(inside && other >= c0 && other < c1)) {
+                publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
+                if (atomicAdd(&flags[p], 1u) == 1u) {
+                    const uint32_t b = p / SUPER;
+                    ready[(size_t)b * SUPER + atomicAdd(&bucket_count[b], 1u)] = p;
+                }
+                return;
             }
-            return;
+            const uint32_t k = p - (uint32_t)c0;
+            if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
+            store_box(&s_box[k * 12 + side * 6], lo, hi);
+            s_h[2 * k + side] = h;
+            heights[2 * p + side] = h;  // every child slot's height (the rotation pass reads them)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (atomicAdd(&s_flag[k], 1u) == 0u) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            float b[12];
+            for (int q = 0; q < 12; ++q) b[q] = s_box[k * 12 + q];
+            float4* dst = reinterpret_cast<float4*>(pairs[p].f);
+            dst[0] = make_float4(b[0], b[1], b[2], b[3]);
+            dst[1] = make_float4(b[4], b[5], b[6], b[7]);
+            dst[2] = make_float4(b[8], b[9], b[10], b[11]);
+            lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
+            hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
+            h = max(s_h[2 * k], s_h[2 * k + 1]) + 1u;
+            ref_first = 1u + 2u * p;
+            ref_num = 0;
+            if (p == 0) {
+                finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
+                return;
+            }
+            link = s_ent[k].z;
         }
-        const uint32_t k = p - (uint32_t)c0;
-        if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
-        store_box(&s_box[k * 12 + side * 6], lo, hi);
-        s_h[2 * k + side] = h;
-        heights[2 * p + side] = h;  // every child slot's height (the rotation pass reads them)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (atomicAdd(&s_flag[k], 1u) == 0u) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        float b[12];
-        for (int q = 0; q < 12; ++q) b[q] = s_box[k * 12 + q];
-        float4* dst = reinterpret_cast<float4*>(pairs[p].f);
-        dst[0] = make_float4(b[0], b[1], b[2], b[3]);
-        dst[1] = make_float4(b[4], b[5], b[6], b[7]);
-        dst[2] = make_float4(b[8], b[9], b[10], b[11]);
-        lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
-        hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
-        h = max(s_h[2 * k], s_h[2 * k + 1]) + 1u;
-        ref_first = 1u + 2u * p;
-        ref_num = 0;
-        if (p == 0) {
-            finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
-            return;
-        }
-        link = s_link[k].x;
-    }
+    };
+    if (j < n) leaf_and_climb();
+    // every pair whose leaf range lies in the block is complete now, with
+    // both child boxes in LDS: its quad, one pair per thread
+    __syncthreads();
+    if (!quads || n < 2 || j + 1 >= n) return;
+    const uint32_t q = (uint32_t)j, other = s_ent[tid].w;
+    if (other >= c0 && other < c1 && pair_gets_quad(q, other, leaf_max))
+        make_quad<true>(lds_pairs{s_box, s_ent, (uint32_t)c0}, q, (uint32_t)(n - 1), leaf_max, quads);
 }
 
 // Climb from node p, whose children are both published, until an arrival
@@ -641,7 +684,9 @@ __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size
                                     pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
                                     uint32_t* __restrict__ heights, vr_bvh_node* __restrict__ ref_nodes,
                                     float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
-                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count)
+                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
+                                    quad_node* __restrict__ quads, uint32_t leaf_max, uint32_t* s_done,
+                                    uint32_t* s_ndone)
 {
     for (int level = 0; level < 128 && p < npairs; ++level) {
         const float4* src = reinterpret_cast<const float4*>(pairs[p].f);
@@ -649,6 +694,14 @@ __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size
         const f3 lo = min3v(mk3(q0.x, q0.y, q0.z), mk3(q1.z, q1.w, q2.x));
         const f3 hi = max3v(mk3(q0.w, q1.x, q1.y), mk3(q2.y, q2.z, q2.w));
         const uint32_t h = max(heights[2 * p], heights[2 * p + 1]) + 1u;
+        // p spans chunks (far above the leaf size) and its subtree is
+        // complete: its quad after the workgroup's climbs (or now, when the
+        // list is full)
+        if (quads) {
+            const uint32_t slot = atomicAdd(s_ndone, 1u);
+            if (slot < DONE_CAP) s_done[slot] = p;
+            else make_quad<true>(global_pairs{pairs}, p, npairs, leaf_max, quads);
+        }
         if (p == 0) {
             finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
             return;
@@ -680,15 +733,21 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, pair_node
                                                              uint32_t* __restrict__ depth_out,
                                                              const uint32_t* __restrict__ ready,
                                                              const uint32_t* __restrict__ bucket_count,
-                                                             uint32_t* __restrict__ top, uint32_t* __restrict__ top_count)
+                                                             uint32_t* __restrict__ top, uint32_t* __restrict__ top_count,
+                                                             quad_node* __restrict__ quads, uint32_t leaf_max)
 {
     const uint32_t npairs = (uint32_t)(n - 1);
     const size_t c0 = (size_t)blockIdx.x * SUPER;
     const size_t c1 = min(n, c0 + (size_t)SUPER);
+    __shared__ uint32_t s_done[DONE_CAP];
+    __shared__ uint32_t s_ndone;
+    if (threadIdx.x == 0) s_ndone = 0u;
+    __syncthreads();
     const uint32_t count = min(bucket_count[blockIdx.x], SUPER);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
         climb_global(ready[c0 + i], npairs, c0, c1, false, pairs, flags, heights, ref_nodes, root_box, depth_out,
-                     top, top_count);
+                     top, top_count, quads, leaf_max, s_done, &s_ndone);
+    quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
 }
 
 // pass 3: one workgroup finishes the nodes that span SUPER blocks
@@ -699,13 +758,19 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, pair_node
                                                              float* __restrict__ root_box,
                                                              uint32_t* __restrict__ depth_out,
                                                              const uint32_t* __restrict__ top,
-                                                             const uint32_t* __restrict__ top_count)
+                                                             const uint32_t* __restrict__ top_count,
+                                                             quad_node* __restrict__ quads, uint32_t leaf_max)
 {
     const uint32_t npairs = (uint32_t)(n - 1);
+    __shared__ uint32_t s_done[DONE_CAP];
+    __shared__ uint32_t s_ndone;
+    if (threadIdx.x == 0) s_ndone = 0u;
+    __syncthreads();
     const uint32_t count = min(*top_count, npairs);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
         climb_global(top[i], npairs, 0, n, true, pairs, flags, heights, ref_nodes, root_box, depth_out, nullptr,
-                     nullptr);
+                     nullptr, quads, leaf_max, s_done, &s_ndone);
+    quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
@@ -797,13 +862,15 @@ void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* 
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
     hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, bs.prims, n, bs.sorted_ids,
                        s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
-                       s.keys0, s.counts, build ? reinterpret_cast<uint32_t*>(bs.indices) : nullptr);
+                       s.keys0, s.counts, build ? reinterpret_cast<uint32_t*>(bs.indices) : nullptr, bs.quads,
+                       LBVH_LEAF_MAX);
     if (n > (size_t)CHUNK)
         hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags,
-                           bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb);
+                           bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb,
+                           bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)SUPER)
         hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags, bs.heights,
-                           ref_nodes, bs.root_box, depth, s.vals0, s.counts + nb);
+                           ref_nodes, bs.root_box, depth, s.vals0, s.counts + nb, bs.quads, LBVH_LEAF_MAX);
 }
 
 }  // namespace
@@ -901,26 +968,30 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
                            bs.pairs, s.leaf_parent);
-    // 5: boxes, traversal-order triangles, reference layout, index list
+    // 5: boxes, traversal-order triangles, reference layout, index list,
+    // and the 4-wide traversal nodes (each pair's quad as soon as its
+    // subtree is complete)
     launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, true, st);
     VR_HIP_CHECK(hipGetLastError());
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
     bs.max_leaf = 1;
+    bs.collapse_leaf_max = LBVH_LEAF_MAX;
 
-    // 6: the 4-wide traversal nodes.  The exact depth stays on the device
-    // (vr_bvh_get_info reads it); 65 bounds every Karras tree, which the
-    // traversal stack covers, so the build never waits on the device.
+    // The exact depth stays on the device (vr_bvh_get_info reads it); 65
+    // bounds every Karras tree, which the traversal stack covers, so the
+    // build never waits on the device.
     bs.depth = n <= 1 ? (uint32_t)n : 65u;
     bs.rotated = false;
     bs.want_rotate = rotate;
     if (rotate && n > 1) {
-        // optional: SAH-driven tree rotations before the collapse
+        // optional: SAH-driven tree rotations, then the quads of the new tree
         uint32_t count = 0;
         rc = lbvh_rotate(bs, LBVH_LEAF_MAX, &count, st);
         if (rc) return rc;
         bs.rotated = count > 0;
+        if (bs.rotated) return collapse_quads(bs, LBVH_LEAF_MAX, st);
     }
-    return collapse_quads(bs, LBVH_LEAF_MAX, st);
+    return VR_OK;
 }
 
 // Refit after the vertices moved: same ids, new boxes (detail/bvh/refit.h).
@@ -936,10 +1007,11 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     if (n > 1)
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags,
                            bs.num_pairs, 0u);
+    // same topology: the bottom-up pass makes the quads from the new boxes
+    // (the greedy choices may move)
     launch_bottom_up(bs, s, bs.ref_nodes, false, st);
     VR_HIP_CHECK(hipGetLastError());
-    // same topology: the traversal nodes take the new boxes
-    return refit_quads(bs, st);
+    return VR_OK;
 }
 
 int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)

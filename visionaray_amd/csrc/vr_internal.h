@@ -538,7 +538,5 @@ inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32
 // (LBVH only: pair e[3] holds the range end) turns inner subtrees of at most
 // leaf_max triangles into one leaf over their contiguous range.
 int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
-// After the pair boxes of `s` were refitted: the quads again from the new boxes.
-int refit_quads(bvh_state& s, hipStream_t st);
 
 }  // namespace vr
