@@ -246,6 +246,55 @@ int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_
 int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits,
                      uint32_t* d_counts, void* stream);
 
+/* ---- instanced two-level BVH ------------------------------------------- */
+/* One instance of a bottom-level BVH (index_bvh_t::inst(mat4x3),
+ * bvh.h:679-782): transform is the mat4x3 as its 12 floats, column by column
+ * (three columns of the 3x3 part, then the translation; mat4x3(T const[12]),
+ * math/detail/matrix4x3.inl:55-61).  As in the reference, the instance maps
+ * object space to world space by the TRANSPOSE of the 3x3 part
+ * (top_left(mat4x3) transposes, matrix4x3.inl:206-218) and then the
+ * translation; inst_id is what set_inst_id stores and what hits report. */
+typedef struct vr_instance {
+    uint32_t blas;    /* index into the BVH list given to vr_tlas_build */
+    int32_t inst_id;
+    float transform[12];
+} vr_instance;
+
+/* hit_record_bvh_inst reduced to what callers read
+ * (detail/bvh/hit_record.h:54-76, math/intersect.h:151-169): prim_id and
+ * geom_id of the triangle hit (-1 on a miss), t, u, v in the instance's
+ * space (t is the same parameter in both spaces), the instance's inst_id and
+ * its position in the instance list. */
+typedef struct vr_inst_hit {
+    int32_t prim_id;
+    float t;
+    float u;
+    float v;
+    int32_t inst_id;
+    int32_t instance;
+    int32_t geom_id;
+    int32_t pad;
+} vr_inst_hit;
+
+typedef struct vr_tlas_t* vr_tlas;
+
+/* The top level over instances of built BVHs: each instance's box is its BVH
+ * root box's eight corners through the instance transform (get_bounds,
+ * detail/bvh/get_bounds.inl:30-56), the tree over the boxes is the
+ * reference's CPU lbvh_builder tree (lbvh.h:500-656, leaves of up to 4
+ * instances), built on the host.  The BVHs must outlive the handle and keep
+ * their arrays (no rebuild to a different size while it lives). */
+int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_instances, uint32_t num_instances,
+                  void* stream, vr_tlas* out);
+/* closest_hit over the top level (detail/bvh/intersect.inl:362-394: the
+ * ray transformed into each instance's space, the instance's BVH traversed,
+ * the closest hit kept). */
+int vr_tlas_closest_hit(vr_tlas tlas, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream);
+/* the top level's reference-layout nodes and instance order (host copies;
+ * num_nodes gets the node count, nodes/indices may be null to query it) */
+int vr_tlas_nodes(vr_tlas tlas, vr_bvh_node* h_nodes, uint32_t* h_indices, size_t* num_nodes);
+int vr_tlas_destroy(vr_tlas tlas);
+
 /* ---- kernels under the scheduler -------------------------------------- */
 /* Ambient occlusion kernel of src/examples/ao/main.cpp:192-260 under the
  * jittered_blend sampler; d_accum/d_color are RGBA32F width*height. */

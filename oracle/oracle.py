@@ -530,3 +530,134 @@ def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, bina
             return json.load(f)
     finally:
         t.close()
+
+
+# ---------------------------------------------------------------------------
+# instanced two-level BVH (bvh.h:679-782, detail/bvh/intersect.inl:362-394)
+# ---------------------------------------------------------------------------
+
+INSTANCE_DTYPE = np.dtype([("blas", "<u4"), ("inst_id", "<i4"), ("transform", "<f4", (12,))])
+INST_HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst_id", "<i4"),
+                           ("geom_id", "<i4")])
+
+
+def inverse3(m):
+    """inverse(matrix<3, 3>) (math/detail/matrix3.inl:300-321) in float32, each
+    product and sum rounded as written; m[r][c], returns the same layout."""
+    f = np.float32
+    m = [[f(m[r][c]) for c in range(3)] for r in range(3)]
+
+    def det2(m00, m01, m10, m11):
+        return f(f(m00 * m11) - f(m10 * m01))
+
+    a00 = det2(m[1][1], m[1][2], m[2][1], m[2][2])
+    a01 = det2(m[1][0], m[1][2], m[2][0], m[2][2])
+    a02 = det2(m[1][0], m[1][1], m[2][0], m[2][1])
+    a10 = det2(m[0][1], m[0][2], m[2][1], m[2][2])
+    a11 = det2(m[0][0], m[0][2], m[2][0], m[2][2])
+    a12 = det2(m[0][0], m[0][1], m[2][0], m[2][1])
+    a20 = det2(m[0][1], m[0][2], m[1][1], m[1][2])
+    a21 = det2(m[0][0], m[0][2], m[1][0], m[1][2])
+    a22 = det2(m[0][0], m[0][1], m[1][0], m[1][1])
+    det = f(f(f(m[0][0] * a00) - f(m[0][1] * a01)) + f(m[0][2] * a02))
+    # matrix<3, 3>(m00, m10, m20, m01, ...): entries column by column
+    return [[f(a00 / det), f(-a10 / det), f(a20 / det)],
+            [f(-a01 / det), f(a11 / det), f(-a21 / det)],
+            [f(a02 / det), f(-a12 / det), f(a22 / det)]]
+
+
+def instance_maps(inst):
+    """(affine_inv, trans_inv) of index_bvh_inst_t(ref, mat4x3) (bvh.h:696-702):
+    inverse(top_left(M)) with top_left(mat4x3)(r, c) = M(c, r), the transpose
+    (math/detail/matrix4x3.inl:206-218), and -column 3."""
+    t = np.asarray(inst["transform"], np.float32)
+    lin = [[t[3 * r + c] for c in range(3)] for r in range(3)]
+    return inverse3(lin), [np.float32(-t[9 + k]) for k in range(3)]
+
+
+def _mat_rows(m, x, y, z):
+    """matrix<3, 3> * vector<3> (matrix3.inl:226-235) over float32 arrays"""
+    return [(m[r][0] * x + m[r][1] * y) + m[r][2] * z for r in range(3)]
+
+
+def instance_box(inst, root_box):
+    """get_bounds(index_bvh_inst_t) (detail/bvh/get_bounds.inl:30-56): the
+    BVH box's corners in compute_vertices order (math/detail/aabb.inl:305-324)
+    through inverse(affine_inv), plus -trans_inv; aabb::insert min / max."""
+    ainv, ti = instance_maps(inst)
+    aff = inverse3(ainv)
+    f = np.float32
+    lo, hi = [f(v) for v in root_box[:3]], [f(v) for v in root_box[3:]]
+    corners = [(hi[0], hi[1], hi[2]), (lo[0], hi[1], hi[2]), (lo[0], lo[1], hi[2]), (hi[0], lo[1], hi[2]),
+               (lo[0], hi[1], lo[2]), (hi[0], hi[1], lo[2]), (hi[0], lo[1], lo[2]), (lo[0], lo[1], lo[2])]
+    blo = [f(np.finfo(np.float32).max)] * 3
+    bhi = [f(-np.finfo(np.float32).max)] * 3
+    for v in corners:
+        w = _mat_rows(aff, *v)
+        for k in range(3):
+            x = f(w[k] + f(-ti[k]))
+            blo[k] = x if x < blo[k] else blo[k]
+            bhi[k] = x if bhi[k] < x else bhi[k]
+    return np.array(blo + bhi, np.float32)
+
+
+def transform_rays(inst, rays):
+    """index_bvh_inst_t::transform_ray (bvh.h:769-776):
+    ori' = affine_inv * (ori + trans_inv), dir' = affine_inv * dir."""
+    ainv, ti = instance_maps(inst)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = rays.copy()
+    o = rays["ori"].astype(np.float32)
+    d = rays["dir"].astype(np.float32)
+    ox, oy, oz = o[:, 0] + ti[0], o[:, 1] + ti[1], o[:, 2] + ti[2]
+    oo = _mat_rows(ainv, ox, oy, oz)
+    dd = _mat_rows(ainv, d[:, 0], d[:, 1], d[:, 2])
+    out["ori"] = np.stack(oo, axis=1)
+    out["dir"] = np.stack(dd, axis=1)
+    return out
+
+
+def trace_instanced(blas, instances, rays):
+    """Closest hit over instances: for each instance the rays moved into its
+    space and traced through its bottom level (trace: the reference's
+    intersect over index_bvh), the nearest hit kept (is_closer: strictly
+    smaller t).  blas = [(nodes, tris, indices)].  Instances are visited in
+    list order here; the reference visits them in its top-level order, so
+    only rays that hit two instances at exactly the same t can differ (the
+    callers count them)."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = np.zeros(rays.size, INST_HIT_DTYPE)
+    out["prim_id"] = -1
+    out["t"] = np.finfo(np.float32).max
+    out["inst_id"] = -1
+    out["geom_id"] = -1
+    ties = np.zeros(rays.size, bool)
+    for inst in np.ascontiguousarray(instances, INSTANCE_DTYPE):
+        nodes, tris, idx = blas[int(inst["blas"])]
+        h = trace(nodes, tris, idx, transform_rays(inst, rays))
+        hit = h["prim_id"] >= 0
+        ties |= hit & (h["t"] == out["t"]) & (out["prim_id"] >= 0)
+        better = hit & (h["t"] < out["t"])
+        out["prim_id"][better] = h["prim_id"][better]
+        out["t"][better] = h["t"][better]
+        out["u"][better] = h["u"][better]
+        out["v"][better] = h["v"][better]
+        out["inst_id"][better] = inst["inst_id"]
+        gid = {int(p): int(g) for p, g in zip(tris["prim_id"], tris["geom_id"])}
+        out["geom_id"][better] = [gid[int(p)] for p in h["prim_id"][better]]
+    return out, ties
+
+
+def ref_inst(tris, offsets, instances, rays):
+    """The reference's two-level BVH (vr_ref inst): CPU lbvh_builder per
+    bottom level and over the instances -> (closest hits, top-level nodes,
+    top-level instance order)."""
+    t = _Tmp()
+    try:
+        _run(["inst", t.put("tris.bin", tris), tris.size, t.put("of.bin", np.asarray(offsets, np.uint32)),
+              len(offsets) - 1, t.put("inst.bin", np.ascontiguousarray(instances, INSTANCE_DTYPE)), len(instances),
+              t.put("rays.bin", rays), rays.size, t.path("out.bin"), t.path("nodes.bin"), t.path("idx.bin")])
+        return (np.fromfile(t.path("out.bin"), INST_HIT_DTYPE), np.fromfile(t.path("nodes.bin"), NODE_DTYPE),
+                np.fromfile(t.path("idx.bin"), np.uint32))
+    finally:
+        t.close()

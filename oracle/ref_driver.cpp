@@ -22,6 +22,11 @@
 //   sah     TRIS N OUTNODES OUTIDX      binned_sah_builder (no spatial splits)
 //   sahcost NODES NN TRIS NP IDX OUT     sah_cost(tree) (statistics.h:38-68), default costs
 //   rotate  NODES NN TRIS NP IDX OUTNODES OUTCOUNT   bvh_optimizer::optimize_tree_rotations
+//   inst    TRIS NP BLASOF NB INST NI RAYS NR OUT OUTNODES OUTIDX
+//                                       two-level BVH: a CPU lbvh_builder tree per bottom
+//                                       level (triangles BLASOF[b]..BLASOF[b+1]), index_bvh_inst_t per
+//                                       instance ({blas, inst_id, mat4x3 as 12 floats}), lbvh_builder over
+//                                       the instances, closest_hit -> {prim_id, t, u, v, inst_id, geom_id}
 //   bench   ...                         CPU baseline timing (see below)
 
 #include <algorithm>
@@ -505,6 +510,46 @@ int main(int argc, char** argv)
         int count = opt.optimize_tree_rotations(tree, pool);
         save(a[5], tree.nodes().data(), tree.nodes().size());
         save(a[6], &count, 1);
+        return 0;
+    }
+    if (mode == "inst" && na == 11) {
+        struct inst_in { uint32_t blas; int32_t inst_id; float m[12]; };
+        struct inst_out { int32_t prim_id; float t, u, v; int32_t inst_id, geom_id; };
+        size_t np = std::strtoull(a[1], nullptr, 10);
+        size_t nb = std::strtoull(a[3], nullptr, 10);
+        size_t ni = std::strtoull(a[5], nullptr, 10);
+        size_t nr = std::strtoull(a[7], nullptr, 10);
+        auto tris = load<tri_t>(a[0], np);
+        auto of = load<uint32_t>(a[2], nb + 1);
+        auto ins = load<inst_in>(a[4], ni);
+        auto rays = load<basic_ray<float>>(a[6], nr);
+        lbvh_builder b;
+        std::vector<index_bvh<tri_t>> blas;
+        blas.reserve(nb);
+        for (size_t k = 0; k < nb; ++k) blas.push_back(b.build(index_bvh<tri_t>{}, tris.data() + of[k], of[k + 1] - of[k]));
+        using inst_t = index_bvh<tri_t>::bvh_inst;
+        std::vector<inst_t> insts;
+        for (size_t i = 0; i < ni; ++i) {
+            auto x = blas[ins[i].blas].inst(mat4x3(ins[i].m));
+            x.set_inst_id(ins[i].inst_id);
+            insts.push_back(x);
+        }
+        auto top = b.build(index_bvh<inst_t>{}, insts.data(), ni);
+        std::vector<index_bvh<inst_t>::bvh_ref> tops{top.ref()};
+        std::vector<inst_out> out(nr);
+        for (size_t i = 0; i < nr; ++i) {
+            auto hr = closest_hit(rays[i], tops.data(), tops.data() + 1);
+            inst_out& o = out[i];
+            o.prim_id = hr.hit ? (int32_t)hr.prim_id : -1;
+            o.t = hr.hit ? hr.t : FLT_MAX;
+            o.u = hr.hit ? hr.u : 0.0f;
+            o.v = hr.hit ? hr.v : 0.0f;
+            o.inst_id = hr.hit ? (int32_t)hr.inst_id : -1;
+            o.geom_id = hr.hit ? (int32_t)hr.geom_id : -1;
+        }
+        save(a[8], out.data(), out.size());
+        save(a[9], top.nodes().data(), top.nodes().size());
+        save(a[10], top.indices().data(), top.indices().size());
         return 0;
     }
     if (mode == "lbvhcpu" && na == 4) {

@@ -91,6 +91,7 @@ def main():
                         pt_params=pt_p, pt=pt)
     nee()
     quality()
+    instances()
 
 
 def quality():
@@ -120,6 +121,24 @@ def nee():
     img = O.ref_ptl(nodes, tris, idx, mats, lights, cam_in, p)
     np.savez_compressed(OUT / "nee.npz", tris=tris, mats=mats, lights=lights, cam_in=cam_in,
                         cam_basis=O.ref_camera(cam_in), params=p, pt=img)
+
+
+def instances():
+    """The reference's two-level BVH (index_bvh_inst_t instances of CPU
+    lbvh_builder trees, lbvh_builder over the instances): closest hits of
+    random rays, the top-level nodes and order, and each instance's box (the
+    top-level root box of that instance alone)."""
+    blas, inst = S.instanced_scene()
+    tris = np.concatenate(blas)
+    offsets = np.cumsum([0] + [b.size for b in blas]).astype(np.uint32)
+    rays = S.instanced_rays(4096, seed=12)
+    hits, top_nodes, top_idx = O.ref_inst(tris, offsets, inst, rays)
+    boxes = np.zeros((inst.size, 6), np.float32)
+    for i in range(inst.size):
+        _, nodes1, _ = O.ref_inst(tris, offsets, inst[i:i + 1], rays[:1])
+        boxes[i, :3], boxes[i, 3:] = nodes1[0]["lo"], nodes1[0]["hi"]
+    np.savez_compressed(OUT / "instances.npz", tris=tris, offsets=offsets, instances=inst, rays=rays, hits=hits,
+                        top_nodes=top_nodes, top_indices=top_idx, boxes=boxes)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,89 @@
+"""Instanced two-level BVH on the GPU (vr_tlas_build / vr_tlas_closest_hit):
+the top level node for node and the closest hits bit for bit against the
+reference's own CPU code (tests/golden/instances.npz) and the oracle, plus
+identity instances against the single-level traversal."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from visionaray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def build(V, blas_tris, instances):
+    blas = [V.lbvh_builder().build(t, keep_nodes=False) for t in blas_tris]
+    return blas, V.tlas(blas, instances)
+
+
+def golden(V):
+    g = load_golden("instances")
+    of = g["offsets"]
+    return g, [g["tris"][of[k]:of[k + 1]].copy() for k in range(of.size - 1)]
+
+
+def compare(got, want, fields=("prim_id", "t", "u", "v", "inst_id", "geom_id")):
+    for k in fields:
+        assert np.array_equal(got[k], want[k]), f"{k}: {int(np.sum(got[k] != want[k]))} rays differ"
+
+
+def test_top_level_is_the_reference_tree(V):
+    g, blas_tris = golden(V)
+    print(f"[inst] {g['instances'].size} instances", flush=True)
+    blas, t = build(V, blas_tris, g["instances"])
+    nodes, idx = t.nodes()
+    assert nodes.tobytes() == g["top_nodes"].tobytes()
+    assert np.array_equal(idx, g["top_indices"])
+
+
+def test_closest_hits_match_reference(V):
+    g, blas_tris = golden(V)
+    blas, t = build(V, blas_tris, g["instances"])
+    hits = t.closest_hit(g["rays"])
+    assert int((hits["prim_id"] >= 0).sum()) > 500
+    compare(hits, g["hits"])
+    # the instance's position in the list agrees with its id
+    hit = hits["prim_id"] >= 0
+    assert np.array_equal(g["instances"]["inst_id"][hits["instance"][hit]], hits["inst_id"][hit])
+
+
+@pytest.mark.parametrize("count,seed", [(1, 21), (5, 22), (300, 23)])
+def test_closest_hits_match_oracle(V, O, count, seed):
+    blas_tris, inst = S.instanced_scene(num_instances=count, seed=seed)
+    rays = S.instanced_rays(20000, seed=seed + 100)
+    print(f"[inst] {count} instances, {rays.size} rays", flush=True)
+    blas, t = build(V, blas_tris, inst)
+    got = t.closest_hit(rays)
+    trees = []
+    for tr in blas_tris:
+        nodes, idx, _, _ = O.lbvh_build(tr)
+        trees.append((nodes, tr, idx))
+    want, ties = O.trace_instanced(trees, inst, rays)
+    assert int(ties.sum()) == 0
+    assert int((want["prim_id"] >= 0).sum()) > 0
+    compare(got, want)
+
+
+def test_identity_instances_equal_the_plain_bvh(V):
+    tris = S.config0_scene(level=3)
+    rays = S.instanced_rays(8192, seed=31, extent=0.9)
+    plain = V.closest_hit(V.lbvh_builder().build(tris), rays)
+    eye = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32)
+    blas, t = build(V, [tris], V.make_instances([0], [eye]))
+    got = t.closest_hit(rays)
+    for k in ("prim_id", "t", "u", "v"):
+        assert np.array_equal(got[k], plain[k]), k
+    assert np.all(got["inst_id"][got["prim_id"] >= 0] == 0)
+
+
+def test_no_instances_and_misses(V):
+    g, blas_tris = golden(V)
+    blas, t = build(V, blas_tris, g["instances"][:0])
+    hits = t.closest_hit(g["rays"][:64])
+    assert np.all(hits["prim_id"] == -1) and np.all(hits["inst_id"] == -1) and np.all(hits["instance"] == -1)
+    # rays pointing away from every instance
+    blas, t = build(V, blas_tris, g["instances"][:3])
+    r = g["rays"][:16].copy()
+    r["ori"] = (100.0, 100.0, 100.0)
+    r["dir"] = (0.0, 0.0, 1.0)
+    assert np.all(t.closest_hit(r)["prim_id"] == -1)

@@ -31,6 +31,7 @@ __all__ = [
     "closest_hit", "any_hit", "trace", "sah_cost_host", "optimize_tree_rotations", "pinhole_camera", "make_kernel_params", "make_sched_params",
     "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
     "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "LIGHT_DTYPE", "synchronize", "model",
+    "tlas", "make_instances", "INSTANCE_DTYPE", "INST_HIT_DTYPE",
 ]
 
 _initialized = False
@@ -310,6 +311,67 @@ def closest_hit(bvh: BVH, rays, stream=None) -> np.ndarray:
 
 def any_hit(bvh: BVH, rays, stream=None) -> np.ndarray:
     return trace(bvh, rays, True, stream=stream)
+
+
+INSTANCE_DTYPE = _lib.INSTANCE_DTYPE
+INST_HIT_DTYPE = _lib.INST_HIT_DTYPE
+
+
+def make_instances(blas_ids, transforms, inst_ids=None) -> np.ndarray:
+    """INSTANCE_DTYPE records: transforms are (n, 12) mat4x3 floats, column by
+    column (three columns of the 3x3 part, then the translation)."""
+    blas_ids = np.asarray(blas_ids, np.uint32).reshape(-1)
+    t = np.asarray(transforms, np.float32).reshape(-1, 12)
+    out = np.zeros(blas_ids.size, INSTANCE_DTYPE)
+    out["blas"] = blas_ids
+    out["inst_id"] = np.arange(blas_ids.size, dtype=np.int32) if inst_ids is None else inst_ids
+    out["transform"] = t
+    return out
+
+
+class tlas:
+    """Instanced two-level BVH: index_bvh<index_bvh_inst_t> over instances of
+    built BVHs (bvh.h:679-782), the top level built like the reference's CPU
+    lbvh_builder; closest_hit follows detail/bvh/intersect.inl:362-394."""
+
+    def __init__(self, blas: list, instances: np.ndarray, stream=None):
+        self.blas = list(blas)  # the bottom levels must outlive the top level
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        handles = (C.c_void_p * max(len(self.blas), 1))(*[b.handle.value for b in self.blas])
+        h = C.c_void_p()
+        check(lib().vr_tlas_build(C.cast(handles, C.c_void_p), len(self.blas), ptr(inst), inst.size,
+                                  _stream(stream), C.byref(h)))
+        self.handle = h
+        self.num_instances = int(inst.size)
+
+    def closest_hit(self, rays, stream=None) -> np.ndarray:
+        r = _as_device(rays, RAY_DTYPE)
+        hits = DeviceArray(r.count, INST_HIT_DTYPE)
+        check(lib().vr_tlas_closest_hit(self.handle, r.ptr, r.count, hits.ptr, _stream(stream)))
+        return hits.to_host()
+
+    def closest_hit_device(self, rays: DeviceArray, hits: DeviceArray, stream=None):
+        check(lib().vr_tlas_closest_hit(self.handle, rays.ptr, rays.count, hits.ptr, _stream(stream)))
+
+    def nodes(self):
+        """(reference-layout top-level nodes, instance order)"""
+        n = C.c_size_t()
+        check(lib().vr_tlas_nodes(self.handle, None, None, C.byref(n)))
+        nodes = np.zeros(max(n.value, 1), NODE_DTYPE)
+        idx = np.zeros(max(self.num_instances, 1), np.uint32)
+        check(lib().vr_tlas_nodes(self.handle, ptr(nodes), ptr(idx), C.byref(n)))
+        return nodes[: n.value].copy(), idx[: self.num_instances].copy()
+
+    def destroy(self):
+        if self.handle and self.handle.value:
+            lib().vr_tlas_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def pinhole_camera(eye, center, up, fovy: float, aspect: float) -> np.ndarray:

@@ -59,6 +59,10 @@ SIGNATURES = {
     "vr_sah_build": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_lbvh_build_host": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_trace": (I, [P, I, P, SZ, P, P]),
+    "vr_tlas_build": (I, [P, U32, P, U32, P, C.POINTER(P)]),
+    "vr_tlas_closest_hit": (I, [P, P, SZ, P, P]),
+    "vr_tlas_nodes": (I, [P, P, P, C.POINTER(SZ)]),
+    "vr_tlas_destroy": (I, [P]),
     "vr_trace_counted": (I, [P, I, P, SZ, P, P, P]),
     "vr_render_ao": (I, [P, P, P, U32, F, P, P, P, P]),
     "vr_render_pt": (I, [P, P, SZ, P, P, P, P, P, P, P]),
@@ -126,6 +130,10 @@ OBJ_MATERIAL_DTYPE = np.dtype([("ca", "<f4", (3,)), ("cd", "<f4", (3,)), ("cs", 
 MESH_INFO_DTYPE = np.dtype([("num_prims", "<u8"), ("num_vertices", "<u8"), ("num_shading_normals", "<u8"),
                             ("num_tex_coords", "<u8"), ("num_materials", "<u8"), ("num_degenerate", "<u8"),
                             ("num_missing", "<u8"), ("bbox_lo", "<f4", (3,)), ("bbox_hi", "<f4", (3,))])
+INSTANCE_DTYPE = np.dtype([("blas", "<u4"), ("inst_id", "<i4"), ("transform", "<f4", (12,))])
+INST_HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst_id", "<i4"),
+                           ("instance", "<i4"), ("geom_id", "<i4"), ("pad", "<i4")])
+assert INSTANCE_DTYPE.itemsize == 56 and INST_HIT_DTYPE.itemsize == 32
 assert OBJ_MATERIAL_DTYPE.itemsize == 72 and MESH_INFO_DTYPE.itemsize == 80
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE_DTYPE.itemsize == 32
 assert CAMERA_DTYPE.itemsize == 48 and KERNEL_PARAMS_DTYPE.itemsize == 32 and SCHED_PARAMS_DTYPE.itemsize == 24

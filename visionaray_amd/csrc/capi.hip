@@ -96,9 +96,6 @@ int current_device()
 
 using namespace vr;
 
-struct vr_bvh_t {
-    bvh_state s;
-};
 
 struct vr_batch_t {
     int device = 0;

@@ -256,29 +256,20 @@ int lbvh_find_split(const std::vector<code_ref>& r, int first, int last)
 
 }  // namespace
 
-int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
-                    uint32_t* indices)
+namespace {
+
+// init (detail/bvh/lbvh.h:545-599) from the primitive boxes: scene and
+// centroid boxes, Morton codes, std::stable_sort by code; then the top-down
+// splits
+int lbvh_build_boxes(const std::vector<box>& pb, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                     uint32_t* indices)
 {
-    *num_nodes = 0;
-    if (n == 0) return VR_OK;
-    VR_CHECK(n < (size_t)1 << 31, VR_ERR_UNSUPPORTED, "vr_lbvh_build_host: too many primitives");
-    if (max_leaf_size <= 0) max_leaf_size = 4;
-    VR_CHECK(max_leaf_size <= 65535, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build_host: leaf size exceeds bvh_node range");
-    // init (detail/bvh/lbvh.h:545-599): primitive boxes, scene and centroid
-    // boxes, Morton codes, std::stable_sort by code
-    std::vector<box> pb(n);
+    const size_t n = pb.size();
     box scene, cent;
     std::vector<f3> cs(n);
     for (size_t i = 0; i < n; ++i) {
-        const vr_triangle& t = prims[i];
-        const f3 v1 = mk3(t.v1[0], t.v1[1], t.v1[2]);
-        box b;
-        b.insert(v1);
-        b.insert(add3(v1, mk3(t.e1[0], t.e1[1], t.e1[2])));
-        b.insert(add3(v1, mk3(t.e2[0], t.e2[1], t.e2[2])));
-        pb[i] = b;
-        scene.insert(b);
-        cs[i] = b.center();
+        scene.insert(pb[i]);
+        cs[i] = pb[i].center();
         cent.insert(cs[i]);
     }
     const f3 center = cent.center(), size = cent.size();
@@ -321,6 +312,46 @@ int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bv
     }
     *num_nodes = node_count;
     return VR_OK;
+}
+
+}  // namespace
+
+int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                    uint32_t* indices)
+{
+    *num_nodes = 0;
+    if (n == 0) return VR_OK;
+    VR_CHECK(n < (size_t)1 << 31, VR_ERR_UNSUPPORTED, "vr_lbvh_build_host: too many primitives");
+    if (max_leaf_size <= 0) max_leaf_size = 4;
+    VR_CHECK(max_leaf_size <= 65535, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_build_host: leaf size exceeds bvh_node range");
+    std::vector<box> pb(n);
+    for (size_t i = 0; i < n; ++i) {
+        // get_bounds(triangle) (math/detail/triangle.inl:41-53)
+        const vr_triangle& t = prims[i];
+        const f3 v1 = mk3(t.v1[0], t.v1[1], t.v1[2]);
+        box b;
+        b.insert(v1);
+        b.insert(add3(v1, mk3(t.e1[0], t.e1[1], t.e1[2])));
+        b.insert(add3(v1, mk3(t.e2[0], t.e2[1], t.e2[2])));
+        pb[i] = b;
+    }
+    return lbvh_build_boxes(pb, max_leaf_size, nodes, num_nodes, indices);
+}
+
+int lbvh_build_host_boxes(const float* boxes, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
+                          uint32_t* indices)
+{
+    *num_nodes = 0;
+    if (n == 0) return VR_OK;
+    VR_CHECK(n < (size_t)1 << 31, VR_ERR_UNSUPPORTED, "lbvh over boxes: too many primitives");
+    if (max_leaf_size <= 0) max_leaf_size = 4;
+    VR_CHECK(max_leaf_size <= 65535, VR_ERR_INVALID_ARGUMENT, "lbvh over boxes: leaf size exceeds bvh_node range");
+    std::vector<box> pb(n);
+    for (size_t i = 0; i < n; ++i) {
+        pb[i].lo = mk3(boxes[6 * i + 0], boxes[6 * i + 1], boxes[6 * i + 2]);
+        pb[i].hi = mk3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]);
+    }
+    return lbvh_build_boxes(pb, max_leaf_size, nodes, num_nodes, indices);
 }
 
 // ---------------------------------------------------------------------------

@@ -424,3 +424,62 @@ def config4_scenes(count: int = 32, nmin: int = 50_000, nmax: int = 1_000_000, s
         d /= np.linalg.norm(d)
         yield dict(tris=tris, mats=mats, camera=(tuple(c + d * 3.0), tuple(c), (0.0, 1.0, 0.0), np.float32(0.8), 1.0),
                    size=(256, 256))
+
+
+RAY_DTYPE = np.dtype([("ori", "<f4", (3,)), ("dir", "<f4", (3,)), ("tmin", "<f4"), ("tmax", "<f4")])
+
+# instance records of a two-level BVH: bottom-level index, id, mat4x3 as 12
+# floats column by column (three columns of the 3x3 part, then translation)
+INSTANCE_DTYPE = np.dtype([("blas", "<u4"), ("inst_id", "<i4"), ("transform", "<f4", (12,))])
+
+
+def random_transform(rng, scale_range=(0.5, 1.5), shear: float = 0.2, extent: float = 2.0) -> np.ndarray:
+    """A random affine mat4x3 (12 floats): rotation x per-axis scale (one in
+    four mirrored) plus a small shear, and a translation in [-extent, extent]^3."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    rot = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                    [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                    [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    s = rng.uniform(*scale_range, size=3)
+    if rng.uniform() < 0.25:
+        s[0] = -s[0]
+    m = rot @ np.diag(s) + rng.uniform(-shear, shear, size=(3, 3))
+    t = rng.uniform(-extent, extent, size=3)
+    return np.concatenate([m[:, 0], m[:, 1], m[:, 2], t]).astype(np.float32)
+
+
+def instanced_scene(num_instances: int = 24, seed: int = 11, sphere_level: int = 2, patch_cells: int = 10,
+                    soup: int = 120):
+    """Three bottom levels (an icosphere, a height-field patch, a triangle
+    soup) and `num_instances` instances of them under random affine
+    transforms -> (list of triangle arrays, instances)."""
+    rng = np.random.default_rng(seed)
+    b0 = icosphere(sphere_level, 0.5)
+    b1 = heightfield(cells=patch_cells, seed=seed, extent=0.6, amplitude=0.2)
+    c = rng.uniform(-0.5, 0.5, size=(soup, 3))
+    e1 = rng.normal(scale=0.08, size=(soup, 3))
+    e2 = rng.normal(scale=0.08, size=(soup, 3))
+    b2 = make_triangles(c, c + e1, c + e2)
+    blas = [b0, b1, b2]
+    for k, b in enumerate(blas):
+        b["geom_id"] = k
+    inst = np.zeros(num_instances, INSTANCE_DTYPE)
+    inst["blas"] = rng.integers(0, len(blas), size=num_instances)
+    inst["inst_id"] = 100 + np.arange(num_instances)
+    for i in range(num_instances):
+        inst["transform"][i] = random_transform(rng)
+    return blas, inst
+
+
+def instanced_rays(n: int, seed: int, extent: float = 3.0) -> np.ndarray:
+    """Rays from points in [-extent, extent]^3 in uniform directions (unit
+    length), tmax = FLT_MAX."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY_DTYPE)
+    r["ori"] = rng.uniform(-extent, extent, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r["tmax"] = np.finfo(np.float32).max
+    return r
