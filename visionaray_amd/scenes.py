@@ -483,3 +483,31 @@ def instanced_rays(n: int, seed: int, extent: float = 3.0) -> np.ndarray:
     r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
     r["tmax"] = np.finfo(np.float32).max
     return r
+
+
+def config1_instanced(cells: int = 1448, spheres: int = 2000, sphere_level: int = 3):
+    """The C1 scene as a two-level BVH: the height field as one identity
+    instance, and one unit level-3 icosphere instanced `spheres` times with the
+    radii and centres config1_scene draws (same seeds) -> (bottom-level
+    triangle arrays, instances).  World vertices equal config1_scene's up to
+    float rounding (radius x vertex + centre is evaluated per ray here)."""
+    hf = heightfield(cells, seed=1)
+    v = np.concatenate([hf["v1"], hf["v1"] + hf["e1"], hf["v1"] + hf["e2"]])
+    lo, hi = v.min(0), v.max(0)
+    hi = hi.copy()
+    hi[1] += 0.1
+    rng = np.random.default_rng(2)  # the draws of icosphere_cloud(..., seed=2)
+    uv, f = icosphere_mesh(sphere_level)
+    radii = np.exp(rng.uniform(np.log(0.005), np.log(0.05), size=spheres))
+    centers = rng.uniform(np.asarray(lo, np.float64), np.asarray(hi, np.float64), size=(spheres, 3))
+    unit = uv.astype(np.float32)[f]
+    sphere = make_triangles(unit[:, 0], unit[:, 1], unit[:, 2], geom_id=1)
+    inst = np.zeros(spheres + 1, INSTANCE_DTYPE)
+    inst["transform"][0] = [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]
+    inst["blas"][1:] = 1
+    inst["inst_id"] = np.arange(spheres + 1)
+    t = np.zeros((spheres, 12), np.float32)
+    t[:, 0] = t[:, 4] = t[:, 8] = radii
+    t[:, 9:12] = centers
+    inst["transform"][1:] = t
+    return [hf, sphere], inst
