@@ -561,7 +561,7 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
                               threads=threads)
         log(f"cpu baseline: reference tiled_sched, {threads} threads, {w}x{h} at {args.cpu_spp} spp")
         t0 = time.time()
-        r = O.ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, timeout=900)
+        r = O.ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, timeout=300, stall=60.0)
         log(f"cpu baseline done in {time.time() - t0:.1f}s: frame {r['frame_s_median']:.3f}s, "
             f"build {r['build_s']:.2f}s")
         frame_s = r["frame_s_median"]
@@ -578,6 +578,7 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
             "total_mrays_per_s": round((n_primary + n_secondary) / frame_s / 1e6, 4),
             "cpu_lbvh_build_s": round(r["build_s"], 3),
             "binary": O.ref_binary().name,
+            **({"stopped": r["stopped"], "timed_frames": r["frames"]} if "stopped" in r else {}),
         }
     return {"value": None, "error": "oracle/_ref not built"}
 

@@ -513,19 +513,55 @@ def ref_lbvh_cpu(tris):
         t.close()
 
 
-def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, binary=None, timeout=600):
-    """Time the reference CPU path (builder + tiled_sched frames)."""
+def ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, binary=None, timeout=600,
+              stall=60.0):
+    """Time the reference CPU path (builder + tiled_sched frames).  The
+    driver's per-frame progress lines are echoed as they come; a run that
+    prints nothing for `stall` seconds (or outlives `timeout`) is stopped and
+    the timed frames it reported are used."""
+    import re
+    import sys
+    import threading
+    import time as _time
     t = _Tmp()
     try:
         out = t.path("bench.json")
         args = ["bench", t.put("tris.bin", tris), tris.size, t.put("mats.bin", mats), mats.size,
                 t.put("cam.bin", cam_in), t.put("params.bin", params), packet, build, frames, out]
         b = str(binary or ref_binary())
-        # the driver's per-frame progress goes straight to this process's stderr
-        res = subprocess.run([b, *map(str, args)], stdout=subprocess.PIPE, stderr=None, text=True,
-                             timeout=timeout)
-        if res.returncode != 0:
-            raise RuntimeError(f"vr_ref bench failed with status {res.returncode}")
+        proc = subprocess.Popen([b, *map(str, args)], stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+        seen = {"last": _time.time(), "frames": [], "build": None}
+
+        def pump():
+            for line in proc.stderr:
+                sys.stderr.write(line)
+                sys.stderr.flush()
+                seen["last"] = _time.time()
+                m = re.search(r"frame \d+ of \d+ \(timed\): ([0-9.]+) s", line)
+                if m:
+                    seen["frames"].append(float(m.group(1)))
+                m = re.search(r"build of \d+ triangles: ([0-9.]+) s", line)
+                if m:
+                    seen["build"] = float(m.group(1))
+
+        th = threading.Thread(target=pump, daemon=True)
+        th.start()
+        t0 = _time.time()
+        while proc.poll() is None:
+            now = _time.time()
+            if now - seen["last"] > stall or now - t0 > timeout:
+                proc.kill()
+                proc.wait()
+                th.join(timeout=5)
+                if not seen["frames"]:
+                    raise RuntimeError("vr_ref bench stalled before a timed frame")
+                fr = sorted(seen["frames"])
+                return {"build_s": seen["build"] or 0.0, "frame_s_median": fr[len(fr) // 2],
+                        "frames": len(fr), "stopped": f"no progress for {stall:.0f} s"}
+            _time.sleep(0.2)
+        th.join(timeout=5)
+        if proc.returncode != 0:
+            raise RuntimeError(f"vr_ref bench failed with status {proc.returncode}")
         with open(out) as f:
             return json.load(f)
     finally:

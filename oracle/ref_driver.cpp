@@ -601,17 +601,16 @@ int main(int argc, char** argv)
         std::fflush(stderr);
         accum_rt rt(p.w, p.h);
         std::vector<double> ft;
+        // one scheduler for all frames, as the viewer keeps one: its thread
+        // pool notifies its workers without holding their mutex, so a pool
+        // torn down per frame can miss a wake-up at join and never return;
+        // the process leaves with _Exit below for the same reason
+        tiled_sched<basic_ray<simd::float4>> sched4(packet == 4 ? p.threads : 1);
+        tiled_sched<basic_ray<float>> sched1(packet == 4 ? 1 : p.threads);
         for (int f = 0; f < frames + 1; ++f) {
             double s0 = now_s();
-            if (packet == 4) {
-                tiled_sched<basic_ray<simd::float4>> sched(p.threads);
-                s0 = now_s();
-                render_pt<basic_ray<simd::float4>>(sched, tree, mats, cam, p, rt);
-            } else {
-                tiled_sched<basic_ray<float>> sched(p.threads);
-                s0 = now_s();
-                render_pt<basic_ray<float>>(sched, tree, mats, cam, p, rt);
-            }
+            if (packet == 4) render_pt<basic_ray<simd::float4>>(sched4, tree, mats, cam, p, rt);
+            else render_pt<basic_ray<float>>(sched1, tree, mats, cam, p, rt);
             double s1 = now_s();
             if (f > 0) ft.push_back(s1 - s0);
             std::fprintf(stderr, "[cpu-ref] frame %d of %d (%s): %.3f s\n", f, frames, f ? "timed" : "warm-up",
@@ -625,7 +624,9 @@ int main(int argc, char** argv)
                      build_s, med, (int)ft.size(), p.threads, packet, build.c_str());
         std::fclose(f);
         save_rt((std::string(a[9]) + ".img").c_str(), rt);
-        return 0;
+        std::fflush(stdout);
+        std::fflush(stderr);
+        std::_Exit(0);
     }
     std::fprintf(stderr, "bad mode/args: %s (%d args)\n", mode.c_str(), na);
     return 2;
