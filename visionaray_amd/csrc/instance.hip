@@ -6,16 +6,17 @@
 // instance transform (get_bounds, detail/bvh/get_bounds.inl:30-56), and the
 // tree over those boxes is the reference's CPU lbvh_builder tree
 // (host_build.cpp), stored as pair records whose leaves are ranges of
-// instance slots.  On the device one lane follows one ray: it walks the top
-// level with the binary traversal's slab test and, at each instance of a
-// leaf, moves the ray into the instance's space (ori' = A (ori - T),
-// dir' = A dir with A the stored inverse) and runs the 4-wide traversal of
-// that instance's BVH, capped at the closest hit so far.  t is the same
-// parameter in both spaces (the direction is not renormalized), so the
-// closest hit over all instances is the smallest t.
+// instance slots, then collapsed 4-wide like the bottom levels.  On the
+// device one lane follows one ray through both levels in one voted loop: at
+// an instance of a top-level leaf it tests the instance's box and moves the
+// ray into the instance's space (ori' = A (ori - T), dir' = A dir with A the
+// stored inverse) for that instance's BVH.  t is the same parameter in both
+// spaces (the direction is not renormalized), so the closest hit over all
+// instances is the smallest t, and it culls boxes in both levels.
 #include "vr_internal.h"
 
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace vr {
@@ -23,7 +24,6 @@ namespace vr {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int TLAS_STACK = 64;  // deferred top-level nodes; the host checks the depth
 
 // one instance on the device
 struct alignas(16) inst_rec {
@@ -31,13 +31,13 @@ struct alignas(16) inst_rec {
     float ti[3];   // the negated translation
     uint32_t blas;
     int32_t inst_id;
-    uint32_t pad[2];
+    float box[6];  // world box (lo, hi): a leaf's instances are tested one by one before entering
 };
-static_assert(sizeof(inst_rec) == 64, "instance record");
+static_assert(sizeof(inst_rec) == 80, "instance record");
 
 struct tlas_args {
-    const pair_node* pairs;
-    uint32_t num_pairs;
+    const quad_node* quads;  // the top level, 4-wide
+    uint32_t num_quads;
     uint32_t root;
     const uint32_t* slots;  // instance per leaf slot
     uint32_t num_slots;
@@ -97,95 +97,195 @@ __device__ inline f3 row_mul(const float* inv, f3 v)
                inv[6] * v.x + inv[7] * v.y + inv[8] * v.z);
 }
 
+// deferred entries per lane: both levels' stacks and the level markers
+constexpr int TWO_LEVEL_STACK = 2 * MAX_STACK + 8;
+// on the stack: back to the top level (restore the world-space ray)
+constexpr uint32_t TO_TOP = 0xFFFFFFFEu;
+// a lane's next entry when the marker is next on its stack: nothing more at
+// this level (the marker is taken once the lane's triangle range is done)
+constexpr uint32_t LEVEL_END = 0xFFFFFFFDu;
+
+// One voted loop over both levels.  Each iteration the wave either visits
+// nodes (a 4-wide top-level node or a 4-wide bottom-level node, whichever
+// level each lane is in) or works on leaves (a triangle test in an instance,
+// or a top-level instance: its box test and, on a hit, the entry into its
+// space), whichever more lanes need -- the same vote as the one-level
+// traversal, so lanes in different levels still advance together.  Entering
+// an instance pushes the rest of its top-level leaf and a TO_TOP marker; the
+// marker's pop restores the world-space ray.  Boxes and triangles are culled
+// by the closest hit over all instances so far.
 __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
 {
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
     uint32_t* st = stack + threadIdx.x;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= A.n) return;
-    const float4* rp = reinterpret_cast<const float4*>(A.rays + i);
-    const float4 r0 = rp[0], r1 = rp[1];
-    const f3 ori = mk3(r0.x, r0.y, r0.z), dir = mk3(r0.w, r1.x, r1.y);
-    const float tmin = r1.z, tmax = r1.w;
-    const f3 inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
-                       dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
-    float best_t = FLT_MAX, best_u = 0.0f, best_v = 0.0f;
-    int32_t best_tri = -1, best_inst = -1;
-    uint32_t tst[TLAS_STACK];
-    int sp = 0;
-    uint32_t cur = A.root;
-    uint32_t budget = 2u * A.num_slots + 2u;  // each node at most once
-    bool done = A.num_slots == 0;
-    while (!done) {
-        while (!(cur & LEAF_BIT)) {
-            if (--budget == 0 || cur >= A.num_pairs) {
-                done = true;
-                break;
-            }
-            const float4* p = reinterpret_cast<const float4*>(A.pairs + cur);
-            const float4 a0 = p[0], q1 = p[1], q2 = p[2];
-            const uint4 q3 = *reinterpret_cast<const uint4*>(p + 3);
-            float tn0, tf0, tn1, tf1;
-            {
-                const f3 tlo = mul3(sub3(mk3(a0.x, a0.y, a0.z), ori), inv);
-                const f3 thi = mul3(sub3(mk3(a0.w, q1.x, q1.y), ori), inv);
-                tn0 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf0 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            {
-                const f3 tlo = mul3(sub3(mk3(q1.z, q1.w, q2.x), ori), inv);
-                const f3 thi = mul3(sub3(mk3(q2.y, q2.z, q2.w), ori), inv);
-                tn1 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf1 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            const bool h0 = tf0 >= tn0 && tn0 < best_t;
-            const bool h1 = tf1 >= tn1 && tn1 < best_t;
-            if (h0 && h1) {
-                const bool near1 = !(tn0 < tn1);
-                if (sp >= TLAS_STACK) {  // cannot happen for trees the host accepted
-                    done = true;
-                    break;
-                }
-                tst[sp++] = near1 ? q3.x : q3.y;
-                cur = near1 ? q3.y : q3.x;
-            } else if (h0) {
-                cur = q3.x;
-            } else if (h1) {
-                cur = q3.y;
-            } else if (sp == 0) {
-                done = true;
-                break;
-            } else {
-                cur = tst[--sp];
-            }
-        }
-        if (done) break;
-        const uint32_t first = cur & LEAF_FIRST_MASK;
-        const uint32_t count = ((cur >> 26) & 31u) + 1u;
-        if (first + count > A.num_slots) break;  // corrupt entry
-        for (uint32_t k = first; k < first + count; ++k) {
-            const uint32_t ii = A.slots[k];
-            const inst_rec* rec = A.insts + ii;
-            float m[9], ti[3];
-            for (int q = 0; q < 9; ++q) m[q] = rec->inv[q];
-            for (int q = 0; q < 3; ++q) ti[q] = rec->ti[q];
-            // index_bvh_inst_t::transform_ray (bvh.h:769-776)
-            const f3 o2 = row_mul(m, mk3(ori.x + ti[0], ori.y + ti[1], ori.z + ti[2]));
-            const f3 d2 = row_mul(m, dir);
-            const hit_t h = traverse<false, false, LDS_STACK, BLOCK>(A.views[rec->blas], o2, d2, tmin,
-                                                                     tmin_f(tmax, best_t), st, nullptr);
-            // is_closer: strictly nearer, inside [tmin, tmax]
-            if (h.tri >= 0 && h.t < best_t && h.t >= tmin && h.t <= tmax) {
-                best_t = h.t;
-                best_u = h.u;
-                best_v = h.v;
-                best_tri = h.tri;
-                best_inst = (int32_t)ii;
-            }
-        }
-        if (sp == 0) break;
-        cur = tst[--sp];
+    const bool live = i < A.n;
+    f3 ow = mk3(0.0f, 0.0f, 0.0f), dw = mk3(0.0f, 0.0f, 1.0f);
+    float tmin = 0.0f, tmax = 0.0f;
+    if (live) {
+        const float4* rp = reinterpret_cast<const float4*>(A.rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        ow = mk3(r0.x, r0.y, r0.z);
+        dw = mk3(r0.w, r1.x, r1.y);
+        tmin = r1.z;
+        tmax = r1.w;
     }
+    const f3 iw = mk3(dw.x != 0.0f ? 1.0f / dw.x : FLT_MAX, dw.y != 0.0f ? 1.0f / dw.y : FLT_MAX,
+                      dw.z != 0.0f ? 1.0f / dw.z : FLT_MAX);
+    // the current space's ray
+    f3 o = ow, d = dw, iv = iw;
+    bool bottom = false;  // in an instance
+    const quad_node* q = A.quads;
+    const tri_node* tris = nullptr;
+    uint32_t nq = A.num_quads, np = A.num_slots;
+    uint32_t cur = live && A.num_slots > 0 ? A.root : TRAV_DONE;
+    uint32_t lf = 0, ln = 0;
+    int sp = 0;
+    uint32_t spill[TWO_LEVEL_STACK - LDS_STACK];
+    float best_t = FLT_MAX, best_u = 0.0f, best_v = 0.0f;
+    int32_t best_tri = -1, best_inst = -1, cur_inst = -1;
+    uint32_t budget = 1u << 26;  // a guard against corrupt trees only
+    auto push = [&](uint32_t e) {
+        if (sp < LDS_STACK) st[sp * BLOCK] = e;
+        else spill[sp - LDS_STACK] = e;
+        ++sp;
+    };
+    auto top = [&]() -> uint32_t { return sp - 1 < LDS_STACK ? st[(sp - 1) * BLOCK] : spill[sp - 1 - LDS_STACK]; };
+    // the next deferred entry of this level (LEVEL_END at the marker)
+    auto pop = [&]() -> uint32_t {
+        if (sp == 0) return TRAV_DONE;
+        const uint32_t e = top();
+        if (e == TO_TOP) return LEVEL_END;
+        --sp;
+        return e;
+    };
+    for (;;) {
+        // the instance is done (its range too): back to the top level
+        if (ln == 0 && cur == LEVEL_END) {
+            --sp;  // the marker
+            o = ow;
+            d = dw;
+            iv = iw;
+            bottom = false;
+            q = A.quads;
+            nq = A.num_quads;
+            np = A.num_slots;
+            cur = pop();
+        }
+        // a leaf entry becomes the lane's range: triangles in an instance,
+        // instance slots at the top
+        if (ln == 0 && (cur & LEAF_BIT) && cur != TRAV_DONE && cur != LEVEL_END) {
+            lf = cur & LEAF_FIRST_MASK;
+            ln = ((cur >> 26) & 31u) + 1u;
+            if (lf + ln > np) {  // corrupt entry
+                ln = 0;
+                cur = TRAV_DONE;
+                sp = 0;
+            } else {
+                cur = pop();
+            }
+        }
+        const bool node_work = !(cur & LEAF_BIT);
+        const bool active = node_work || ln > 0;
+        if (__ballot(active) == 0) break;
+        if (active && --budget == 0) {
+            cur = TRAV_DONE;
+            ln = 0;
+            sp = 0;
+            continue;
+        }
+        const uint32_t n_node = (uint32_t)__popcll(__ballot(node_work));
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(ln > 0));
+        if ((float)n_node >= VR_TRAV_NODE_BIAS * (float)n_leaf) {
+            if (!node_work) continue;
+            if (cur >= nq) {  // corrupt entry
+                cur = TRAV_DONE;
+                ln = 0;
+                sp = 0;
+                continue;
+            }
+            const float4* p = reinterpret_cast<const float4*>(q + cur);
+            const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+            const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
+            float k0 = slab_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, iv, tmin, tmax, best_t);
+            float k1 = slab_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, iv, tmin, tmax, best_t);
+            float k2 = slab_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, iv, tmin, tmax, best_t);
+            float k3 = slab_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, iv, tmin, tmax, best_t);
+            uint32_t e0 = ch.x, e1 = ch.y, e2 = ch.z, e3 = ch.w;
+            cswap(k0, e0, k1, e1);
+            cswap(k2, e2, k3, e3);
+            cswap(k0, e0, k2, e2);
+            cswap(k1, e1, k3, e3);
+            cswap(k1, e1, k2, e2);
+            if (!(k0 < __builtin_inff())) {
+                cur = pop();
+            } else {
+                if (sp + 3 > TWO_LEVEL_STACK) {  // cannot happen for trees the host accepted
+                    cur = TRAV_DONE;
+                    ln = 0;
+                    sp = 0;
+                    continue;
+                }
+                if (k3 < __builtin_inff()) push(e3);
+                if (k2 < __builtin_inff()) push(e2);
+                if (k1 < __builtin_inff()) push(e1);
+                cur = e0;
+            }
+            continue;
+        }
+        if (ln == 0) continue;
+        const uint32_t k = lf++;
+        --ln;
+        if (bottom) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + k);
+            const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+            float t, u, v;
+            if (intersect_tri(o, d, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v) &&
+                t < best_t && t >= tmin && t <= tmax) {
+                best_t = t;
+                best_u = u;
+                best_v = v;
+                best_tri = (int32_t)k;
+                best_inst = cur_inst;
+            }
+            continue;
+        }
+        // an instance of the top-level leaf: its own box, then its space
+        const uint32_t ii = A.slots[k];
+        const inst_rec* rec = A.insts + ii;
+        if (!(slab_enter(rec->box[0], rec->box[1], rec->box[2], rec->box[3], rec->box[4], rec->box[5], ow, iw, tmin,
+                         tmax, best_t) < __builtin_inff()))
+            continue;
+        if (sp + 3 > TWO_LEVEL_STACK) {
+            cur = TRAV_DONE;
+            ln = 0;
+            sp = 0;
+            continue;
+        }
+        // the rest of this leaf waits under the marker; cur (deferred at the
+        // top level) goes below both
+        if (cur != TRAV_DONE) push(cur);
+        if (ln > 0) push(leaf_entry(lf, ln));
+        push(TO_TOP);
+        ln = 0;
+        float m[9], ti[3];
+        for (int c = 0; c < 9; ++c) m[c] = rec->inv[c];
+        for (int c = 0; c < 3; ++c) ti[c] = rec->ti[c];
+        // index_bvh_inst_t::transform_ray (bvh.h:769-776)
+        o = row_mul(m, mk3(ow.x + ti[0], ow.y + ti[1], ow.z + ti[2]));
+        d = row_mul(m, dw);
+        iv = mk3(d.x != 0.0f ? 1.0f / d.x : FLT_MAX, d.y != 0.0f ? 1.0f / d.y : FLT_MAX,
+                 d.z != 0.0f ? 1.0f / d.z : FLT_MAX);
+        const bvh_view& bv = A.views[rec->blas];
+        bottom = true;
+        q = bv.quads;
+        tris = bv.tris;
+        nq = bv.num_quads;
+        np = bv.num_prims;
+        cur = bv.root;
+        cur_inst = (int32_t)ii;
+    }
+    if (!live) return;
     vr_inst_hit out;
     out.pad = 0;
     if (best_tri >= 0) {
@@ -221,7 +321,7 @@ struct vr_tlas_t {
     uint32_t num_instances = 0;
     uint32_t num_pairs = 0;
     uint32_t root = 0;
-    pair_node* pairs = nullptr;
+    quad_node* quads = nullptr;
     uint32_t* slots = nullptr;
     inst_rec* insts = nullptr;
     bvh_view* views = nullptr;
@@ -235,7 +335,7 @@ int vr_tlas_destroy(vr_tlas t)
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(t->device);
-    void* ptrs[] = {t->pairs, t->slots, t->insts, t->views};
+    void* ptrs[] = {t->quads, t->slots, t->insts, t->views};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     (void)hipSetDevice(cur);
@@ -283,7 +383,6 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
         for (int k = 0; k < 3; ++k) rec.ti[k] = -in.transform[9 + k];
         rec.blas = in.blas;
         rec.inst_id = in.inst_id;
-        rec.pad[0] = rec.pad[1] = 0;
         // get_bounds(inst): affine = inverse(affine_inv), trans = -trans_inv,
         // the BVH box's corners in compute_vertices order (aabb.inl:305-324)
         const mat3h aff = inverse3(ainv);
@@ -303,8 +402,8 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
             }
         }
         for (int k = 0; k < 3; ++k) {
-            boxes[6 * (size_t)i + k] = blo[k];
-            boxes[6 * (size_t)i + 3 + k] = bhi[k];
+            boxes[6 * (size_t)i + k] = rec.box[k] = blo[k];
+            boxes[6 * (size_t)i + 3 + k] = rec.box[3 + k] = bhi[k];
         }
     }
     vr_tlas t = new vr_tlas_t;
@@ -360,16 +459,73 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
             if (it.parent < 0) t->root = entry;
             else pairs[(size_t)it.parent].e[it.side] = entry;
         }
-        if (depth > (uint32_t)TLAS_STACK)
+        if (3 * depth + 1 > (uint32_t)MAX_STACK)
             return bail(fail(VR_ERR_UNSUPPORTED, "vr_tlas_build: top level deeper than its traversal stack"));
     }
     t->num_pairs = (uint32_t)pairs.size();
+    // the 4-wide top level: quad p is the collapse rooted at pair p (open the
+    // largest-area inner child until four children), as for the bottom levels
+    std::vector<quad_node> quads(pairs.size());
+    for (size_t p = 0; p < pairs.size(); ++p) {
+        struct child {
+            uint32_t e;
+            float lo[3], hi[3];
+        };
+        auto from = [](const pair_node& r, int side) {
+            child c;
+            c.e = r.e[side];
+            for (int k = 0; k < 3; ++k) {
+                c.lo[k] = r.f[6 * side + k];
+                c.hi[k] = r.f[6 * side + 3 + k];
+            }
+            return c;
+        };
+        child c[4] = {from(pairs[p], 0), from(pairs[p], 1), {}, {}};
+        int nc = 2;
+        while (nc < 4) {
+            int best = -1;
+            float best_a = -1.0f;
+            for (int k = 0; k < nc; ++k) {
+                if ((c[k].e & LEAF_BIT) || c[k].e >= pairs.size()) continue;
+                const float dx = c[k].hi[0] - c[k].lo[0], dy = c[k].hi[1] - c[k].lo[1], dz = c[k].hi[2] - c[k].lo[2];
+                const float a = dx * dy + dy * dz + dz * dx;
+                if (a > best_a) {
+                    best_a = a;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            const pair_node& r = pairs[c[best].e];
+            c[nc] = from(r, 1);
+            c[best] = from(r, 0);
+            ++nc;
+        }
+        const float inf = std::numeric_limits<float>::infinity();
+        float v[6][4];
+        uint32_t ce[4];
+        for (int k = 0; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                v[a][k] = k < nc ? c[k].lo[a] : inf;
+                v[3 + a][k] = k < nc ? c[k].hi[a] : inf;
+            }
+            ce[k] = k < nc ? c[k].e : LEAF_BIT;
+        }
+        quad_node& qn = quads[p];
+        qn.lo_x = make_float4(v[0][0], v[0][1], v[0][2], v[0][3]);
+        qn.lo_y = make_float4(v[1][0], v[1][1], v[1][2], v[1][3]);
+        qn.lo_z = make_float4(v[2][0], v[2][1], v[2][2], v[2][3]);
+        qn.hi_x = make_float4(v[3][0], v[3][1], v[3][2], v[3][3]);
+        qn.hi_y = make_float4(v[4][0], v[4][1], v[4][2], v[4][3]);
+        qn.hi_z = make_float4(v[5][0], v[5][1], v[5][2], v[5][3]);
+        qn.child = make_uint4(ce[0], ce[1], ce[2], ce[3]);
+        qn.meta = make_uint4((uint32_t)nc, 0u, 0u, 0u);
+    }
     auto upload = [&](auto** dst, const auto* src, size_t count) -> bool {
         const size_t bytes = std::max<size_t>(count, 1) * sizeof(**dst);
         if (hipMalloc(reinterpret_cast<void**>(dst), bytes) != hipSuccess) return false;
         return count == 0 || hipMemcpy(*dst, src, count * sizeof(**dst), hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!upload(&t->pairs, pairs.data(), pairs.size()) || !upload(&t->slots, slots.data(), slots.size()) ||
+    if (!upload(&t->quads, quads.data(), quads.size()) || !upload(&t->slots, slots.data(), slots.size()) ||
         !upload(&t->insts, recs.data(), recs.size()) || !upload(&t->views, views.data(), views.size()))
         return bail(fail(VR_ERR_OUT_OF_MEMORY, "vr_tlas_build: device allocation or upload failed"));
     *out = t;
@@ -385,7 +541,7 @@ int vr_tlas_closest_hit(vr_tlas t, const vr_ray* d_rays, size_t n, vr_inst_hit* 
     if (n == 0) return VR_OK;
     int rc = ensure_device();
     if (rc) return rc;
-    tlas_args A{t->pairs, t->num_pairs, t->root, t->slots, t->num_instances, t->insts, t->views,
+    tlas_args A{t->quads, t->num_pairs, t->root, t->slots, t->num_instances, t->insts, t->views,
                 d_rays, (uint32_t)n, d_hits};
     hipLaunchKernelGGL(k_tlas_trace, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
     VR_HIP_CHECK(hipGetLastError());
