@@ -290,6 +290,9 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
  * ray transformed into each instance's space, the instance's BVH traversed,
  * the closest hit kept). */
 int vr_tlas_closest_hit(vr_tlas tlas, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream);
+/* any_hit over the top level (the same overload with the AnyHit traversal):
+ * a ray stops at its first hit inside [tmin, tmax], in any instance. */
+int vr_tlas_any_hit(vr_tlas tlas, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream);
 /* the top level's reference-layout nodes and instance order (host copies;
  * num_nodes gets the node count, nodes/indices may be null to query it) */
 int vr_tlas_nodes(vr_tlas tlas, vr_bvh_node* h_nodes, uint32_t* h_indices, size_t* num_nodes);

@@ -87,3 +87,42 @@ def test_no_instances_and_misses(V):
     r["ori"] = (100.0, 100.0, 100.0)
     r["dir"] = (0.0, 0.0, 1.0)
     assert np.all(t.closest_hit(r)["prim_id"] == -1)
+
+
+def test_any_hit_finds_a_hit_exactly_where_closest_hit_does(V, O):
+    blas_tris, inst = S.instanced_scene(num_instances=40, seed=41)
+    rays = S.instanced_rays(20000, seed=42)
+    blas, t = build(V, blas_tris, inst)
+    closest = t.closest_hit(rays)
+    anyh = t.any_hit(rays)
+    hit = closest["prim_id"] >= 0
+    assert np.array_equal(anyh["prim_id"] >= 0, hit)
+    # each reported hit is a real intersection of that instance's triangle at
+    # the reported t, no nearer than the closest
+    assert np.all(anyh["t"][hit] >= closest["t"][hit])
+    by_index = {i: inst[i] for i in range(inst.size)}
+    sel = np.nonzero(hit)[0][:500]
+    for k in sel:
+        ins = by_index[int(anyh["instance"][k])]
+        r = O.transform_rays(ins, rays[k:k + 1])
+        tri = blas_tris[int(ins["blas"])]
+        re = O.intersect(tri, r, np.array([int(anyh["prim_id"][k])]))
+        assert re["prim_id"][0] >= 0 and re["t"][0] == anyh["t"][k]
+
+
+def test_instances_of_host_sah_trees(V, O):
+    blas_tris, inst = S.instanced_scene(num_instances=30, seed=51)
+    rays = S.instanced_rays(20000, seed=52)
+    blas = []
+    for tr in blas_tris:
+        nodes, idx = V.sah_build(tr)
+        blas.append(V.bvh_from_nodes(tr, nodes, idx))
+    t = V.tlas(blas, inst)
+    got = t.closest_hit(rays)
+    trees = []
+    for tr in blas_tris:
+        nodes, idx, _, _ = O.lbvh_build(tr)
+        trees.append((nodes, tr, idx))
+    want, ties = O.trace_instanced(trees, inst, rays)
+    assert int(ties.sum()) == 0
+    compare(got, want)

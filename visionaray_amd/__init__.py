@@ -350,6 +350,12 @@ class tlas:
         check(lib().vr_tlas_closest_hit(self.handle, r.ptr, r.count, hits.ptr, _stream(stream)))
         return hits.to_host()
 
+    def any_hit(self, rays, stream=None) -> np.ndarray:
+        r = _as_device(rays, RAY_DTYPE)
+        hits = DeviceArray(r.count, INST_HIT_DTYPE)
+        check(lib().vr_tlas_any_hit(self.handle, r.ptr, r.count, hits.ptr, _stream(stream)))
+        return hits.to_host()
+
     def closest_hit_device(self, rays: DeviceArray, hits: DeviceArray, stream=None):
         check(lib().vr_tlas_closest_hit(self.handle, rays.ptr, rays.count, hits.ptr, _stream(stream)))
 

@@ -61,6 +61,7 @@ SIGNATURES = {
     "vr_trace": (I, [P, I, P, SZ, P, P]),
     "vr_tlas_build": (I, [P, U32, P, U32, P, C.POINTER(P)]),
     "vr_tlas_closest_hit": (I, [P, P, SZ, P, P]),
+    "vr_tlas_any_hit": (I, [P, P, SZ, P, P]),
     "vr_tlas_nodes": (I, [P, P, P, C.POINTER(SZ)]),
     "vr_tlas_destroy": (I, [P]),
     "vr_trace_counted": (I, [P, I, P, SZ, P, P, P]),

@@ -114,6 +114,7 @@ constexpr uint32_t LEVEL_END = 0xFFFFFFFDu;
 // an instance pushes the rest of its top-level leaf and a TO_TOP marker; the
 // marker's pop restores the world-space ray.  Boxes and triangles are culled
 // by the closest hit over all instances so far.
+template <bool ANY_HIT>
 __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
 {
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
@@ -247,6 +248,11 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
                 best_v = v;
                 best_tri = (int32_t)k;
                 best_inst = cur_inst;
+                if (ANY_HIT) {  // any hit ends the query
+                    cur = TRAV_DONE;
+                    ln = 0;
+                    sp = 0;
+                }
             }
             continue;
         }
@@ -362,8 +368,16 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
         VR_CHECK(s.num_prims > 0, VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: an empty BVH has no box to instance");
         VR_CHECK(s.quads, VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: BVH without traversal nodes");
         views[b] = s.view();
-        VR_HIP_CHECK(hipMemcpyAsync(&root_boxes[6 * (size_t)b], s.root_box, 6 * sizeof(float), hipMemcpyDeviceToHost,
-                                    st));
+        // the BVH's root box (node(0).get_bounds()): kept by LBVH builds, the
+        // first reference node of adopted trees
+        float* rb = &root_boxes[6 * (size_t)b];
+        if (s.root_box) {
+            VR_HIP_CHECK(hipMemcpyAsync(rb, s.root_box, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+        } else if (s.ref_nodes) {
+            VR_HIP_CHECK(hipMemcpyAsync(rb, s.ref_nodes, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+        } else {
+            return fail(VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: BVH without a root box");
+        }
     }
     VR_HIP_CHECK(hipStreamSynchronize(st));
     // the instances: inverse maps and world boxes
@@ -533,19 +547,36 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
     VR_TRY_END
 }
 
-int vr_tlas_closest_hit(vr_tlas t, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream)
+namespace {
+
+int tlas_trace(vr_tlas t, bool any_hit, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream)
 {
-    VR_CHECK(t, VR_ERR_INVALID_ARGUMENT, "vr_tlas_closest_hit: null handle");
-    VR_CHECK(n == 0 || (d_rays && d_hits), VR_ERR_INVALID_ARGUMENT, "vr_tlas_closest_hit: null array");
-    VR_CHECK(n <= 0xFFFFFFFFull, VR_ERR_UNSUPPORTED, "vr_tlas_closest_hit: too many rays");
+    VR_CHECK(t, VR_ERR_INVALID_ARGUMENT, "vr_tlas trace: null handle");
+    VR_CHECK(n == 0 || (d_rays && d_hits), VR_ERR_INVALID_ARGUMENT, "vr_tlas trace: null array");
+    VR_CHECK(n <= 0xFFFFFFFFull, VR_ERR_UNSUPPORTED, "vr_tlas trace: too many rays");
     if (n == 0) return VR_OK;
     int rc = ensure_device();
     if (rc) return rc;
     tlas_args A{t->quads, t->num_pairs, t->root, t->slots, t->num_instances, t->insts, t->views,
                 d_rays, (uint32_t)n, d_hits};
-    hipLaunchKernelGGL(k_tlas_trace, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
+    if (any_hit)
+        hipLaunchKernelGGL(k_tlas_trace<true>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
+    else
+        hipLaunchKernelGGL(k_tlas_trace<false>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
+}
+
+}  // namespace
+
+int vr_tlas_closest_hit(vr_tlas t, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream)
+{
+    return tlas_trace(t, false, d_rays, n, d_hits, stream);
+}
+
+int vr_tlas_any_hit(vr_tlas t, const vr_ray* d_rays, size_t n, vr_inst_hit* d_hits, void* stream)
+{
+    return tlas_trace(t, true, d_rays, n, d_hits, stream);
 }
 
 int vr_tlas_nodes(vr_tlas t, vr_bvh_node* h_nodes, uint32_t* h_indices, size_t* num_nodes)
