@@ -9,6 +9,8 @@
 //   binned_sah_builder      detail/bvh/sah.h:176   binned_sah_builder (host build + upload)
 //   bvh_refitter::refit     detail/bvh/refit.h:28  bvh_refitter::refit
 //   closest_hit / any_hit   traverse_linear.inl    closest_hit / any_hit (ray batches)
+//   tree.inst(mat4x3) + lbvh_builder over index_bvh<bvh_inst>  bvh.h:679-782
+//                                                  bvh_inst + instanced_bvh (top level built on the host)
 //   pinhole_camera          pinhole_camera.h       pinhole_camera
 //   make_kernel_params      kernels.h:87           make_kernel_params
 //   pathtracing::kernel     detail/pathtracing.inl pathtracing::kernel
@@ -214,6 +216,67 @@ inline void closest_hit(const hip_index_bvh& b, const ray* d_rays, size_t n, hit
 inline void any_hit(const hip_index_bvh& b, const ray* d_rays, size_t n, hit_record* d_hits, void* stream = nullptr)
 {
     check(vr_trace(b.handle(), 1, d_rays, n, d_hits, stream));
+}
+
+// One instance of a built BVH (index_bvh_inst_t, bvh.h:679-782): the
+// mat4x3 as its 12 floats column by column (mat4x3(T const[12])); set_inst_id
+// as in the reference.  The top level over instances (the viewer's
+// lbvh_builder over index_bvh<bvh_inst>, src/viewer/viewer.cpp:1534-1560) is
+// instanced_bvh; the instanced BVHs must outlive it.
+using inst_hit_record = vr_inst_hit;
+
+struct bvh_inst {
+    vr_instance rec{};
+    bvh_inst(uint32_t blas_index, const float mat4x3[12])
+    {
+        rec.blas = blas_index;
+        rec.inst_id = -1;  // ~0u, as index_bvh_inst_t starts
+        for (int k = 0; k < 12; ++k) rec.transform[k] = mat4x3[k];
+    }
+    void set_inst_id(int id) { rec.inst_id = id; }
+    int get_inst_id() const { return rec.inst_id; }
+};
+
+class instanced_bvh {
+public:
+    instanced_bvh(const std::vector<const hip_index_bvh*>& blas, const std::vector<bvh_inst>& instances,
+                  void* stream = nullptr)
+    {
+        std::vector<vr_bvh> hs;
+        for (const hip_index_bvh* b : blas) hs.push_back(b->handle());
+        std::vector<vr_instance> recs;
+        for (const bvh_inst& i : instances) recs.push_back(i.rec);
+        check(vr_tlas_build(hs.data(), (uint32_t)hs.size(), recs.data(), (uint32_t)recs.size(), stream, &h_));
+    }
+    instanced_bvh(const instanced_bvh&) = delete;
+    instanced_bvh& operator=(const instanced_bvh&) = delete;
+    ~instanced_bvh() { vr_tlas_destroy(h_); }
+    vr_tlas handle() const { return h_; }
+    // the top level's reference-layout nodes and instance order
+    std::pair<std::vector<bvh_node>, std::vector<uint32_t>> nodes() const
+    {
+        size_t n = 0;
+        check(vr_tlas_nodes(h_, nullptr, nullptr, &n));
+        std::vector<bvh_node> nodes(n);
+        std::vector<uint32_t> idx(n ? (n + 1) / 2 : 0);
+        check(vr_tlas_nodes(h_, nodes.data(), idx.data(), &n));
+        return {nodes, idx};
+    }
+
+private:
+    vr_tlas h_ = nullptr;
+};
+
+// closest_hit / any_hit over the top level (detail/bvh/intersect.inl:362-394)
+inline void closest_hit(const instanced_bvh& b, const ray* d_rays, size_t n, inst_hit_record* d_hits,
+                        void* stream = nullptr)
+{
+    check(vr_tlas_closest_hit(b.handle(), d_rays, n, d_hits, stream));
+}
+inline void any_hit(const instanced_bvh& b, const ray* d_rays, size_t n, inst_hit_record* d_hits,
+                    void* stream = nullptr)
+{
+    check(vr_tlas_any_hit(b.handle(), d_rays, n, d_hits, stream));
 }
 
 // pinhole_camera (pinhole_camera.h:20-116): perspective + look_at, then

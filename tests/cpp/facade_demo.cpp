@@ -61,6 +61,24 @@ int main()
         std::printf("]");
     }
 
+    // two instances of the SAH tree: in place (inst_id 7) and 10 units down
+    // z (inst_id 8); the same rays see the first one only
+    {
+        const float eye3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+        const float down[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, -10};
+        bvh_inst a(0, eye3), b(0, down);
+        a.set_inst_id(7);
+        b.set_inst_id(8);
+        instanced_bvh top({&sbvh}, {a, b});
+        device_vector<inst_hit_record> d_ih(rays.size());
+        closest_hit(top, d_rays.data(), rays.size(), d_ih.data());
+        auto ih = d_ih.to_host();
+        std::printf(", \"inst\": [");
+        for (size_t i = 0; i < ih.size(); ++i)
+            std::printf("%s[%d, %.9g, %d]", i ? ", " : "", ih[i].prim_id, (double)ih[i].t, ih[i].inst_id);
+        std::printf("], \"inst_top_nodes\": %zu", top.nodes().first.size());
+    }
+
     // refit after moving the far triangle from z = -1 to z = -3
     for (int k = 0; k < 1; ++k) prims[1].v1[2] = -3.0f;
     device_vector<basic_triangle> d_moved(prims);

@@ -26,6 +26,9 @@ def test_facade_demo():
                 assert t == wt
     assert [h[0] >= 0 for h in out["any"]] == [True, True, False, False]
     assert out["refit_t1"] == 3.0
+    assert [h[0] for h in out["inst"]] == [0, 1, -1, -1]
+    assert [h[2] for h in out["inst"]] == [7, 7, -1, -1] and out["inst"][0][1] == 1.0
+    assert out["inst_top_nodes"] == 1  # two instances: one leaf
     assert np.isfinite(out["pt_sum"]) and out["pt_sum"] > 0
     assert np.isfinite(out["ao_sum"]) and out["ao_sum"] > 0
     assert out["frames"] == 2

@@ -366,7 +366,9 @@ int vr_tlas_build(const vr_bvh* blas, uint32_t num_blas, const vr_instance* h_in
         VR_CHECK(blas[b], VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: null BVH");
         const bvh_state& s = blas[b]->s;
         VR_CHECK(s.num_prims > 0, VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: an empty BVH has no box to instance");
-        VR_CHECK(s.quads, VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: BVH without traversal nodes");
+        // (a tree whose root is a leaf has no traversal nodes; its root entry
+        // is the leaf)
+        VR_CHECK(s.quads || (s.root & LEAF_BIT), VR_ERR_INVALID_ARGUMENT, "vr_tlas_build: BVH without traversal nodes");
         views[b] = s.view();
         // the BVH's root box (node(0).get_bounds()): kept by LBVH builds, the
         // first reference node of adopted trees
