@@ -53,12 +53,15 @@ def test_golden_sah_hits(V):
     assert_hits_match(hits, g["ref_sah_closest"], g["tris"], g["rays"])
 
 
-@pytest.mark.parametrize("scene", ["c0_level5", "clusters", "heightfield"])
+@pytest.mark.parametrize("scene", ["c0_level5", "clusters", "heightfield", "clusters_large"])
 @pytest.mark.parametrize("any_hit", [False, True])
 def test_lbvh_trace_matches_oracle(V, O, scene, any_hit):
+    # clusters_large has more than 65,536 triangles: its upper quads come from
+    # the build's last (top) bottom-up pass
     tris = {"c0_level5": lambda: S.config0_scene(level=5),
             "clusters": lambda: S.random_clusters(40_000, seed=2),
-            "heightfield": lambda: S.heightfield(96, seed=1)}[scene]()
+            "heightfield": lambda: S.heightfield(96, seed=1),
+            "clusters_large": lambda: S.random_clusters(300_000, seed=6)}[scene]()
     print(f"[case] trace {scene} any={any_hit} n={tris.size}", flush=True)
     bvh = V.lbvh_builder().build(tris)
     nodes, idx, _, _ = O.lbvh_build(tris)
