@@ -141,6 +141,8 @@ static uint32_t* work_counter()
     return ds->work + 32u * (ds->next.fetch_add(1u, std::memory_order_relaxed) % WORK_SLOTS);
 }
 
+uint32_t* vr::next_work_counter() { return work_counter(); }
+
 extern "C" {
 
 int vr_init(int device)

@@ -15,6 +15,7 @@
 // instances is the smallest t, and it culls boxes in both levels.
 #include "vr_internal.h"
 
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -114,38 +115,35 @@ constexpr uint32_t LEVEL_END = 0xFFFFFFFDu;
 // an instance pushes the rest of its top-level leaf and a TO_TOP marker; the
 // marker's pop restores the world-space ray.  Boxes and triangles are culled
 // by the closest hit over all instances so far.
+constexpr uint32_t RAY_CHUNK = 64;  // rays a wave reserves at once
+
+// Persistent lanes: a resident grid; a lane whose query ends takes the next
+// ray of its wave's reserved chunk (one atomic per 64 rays), so a wave keeps
+// all lanes busy until the rays run out.
 template <bool ANY_HIT>
-__global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
+__global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A, uint32_t* __restrict__ work)
 {
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
     uint32_t* st = stack + threadIdx.x;
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    const bool live = i < A.n;
-    f3 ow = mk3(0.0f, 0.0f, 0.0f), dw = mk3(0.0f, 0.0f, 1.0f);
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    // the query
+    f3 ow = mk3(0.0f, 0.0f, 0.0f), dw = mk3(0.0f, 0.0f, 1.0f), iw = dw;
     float tmin = 0.0f, tmax = 0.0f;
-    if (live) {
-        const float4* rp = reinterpret_cast<const float4*>(A.rays + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        ow = mk3(r0.x, r0.y, r0.z);
-        dw = mk3(r0.w, r1.x, r1.y);
-        tmin = r1.z;
-        tmax = r1.w;
-    }
-    const f3 iw = mk3(dw.x != 0.0f ? 1.0f / dw.x : FLT_MAX, dw.y != 0.0f ? 1.0f / dw.y : FLT_MAX,
-                      dw.z != 0.0f ? 1.0f / dw.z : FLT_MAX);
-    // the current space's ray
+    // the current space's ray and level
     f3 o = ow, d = dw, iv = iw;
-    bool bottom = false;  // in an instance
+    bool bottom = false;
     const quad_node* q = A.quads;
     const tri_node* tris = nullptr;
     uint32_t nq = A.num_quads, np = A.num_slots;
-    uint32_t cur = live && A.num_slots > 0 ? A.root : TRAV_DONE;
-    uint32_t lf = 0, ln = 0;
+    uint32_t cur = TRAV_DONE, lf = 0, ln = 0;
     int sp = 0;
     uint32_t spill[TWO_LEVEL_STACK - LDS_STACK];
     float best_t = FLT_MAX, best_u = 0.0f, best_v = 0.0f;
     int32_t best_tri = -1, best_inst = -1, cur_inst = -1;
-    uint32_t budget = 1u << 26;  // a guard against corrupt trees only
+    uint32_t budget = 0;
+    uint32_t idx = 0;
+    bool tracing = false, done = false;
+    uint32_t chunk_base = 0, chunk_left = 0;  // the wave's reserved ray indices (uniform)
     auto push = [&](uint32_t e) {
         if (sp < LDS_STACK) st[sp * BLOCK] = e;
         else spill[sp - LDS_STACK] = e;
@@ -160,49 +158,125 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
         --sp;
         return e;
     };
+    auto to_top = [&]() {
+        o = ow;
+        d = dw;
+        iv = iw;
+        bottom = false;
+        q = A.quads;
+        nq = A.num_quads;
+        np = A.num_slots;
+    };
+    auto abandon = [&]() {  // a corrupt tree: end the query
+        cur = TRAV_DONE;
+        ln = 0;
+        sp = 0;
+        to_top();
+    };
     for (;;) {
-        // the instance is done (its range too): back to the top level
-        if (ln == 0 && cur == LEVEL_END) {
-            --sp;  // the marker
-            o = ow;
-            d = dw;
-            iv = iw;
-            bottom = false;
-            q = A.quads;
-            nq = A.num_quads;
-            np = A.num_slots;
-            cur = pop();
-        }
-        // a leaf entry becomes the lane's range: triangles in an instance,
-        // instance slots at the top
-        if (ln == 0 && (cur & LEAF_BIT) && cur != TRAV_DONE && cur != LEVEL_END) {
-            lf = cur & LEAF_FIRST_MASK;
-            ln = ((cur >> 26) & 31u) + 1u;
-            if (lf + ln > np) {  // corrupt entry
-                ln = 0;
-                cur = TRAV_DONE;
-                sp = 0;
+        // lanes without a query take the next rays of the wave's chunk, once
+        // a quarter of the wave waits (or no lane has a query left)
+        const bool need = !done && !tracing;
+        const uint64_t needm = __ballot(need);
+        if (needm && (__popcll(needm) >= 16 || __ballot(tracing) == 0)) {
+            const uint32_t k = (uint32_t)__popcll(needm);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(needm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)needm, 0u));
+            uint32_t fresh = 0;
+            if (chunk_left < k) {
+                const int leader = __ffsll((unsigned long long)needm) - 1;
+                if ((int)lane == leader) fresh = atomicAdd(work, RAY_CHUNK);
+                fresh = __shfl(fresh, leader);
+            }
+            if (need) {
+                const uint32_t my = rank < chunk_left ? chunk_base + rank : fresh + (rank - chunk_left);
+                if (my < A.n) {
+                    const float4* rp = reinterpret_cast<const float4*>(A.rays + my);
+                    const float4 r0 = rp[0], r1 = rp[1];
+                    ow = mk3(r0.x, r0.y, r0.z);
+                    dw = mk3(r0.w, r1.x, r1.y);
+                    tmin = r1.z;
+                    tmax = r1.w;
+                    iw = mk3(dw.x != 0.0f ? 1.0f / dw.x : FLT_MAX, dw.y != 0.0f ? 1.0f / dw.y : FLT_MAX,
+                             dw.z != 0.0f ? 1.0f / dw.z : FLT_MAX);
+                    to_top();
+                    cur = A.num_slots > 0 ? A.root : TRAV_DONE;
+                    lf = ln = 0;
+                    sp = 0;
+                    best_t = FLT_MAX;
+                    best_u = best_v = 0.0f;
+                    best_tri = best_inst = cur_inst = -1;
+                    budget = 1u << 26;  // a guard against corrupt trees only
+                    idx = my;
+                    tracing = true;
+                } else {
+                    done = true;
+                }
+            }
+            if (chunk_left < k) {
+                chunk_base = fresh + (k - chunk_left);
+                chunk_left = RAY_CHUNK - (k - chunk_left);
             } else {
-                cur = pop();
+                chunk_base += k;
+                chunk_left -= k;
             }
         }
-        const bool node_work = !(cur & LEAF_BIT);
-        const bool active = node_work || ln > 0;
-        if (__ballot(active) == 0) break;
-        if (active && --budget == 0) {
-            cur = TRAV_DONE;
-            ln = 0;
-            sp = 0;
+        if (__ballot(!done) == 0) break;
+        if (tracing) {
+            // the instance is done (its range too): back to the top level
+            if (ln == 0 && cur == LEVEL_END) {
+                --sp;  // the marker
+                to_top();
+                cur = pop();
+            }
+            // a leaf entry becomes the lane's range: triangles in an
+            // instance, instance slots at the top
+            if (ln == 0 && (cur & LEAF_BIT) && cur != TRAV_DONE && cur != LEVEL_END) {
+                lf = cur & LEAF_FIRST_MASK;
+                ln = ((cur >> 26) & 31u) + 1u;
+                if (lf + ln > np) abandon();
+                else cur = pop();
+            }
+        }
+        const bool node_work = tracing && !(cur & LEAF_BIT);
+        const bool leaf_work = tracing && ln > 0;
+        if (tracing && !node_work && !leaf_work) {
+            // the query is done: its hit record
+            vr_inst_hit out;
+            out.pad = 0;
+            if (best_tri >= 0) {
+                const inst_rec* rec = A.insts + best_inst;
+                const tri_node* tn = A.views[rec->blas].tris + best_tri;
+                out.prim_id = (int32_t)__float_as_uint(tn->a.w);
+                out.geom_id = (int32_t)__float_as_uint(tn->b.w);
+                out.t = best_t;
+                out.u = best_u;
+                out.v = best_v;
+                out.inst_id = rec->inst_id;
+                out.instance = best_inst;
+            } else {
+                out.prim_id = -1;
+                out.geom_id = -1;
+                out.t = FLT_MAX;
+                out.u = 0.0f;
+                out.v = 0.0f;
+                out.inst_id = -1;
+                out.instance = -1;
+            }
+            A.hits[idx] = out;
+            tracing = false;
+            continue;
+        }
+        if ((node_work || leaf_work) && --budget == 0) {
+            abandon();
             continue;
         }
         const uint32_t n_node = (uint32_t)__popcll(__ballot(node_work));
-        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(ln > 0));
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(leaf_work));
         if ((float)n_node >= VR_TRAV_NODE_BIAS * (float)n_leaf) {
             if (!node_work) continue;
-            if (cur >= nq) {  // corrupt entry
-                cur = TRAV_DONE;
-                ln = 0;
-                sp = 0;
+            if (cur >= nq) {
+                abandon();
                 continue;
             }
             const float4* p = reinterpret_cast<const float4*>(q + cur);
@@ -222,9 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
                 cur = pop();
             } else {
                 if (sp + 3 > TWO_LEVEL_STACK) {  // cannot happen for trees the host accepted
-                    cur = TRAV_DONE;
-                    ln = 0;
-                    sp = 0;
+                    abandon();
                     continue;
                 }
                 if (k3 < __builtin_inff()) push(e3);
@@ -234,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
             }
             continue;
         }
-        if (ln == 0) continue;
+        if (!leaf_work) continue;
         const uint32_t k = lf++;
         --ln;
         if (bottom) {
@@ -252,6 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
                     cur = TRAV_DONE;
                     ln = 0;
                     sp = 0;
+                    to_top();
                 }
             }
             continue;
@@ -263,9 +336,7 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
                          tmax, best_t) < __builtin_inff()))
             continue;
         if (sp + 3 > TWO_LEVEL_STACK) {
-            cur = TRAV_DONE;
-            ln = 0;
-            sp = 0;
+            abandon();
             continue;
         }
         // the rest of this leaf waits under the marker; cur (deferred at the
@@ -291,30 +362,9 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A)
         cur = bv.root;
         cur_inst = (int32_t)ii;
     }
-    if (!live) return;
-    vr_inst_hit out;
-    out.pad = 0;
-    if (best_tri >= 0) {
-        const inst_rec* rec = A.insts + best_inst;
-        const tri_node* tn = A.views[rec->blas].tris + best_tri;
-        out.prim_id = (int32_t)__float_as_uint(tn->a.w);
-        out.geom_id = (int32_t)__float_as_uint(tn->b.w);
-        out.t = best_t;
-        out.u = best_u;
-        out.v = best_v;
-        out.inst_id = rec->inst_id;
-        out.instance = best_inst;
-    } else {
-        out.prim_id = -1;
-        out.geom_id = -1;
-        out.t = FLT_MAX;
-        out.u = 0.0f;
-        out.v = 0.0f;
-        out.inst_id = -1;
-        out.instance = -1;
-    }
-    A.hits[i] = out;
 }
+
+__global__ void k_reset_work(uint32_t* work) { *work = 0u; }
 
 }  // namespace
 
@@ -561,10 +611,24 @@ int tlas_trace(vr_tlas t, bool any_hit, const vr_ray* d_rays, size_t n, vr_inst_
     if (rc) return rc;
     tlas_args A{t->quads, t->num_pairs, t->root, t->slots, t->num_instances, t->insts, t->views,
                 d_rays, (uint32_t)n, d_hits};
-    if (any_hit)
-        hipLaunchKernelGGL(k_tlas_trace<true>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
-    else
-        hipLaunchKernelGGL(k_tlas_trace<false>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, A);
+    // a resident grid (the blocks the device holds at once), no more waves
+    // than there are rays
+    static int grid[2] = {0, 0};
+    int& g = grid[any_hit ? 1 : 0];
+    if (g == 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        VR_HIP_CHECK(hipGetDevice(&dev));
+        if (any_hit) VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tlas_trace<true>, BLOCK, 0));
+        else VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tlas_trace<false>, BLOCK, 0));
+        VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        g = std::max(1, per_cu) * std::max(1, cus);
+    }
+    const unsigned blocks = std::max(1u, std::min((unsigned)g, div_up(n, BLOCK)));
+    uint32_t* work = next_work_counter();
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, st, work);
+    if (any_hit) hipLaunchKernelGGL(k_tlas_trace<true>, dim3(blocks), dim3(BLOCK), 0, st, A, work);
+    else hipLaunchKernelGGL(k_tlas_trace<false>, dim3(blocks), dim3(BLOCK), 0, st, A, work);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }

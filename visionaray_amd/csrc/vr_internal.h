@@ -539,6 +539,10 @@ inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32
 // leaf_max triangles into one leaf over their contiguous range.
 int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
 
+// the work counter of the next persistent launch on the current device (one
+// of a ring of counters; the launch resets it on its stream first)
+uint32_t* next_work_counter();
+
 // host side of the lbvh over boxes (host_build.cpp): the CPU lbvh_builder
 // tree over 6-float boxes (lo xyz, hi xyz)
 int lbvh_build_host_boxes(const float* boxes, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
