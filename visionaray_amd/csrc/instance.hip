@@ -175,10 +175,10 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A, uint32_t* __r
     };
     for (;;) {
         // lanes without a query take the next rays of the wave's chunk, once
-        // a quarter of the wave waits (or no lane has a query left)
+        // VR_TLAS_REFILL_LANES of them wait (or no lane has a query left)
         const bool need = !done && !tracing;
         const uint64_t needm = __ballot(need);
-        if (needm && (__popcll(needm) >= 16 || __ballot(tracing) == 0)) {
+        if (needm && (__popcll(needm) >= VR_TLAS_REFILL_LANES || __ballot(tracing) == 0)) {
             const uint32_t k = (uint32_t)__popcll(needm);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(needm >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)needm, 0u));

@@ -35,9 +35,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
     bool tracing = false, done = false;
     uint32_t chunk_base = 0, chunk_left = 0;  // the wave's reserved indices (uniform)
     for (;;) {
+        // lanes without a query take the next rays of the wave's chunk, once
+        // VR_TRACE_REFILL_LANES of them wait (or no lane has a query left)
         const bool need = !done && !tracing;
         const uint64_t needm = __ballot(need);
-        if (needm) {
+        if (needm && (__popcll(needm) >= VR_TRACE_REFILL_LANES || __ballot(tracing) == 0)) {
             const uint32_t k = (uint32_t)__popcll(needm);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(needm >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)needm, 0u));

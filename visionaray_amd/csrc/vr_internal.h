@@ -324,6 +324,16 @@ constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
 // node iterations are chosen while at least this many times as many lanes
 // have node work as have a triangle to test (a node visit costs about twice a
 // triangle test)
+// persistent ray batches: idle lanes take new rays once this many of their
+// wave wait (fewer refills, each serving more lanes); measured best on the
+// large scene's bounce rays: 32 for the one-level trace, 16 for the
+// two-level one
+#ifndef VR_TRACE_REFILL_LANES
+#define VR_TRACE_REFILL_LANES 32
+#endif
+#ifndef VR_TLAS_REFILL_LANES
+#define VR_TLAS_REFILL_LANES 16
+#endif
 #ifndef VR_TRAV_NODE_BIAS
 #define VR_TRAV_NODE_BIAS 2.0f
 #endif
