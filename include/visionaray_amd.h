@@ -199,12 +199,23 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
  * moved vertices (same count and order as at build time). */
 int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream);
 
+/* Counts and depth.  The depth of an LBVH is read back from the device after
+ * the handle's last build / rebuild / refit (a wait on that work's stream up
+ * to it, not on the whole device). */
 int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info);
 /* Reference-layout nodes (tree.nodes(), bvh.h:910) into d_nodes[num_nodes]. */
 int vr_bvh_nodes(vr_bvh bvh, vr_bvh_node* d_nodes, void* stream);
 /* tree.indices() (bvh.h:913) into d_indices[num_prims]. */
 int vr_bvh_indices(vr_bvh bvh, uint32_t* d_indices, void* stream);
+/* Releases the handle.  Waits for the device first, as the reference's
+ * device_vector destructor does (cudaFree, cuda/device_vector.h): work on any
+ * stream may still read the tree. */
 int vr_bvh_destroy(vr_bvh bvh);
+/* Stream-ordered release with no host wait (per-frame rebuild loops, several
+ * streams): the arrays go back to the device's pool once the work enqueued on
+ * `stream` before this call is done; the caller orders any use of the tree on
+ * other streams before it.  No counterpart in the reference. */
+int vr_bvh_destroy_async(vr_bvh bvh, void* stream);
 
 /* Host binned SAH build (binned_sah_builder::build, detail/bvh/sah.h:176-807,
  * 16 bins, no spatial splits).  h_nodes must hold 2n - 1 nodes; the node count

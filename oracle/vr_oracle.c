@@ -861,9 +861,38 @@ static void pt_sample(const vro_node* nodes, const vro_tri* tris, const uint32_t
  * probabilities of the vertices j >= 2.  The roulette probabilities are held
  * constant (their decisions and values are not differentiated), which keeps
  * the gradient estimate unbiased for the gradient of the expected image:
- *   d I_c / d cd_{m,c} = I_c * #{j : m_j = m} / cd_{m,c}
- *   d I_c / d ce_{e,c} = T_c * ls_e   for the ending emitter e.
+ *   d I_c / d cd_{m,c} = E_c * sum_{j : m_j = m} g_j * prod_{k != j} f_{k,c}
+ *   d I_c / d ce_{e,c} = T_c * ls_e   for the ending emitter e,
+ * with f_{j,c} = cd_{m_j,c} * g_j, g_j = kd / pi * (cos / pdf).  When no
+ * factor of channel c is zero the sum is I_c * #{j : m_j = m} / cd_{m,c}.
+ * When exactly one factor (vertex j0) is zero -- a zero albedo channel, a
+ * zero kd, or an underflow -- only material m_{j0} has a nonzero derivative,
+ * g_{j0} * (product of the other factors) * E_c; with two or more zero
+ * factors every term holds a zero factor.  A zero albedo channel therefore
+ * gets its finite derivative instead of 0 / 0.
  * ------------------------------------------------------------------------- */
+
+/* leave-one-out bookkeeping of a path's per-channel throughput factors */
+typedef struct {
+    double nzp[3];   /* product of the nonzero factors over the roulette probabilities */
+    int zc[3];       /* zero factors so far */
+    uint32_t zm[3];  /* material of the (first) zero-factor vertex */
+    double zg[3];    /* its d factor / d cd */
+} adj_factors;
+
+/* d I / d cd for every material of the path's matte vertices, given the
+ * ending term's emission E_c (ambient or ce * ls) and I_c = T_c * E_c */
+static void adj_add_cd(const adj_factors* af, const uint32_t* mids, int nm, const vro_material* mats,
+                       const double w[3], const double E[3], const double I[3], double* grad_cd)
+{
+    for (int c = 0; c < 3; ++c) {
+        if (af->zc[c] == 0) {
+            for (int j = 0; j < nm; ++j) grad_cd[3 * mids[j] + c] += w[c] * I[c] / (double)mats[mids[j]].cd[c];
+        } else if (af->zc[c] == 1) {
+            grad_cd[3 * af->zm[c] + c] += w[c] * E[c] * af->zg[c] * af->nzp[c];
+        }
+    }
+}
 static void pt_sample_adjoint(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
                               const vro_material* mats, const vro_render_params* p, vro_ray ray, rng_t* g,
                               const double w[3], double* grad_cd, double* grad_ce)
@@ -871,15 +900,16 @@ static void pt_sample_adjoint(const vro_node* nodes, const vro_tri* tris, const 
     float tp[3] = {1, 1, 1};
     uint32_t mids[64];
     int nm = 0;
+    adj_factors af = {{1.0, 1.0, 1.0}, {0, 0, 0}, {0, 0, 0}, {0.0, 0.0, 0.0}};
     for (uint32_t bounce = 0; bounce < p->num_bounces; ++bounce) {
         hitrec hr = trace_one(nodes, tris, indices, num_prims, &ray, 0, NULL);
         if (!hr.hit) {
-            double I[3];
-            for (int k = 0; k < 3; ++k) I[k] = (double)(p->ambient[k] * tp[k]);
-            for (int j = 0; j < nm; ++j) {
-                const vro_material* m = &mats[mids[j]];
-                for (int c = 0; c < 3; ++c) grad_cd[3 * mids[j] + c] += w[c] * I[c] / (double)m->cd[c];
+            double I[3], E[3];
+            for (int k = 0; k < 3; ++k) {
+                I[k] = (double)(p->ambient[k] * tp[k]);
+                E[k] = (double)p->ambient[k];
             }
+            adj_add_cd(&af, mids, nm, mats, w, E, I, grad_cd);
             return;
         }
         v3 dir = ld3(ray.dir), ori = ld3(ray.ori);
@@ -890,13 +920,13 @@ static void pt_sample_adjoint(const vro_node* nodes, const vro_tri* tris, const 
         uint32_t mid = hr.geom_id;
         const vro_material* m = &mats[mid];
         if (m->type == 1) {
-            double I[3];
-            for (int k = 0; k < 3; ++k) I[k] = (double)((1.0f * tp[k]) * ((1.0f * m->ce[k]) * m->ls));
-            for (int c = 0; c < 3; ++c) grad_ce[3 * mid + c] += w[c] * (double)tp[c] * (double)m->ls;
-            for (int j = 0; j < nm; ++j) {
-                const vro_material* mj = &mats[mids[j]];
-                for (int c = 0; c < 3; ++c) grad_cd[3 * mids[j] + c] += w[c] * I[c] / (double)mj->cd[c];
+            double I[3], E[3];
+            for (int k = 0; k < 3; ++k) {
+                E[k] = (double)((1.0f * m->ce[k]) * m->ls);
+                I[k] = (double)((1.0f * tp[k]) * ((1.0f * m->ce[k]) * m->ls));
             }
+            for (int c = 0; c < 3; ++c) grad_ce[3 * mid + c] += w[c] * (double)tp[c] * (double)m->ls;
+            adj_add_cd(&af, mids, nm, mats, w, E, I, grad_cd);
             /* the forward still draws its roulette number here */
             if (bounce >= 2) (void)rng_next(g);
             return;
@@ -917,13 +947,27 @@ static void pt_sample_adjoint(const vro_node* nodes, const vro_tri* tris, const 
         }
         if (nm < 64) mids[nm++] = mid;
         float q = dot3(n, refl) / pdf;
-        for (int k = 0; k < 3; ++k) tp[k] = tp[k] * (src[k] * q);
+        for (int k = 0; k < 3; ++k) {
+            const float f = src[k] * q;
+            tp[k] = tp[k] * f;
+            if (f == 0.0f) {
+                if (af.zc[k]++ == 0) {
+                    af.zm[k] = mid;
+                    af.zg[k] = (double)m->kd / 3.14159265358979323846 * (double)q;
+                }
+            } else {
+                af.nzp[k] *= (double)f;
+            }
+        }
         if (bounce >= 2) {
             float prob = tp[0];
             if (tp[1] > prob) prob = tp[1];
             if (tp[2] > prob) prob = tp[2];
             int terminate = rng_next(g) > prob;
-            for (int k = 0; k < 3; ++k) tp[k] = tp[k] / prob;
+            for (int k = 0; k < 3; ++k) {
+                tp[k] = tp[k] / prob;
+                af.nzp[k] /= (double)prob;
+            }
             if (terminate) return;
         }
         v3 o2 = add3(pos, muls(refl, p->epsilon));

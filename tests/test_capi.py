@@ -95,3 +95,33 @@ def test_host_lbvh_matches_reference_cpu_tree():
     assert nodes.size == want_n.size and np.array_equal(idx, want_i)
     for k in ("lo", "hi", "first", "num_prims"):
         assert np.array_equal(nodes[k], want_n[k]), k
+
+
+def test_bvh_from_nodes_rejects_bad_trees(lib):
+    """vr_bvh_from_nodes checks the host tree before any device call: a child
+    index that wraps 32-bit arithmetic, and shared subtrees (a DAG), are
+    invalid arguments."""
+    from visionaray_amd import scenes as S
+    from visionaray_amd._lib import NODE_DTYPE, VR_ERR_INVALID_ARGUMENT, ptr
+    tris = np.zeros(2, S.TRI_DTYPE)
+    tris["e1"] = (1, 0, 0)
+    tris["e2"] = (0, 1, 0)
+    idx = np.arange(2, dtype=np.uint32)
+    out = C.c_void_p()
+
+    def call(nodes):
+        return lib.vr_bvh_from_nodes(ptr(tris), 2, ptr(nodes), nodes.size, ptr(idx), C.byref(out))
+
+    # first = 0xFFFFFFFF: first + 1 wraps to 0 in 32 bits
+    nodes = np.zeros(3, NODE_DTYPE)
+    nodes["first"][0] = 0xFFFFFFFF
+    nodes["first"][1:] = (0, 1)
+    nodes["num_prims"][1:] = 1
+    assert call(nodes) == VR_ERR_INVALID_ARGUMENT
+    assert b"out of range" in lib.vr_last_error()
+    # a DAG: inner nodes 1 and 2 share their children (leaves 3 and 4)
+    dag = np.zeros(5, NODE_DTYPE)
+    dag["first"] = (1, 3, 3, 0, 1)
+    dag["num_prims"] = (0, 0, 0, 1, 1)
+    assert call(dag) == VR_ERR_INVALID_ARGUMENT
+    assert b"reached twice" in lib.vr_last_error()

@@ -56,6 +56,7 @@ SIGNATURES = {
     "vr_bvh_nodes": (I, [P, P, P]),
     "vr_bvh_indices": (I, [P, P, P]),
     "vr_bvh_destroy": (I, [P]),
+    "vr_bvh_destroy_async": (I, [P, P]),
     "vr_sah_build": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_lbvh_build_host": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_trace": (I, [P, I, P, SZ, P, P]),

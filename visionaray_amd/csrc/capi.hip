@@ -66,6 +66,9 @@ struct device_state {
     bool ready = false;
     uint32_t* work = nullptr;  // WORK_SLOTS counters, 32 words apart
     std::atomic<uint32_t> next{0};
+    // the library's own non-blocking stream: small read-backs and host-tree
+    // uploads that must not wait for the caller's streams
+    hipStream_t aux = nullptr;
 };
 std::mutex g_mutex;
 device_state g_dev[64];
@@ -108,6 +111,9 @@ struct vr_batch_t {
     uint32_t* d_prefix = nullptr;  // num_scenes + 1
     uint32_t* d_frame = nullptr;   // frame counter (starts at -1, advanced before each frame)
     uint32_t* d_work = nullptr;    // the batch's own work counter (graph replays)
+    // every scene has d loss / d image and at most ADJ_MAX_BOUNCES bounces
+    // (vr_batch_backward needs both)
+    bool adjoint_ok = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -129,6 +135,7 @@ int vr::ensure_device()
         VR_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
         uint64_t keep = UINT64_MAX;
         VR_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        VR_HIP_CHECK(hipStreamCreateWithFlags(&ds->aux, hipStreamNonBlocking));
         ds->ready = true;
     }
     return VR_OK;
@@ -142,6 +149,27 @@ static uint32_t* work_counter()
 }
 
 uint32_t* vr::next_work_counter() { return work_counter(); }
+
+static hipStream_t aux_stream() { return dev_state()->aux; }
+
+// The handle's last build, rebuild or refit: an event on the caller's stream,
+// so a read-back of what the build wrote waits for that work only.
+static int mark_built(vr_bvh h, hipStream_t st)
+{
+    if (!h->built) VR_HIP_CHECK(hipEventCreateWithFlags(&h->built, hipEventDisableTiming));
+    VR_HIP_CHECK(hipEventRecord(h->built, st));
+    return VR_OK;
+}
+
+// copy device -> host after the handle's last build, on the library's own
+// stream (no wait for unrelated work)
+static int read_after_build(vr_bvh h, void* dst, const void* src, size_t bytes)
+{
+    if (h->built) VR_HIP_CHECK(hipEventSynchronize(h->built));
+    VR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, aux_stream()));
+    VR_HIP_CHECK(hipStreamSynchronize(aux_stream()));
+    return VR_OK;
+}
 
 extern "C" {
 
@@ -300,6 +328,7 @@ int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* st
     h->s.device = current_device();
     rc = lbvh_build_into(h->s, d_prims, n, (flags & VR_BVH_KEEP_NODES) != 0, (flags & VR_BVH_ROTATE) != 0,
                          (hipStream_t)stream);
+    if (rc == VR_OK) rc = mark_built(h, (hipStream_t)stream);
     if (rc) {
         vr_bvh_destroy(h);
         return rc;
@@ -317,7 +346,8 @@ int vr_lbvh_rebuild(vr_bvh bvh, const vr_triangle* d_prims, size_t n, void* stre
     VR_CHECK(n == 0 || d_prims, VR_ERR_INVALID_ARGUMENT, "vr_lbvh_rebuild: null primitives");
     int rc = ensure_device();
     if (rc) return rc;
-    return lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, bvh->s.want_rotate, (hipStream_t)stream);
+    rc = lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, bvh->s.want_rotate, (hipStream_t)stream);
+    return rc ? rc : mark_built(bvh, (hipStream_t)stream);
     VR_TRY_END
 }
 
@@ -352,8 +382,7 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
     VR_CHECK(num_prims == 0 || (h_prims && h_nodes && h_indices && num_nodes > 0), VR_ERR_INVALID_ARGUMENT,
              "vr_bvh_from_nodes: null array");
     VR_CHECK(num_prims <= (size_t)MAX_LEAF_FIRST, VR_ERR_UNSUPPORTED, "vr_bvh_from_nodes: too many primitives");
-    int rc = ensure_device();
-    if (rc) return rc;
+    // the tree is checked and converted on the host before any device call
     std::vector<pair_node> pairs;
     std::vector<tri_node> tris;
     tris.reserve(num_prims);
@@ -367,6 +396,7 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
             uint32_t side;
         };
         std::vector<item> stack{{0u, 1u, -1, 0u}};
+        std::vector<bool> seen(num_nodes, false);
         // append the leaf's triangles in index order (ranges checked by the caller)
         auto add_leaf = [&](const vr_bvh_node& nd) -> uint32_t {
             const uint32_t first = (uint32_t)tris.size();
@@ -391,6 +421,10 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
             stack.pop_back();
             VR_CHECK(it.node < num_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: child index out of range");
             VR_CHECK(it.depth <= 4096, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: tree too deep (cycle?)");
+            // a tree reaches every node once: a node reached twice means shared
+            // subtrees (a DAG, which the DFS could expand exponentially) or a cycle
+            VR_CHECK(!seen[it.node], VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: node reached twice (not a tree)");
+            seen[it.node] = true;
             const vr_bvh_node& nd = h_nodes[it.node];
             depth = std::max(depth, it.depth);
             uint32_t entry;
@@ -402,7 +436,8 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
                     VR_CHECK(h_indices[k] < num_prims, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: bad index");
                 entry = add_leaf(nd);
             } else {
-                VR_CHECK(nd.first + 1 < num_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_from_nodes: child index out of range");
+                VR_CHECK((size_t)nd.first + 1 < num_nodes, VR_ERR_INVALID_ARGUMENT,
+                         "vr_bvh_from_nodes: child index out of range");
                 const uint32_t p = (uint32_t)pairs.size();
                 pair_node pn;
                 std::memset(&pn, 0, sizeof(pn));
@@ -424,6 +459,8 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
             else pairs[(size_t)it.parent_pair].e[it.side] = entry;
         }
     }
+    int rc = ensure_device();
+    if (rc) return rc;
     vr_bvh h = new vr_bvh_t;
     bvh_state& s = h->s;
     s.device = current_device();
@@ -452,8 +489,8 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
             hipMemcpy(s.ref_nodes, h_nodes, num_nodes * sizeof(vr_bvh_node), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s.indices, h_indices, num_prims * 4, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(VR_ERR_HIP, "vr_bvh_from_nodes: upload failed"));
-        const int rc = collapse_quads(s, 0u, nullptr);
-        if (rc == VR_OK && hipDeviceSynchronize() != hipSuccess)
+        const int rc = collapse_quads(s, 0u, aux_stream());
+        if (rc == VR_OK && hipStreamSynchronize(aux_stream()) != hipSuccess)
             return bail(fail(VR_ERR_HIP, "vr_bvh_from_nodes: collapse failed"));
         if (rc) return bail(rc);
     }
@@ -466,7 +503,8 @@ int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream)
 {
     VR_CHECK(bvh && (bvh->s.num_prims == 0 || d_prims), VR_ERR_INVALID_ARGUMENT, "vr_bvh_refit: null argument");
     VR_CHECK(bvh->s.is_lbvh, VR_ERR_UNSUPPORTED, "vr_bvh_refit: only LBVHs are refitted on the device");
-    return lbvh_refit(bvh->s, d_prims, (hipStream_t)stream);
+    const int rc = lbvh_refit(bvh->s, d_prims, (hipStream_t)stream);
+    return rc ? rc : mark_built(bvh, (hipStream_t)stream);
 }
 
 int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info)
@@ -478,9 +516,10 @@ int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info)
     info->max_leaf_size = s.max_leaf;
     info->depth = s.depth;
     if (s.is_lbvh && s.num_prims > 1) {
+        // written by the build: wait for the build's stream up to the build
         uint32_t d = 0;
-        VR_HIP_CHECK(hipDeviceSynchronize());
-        VR_HIP_CHECK(hipMemcpy(&d, s.root_box + 6, 4, hipMemcpyDeviceToHost));
+        const int rc = read_after_build(bvh, &d, s.root_box + 6, 4);
+        if (rc) return rc;
         info->depth = d;
     }
     return VR_OK;
@@ -504,6 +543,36 @@ int vr_bvh_indices(vr_bvh bvh, uint32_t* d_indices, void* stream)
     const bvh_state& s = bvh->s;
     if (s.num_prims == 0) return VR_OK;
     VR_HIP_CHECK(hipMemcpyAsync(d_indices, s.indices, s.num_prims * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return VR_OK;
+}
+
+int vr_bvh_destroy_async(vr_bvh bvh, void* stream)
+{
+    if (!bvh) return VR_OK;
+    bvh_state& s = bvh->s;
+    void* ptrs[] = {s.pairs, s.quads, s.tris, s.prims, s.sorted_codes, s.sorted_ids, s.parents,
+                    s.flags, s.heights, s.ref_nodes, s.indices, s.root_box, s.scratch};
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(s.device);
+    if (s.pooled) {
+        // stream-ordered release: work enqueued on `stream` before this call
+        // still sees the arrays; the caller orders other streams' use before it
+        if (s.arena) {
+            (void)hipFreeAsync(s.arena, (hipStream_t)stream);
+        } else {
+            for (void* p : ptrs)
+                if (p) (void)hipFreeAsync(p, (hipStream_t)stream);
+        }
+    } else {
+        // plain allocations: wait for `stream` only, then free
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    if (bvh->built) (void)hipEventDestroy(bvh->built);
+    (void)hipSetDevice(cur);
+    delete bvh;
     return VR_OK;
 }
 
@@ -531,6 +600,7 @@ int vr_bvh_destroy(vr_bvh bvh)
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }
+    if (bvh->built) (void)hipEventDestroy(bvh->built);
     delete bvh;
     return VR_OK;
 }
@@ -717,6 +787,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
     std::vector<uint32_t> prefix(num_scenes + 1, 0);
     uint64_t total = 0, mats = 0;
     uint32_t depth = 0;
+    bool adjoint_ok = true;
     for (size_t i = 0; i < num_scenes; ++i) {
         const vr_scene& sc = h_scenes[i];
         VR_CHECK(sc.bvh && sc.d_accum && sc.d_color, VR_ERR_INVALID_ARGUMENT, "vr_batch_create: scene without bvh/target");
@@ -729,6 +800,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
         total += padded_pixel_count(sc.sp.width, sc.sp.height, g_shape.tile);
         mats += sc.num_materials;
         depth = std::max(depth, sc.bvh->s.depth);
+        adjoint_ok = adjoint_ok && sc.d_dl_dimage != nullptr && sc.kp.num_bounces <= ADJ_MAX_BOUNCES;
     }
     VR_CHECK(total < (1ull << 31), VR_ERR_INVALID_ARGUMENT, "vr_batch_create: more than 2^31 pixels");
     VR_CHECK(3u * depth + 1u <= (uint32_t)MAX_STACK, VR_ERR_UNSUPPORTED,
@@ -740,6 +812,7 @@ int vr_batch_create(const vr_scene* h_scenes, size_t num_scenes, vr_batch* out)
     b->total_pixels = (uint32_t)total;
     b->total_materials = (uint32_t)std::min<uint64_t>(mats, 0xFFFFFFFFu);
     b->max_depth = depth;
+    b->adjoint_ok = adjoint_ok;
     b->shape = g_shape;
     b->shape.schedule = VR_SCHEDULE_PERSISTENT;  // a batch is one persistent launch
     const uint32_t start = 0xFFFFFFFFu;
@@ -769,6 +842,10 @@ int vr_batch_render(vr_batch batch, uint64_t* d_ray_counts, void* stream)
 int vr_batch_backward(vr_batch batch, double* d_grad, uint64_t* d_ray_counts, void* stream)
 {
     VR_CHECK(batch && d_grad, VR_ERR_INVALID_ARGUMENT, "vr_batch_backward: null argument");
+    VR_CHECK(batch->adjoint_ok, VR_ERR_INVALID_ARGUMENT,
+             "vr_batch_backward: every scene needs d_dl_dimage and at most 16 bounces");
+    VR_CHECK(batch->total_materials <= ADJ_MAX_MATERIALS_TOTAL, VR_ERR_UNSUPPORTED,
+             "vr_batch_backward: more than 2048 materials over all scenes");
     return render_pt_batch(batch->d_scenes, batch->d_prefix, batch->num_scenes, batch->total_pixels, batch->d_frame,
                            false, reinterpret_cast<unsigned long long*>(d_ray_counts), batch->d_work,
                            batch->total_materials, d_grad, batch->shape, (hipStream_t)stream);
@@ -943,9 +1020,7 @@ int vr_bvh_download_nodes(vr_bvh bvh, vr_bvh_node* h_nodes, size_t* num_nodes)
     if (nn == 0) return VR_OK;
     VR_CHECK(s.ref_nodes, VR_ERR_UNSUPPORTED,
              "vr_bvh_download_nodes: build with VR_BVH_KEEP_NODES to keep the node layout");
-    VR_HIP_CHECK(hipDeviceSynchronize());
-    VR_HIP_CHECK(hipMemcpy(h_nodes, s.ref_nodes, nn * sizeof(vr_bvh_node), hipMemcpyDeviceToHost));
-    return VR_OK;
+    return read_after_build(bvh, h_nodes, s.ref_nodes, nn * sizeof(vr_bvh_node));
 }
 
 int vr_trace_closest(vr_bvh bvh, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)

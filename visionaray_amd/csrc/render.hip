@@ -187,7 +187,7 @@ __device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t
 // kernel modes; PT_NEE (a light list: next-event estimation with MIS) is
 // combined with PT_RENDER or PT_COUNT
 enum pt_mode { PT_RENDER = 0, PT_COUNT = 1, PT_ADJOINT = 2, PT_FROZEN_RR = 4, PT_NEE = 8 };
-constexpr int ADJ_MAX_VERTICES = 16;
+constexpr int ADJ_MAX_VERTICES = (int)ADJ_MAX_BOUNCES;
 
 // State of one pixel's path (the reference keeps it in the locals of
 // pathtracing::kernel and sample_pixel, detail/pathtracing.inl:43-233,
@@ -203,6 +203,11 @@ struct path_t {
     f3 wgt;           // adjoint: d loss / d sample
     uint32_t mids[ADJ ? ADJ_MAX_VERTICES : 1];
     uint32_t nmid;
+    // adjoint, per channel: product of the nonzero vertex factors over the
+    // roulette probabilities, count of zero factors, material and
+    // d factor / d cd of the first zero-factor vertex
+    f3 nzp, zg;
+    uint32_t zc[3], zm[3];
 };
 
 // Next-event estimation state of a lane: the query in flight is the shadow
@@ -229,7 +234,41 @@ __device__ __forceinline__ void start_sample(path_t<ADJ>& P)
     P.I = mk3(0, 0, 0);
     P.bounce = 0;
     P.rhit = false;
-    if (ADJ) P.nmid = 0;
+    if (ADJ) {
+        P.nmid = 0;
+        P.nzp = mk3(1, 1, 1);
+        P.zg = mk3(0, 0, 0);
+        P.zc[0] = P.zc[1] = P.zc[2] = 0;
+    }
+}
+
+// Adjoint: d I_c / d cd_{m,c} for the materials of the path's matte
+// vertices, given the ending term's emission E (ambient or ce * ls) and its
+// contribution e = T * E.  With f_{j,c} = cd_{m_j,c} * g_j (g_j = kd / pi *
+// cos / pdf) the derivative is E_c * sum_{j : m_j = m} g_j * prod_{k != j}
+// f_{k,c}: e_c * (vertices with m) / cd_{m,c} when no factor of channel c is
+// zero; only the zero vertex's material, g * (other factors) * E_c, when
+// exactly one is (a zero albedo channel gets its finite derivative, not
+// 0 / 0); nothing when two or more are.
+template <bool ADJ>
+__device__ __forceinline__ void adj_add_cd(const scene_desc& S, const path_t<ADJ>& P, f3 E, f3 e, float* s_grad)
+{
+    const float w[3] = {P.wgt.x, P.wgt.y, P.wgt.z};
+    const float ec[3] = {e.x, e.y, e.z};
+    const float Ec[3] = {E.x, E.y, E.z};
+    const float nz[3] = {P.nzp.x, P.nzp.y, P.nzp.z};
+    const float zg[3] = {P.zg.x, P.zg.y, P.zg.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (P.zc[c] == 0) {
+            for (uint32_t j = 0; j < P.nmid; ++j) {
+                const vr_material* mj = S.mats + P.mids[j];
+                atomicAdd(&s_grad[6 * (S.grad_base + P.mids[j]) + c], w[c] * ec[c] / mj->cd[c]);
+            }
+        } else if (P.zc[c] == 1) {
+            atomicAdd(&s_grad[6 * (S.grad_base + P.zm[c]) + c], w[c] * ((Ec[c] * zg[c]) * nz[c]));
+        }
+    }
 }
 
 // One bounce of pathtracing::kernel for the query (o, d) with result hr:
@@ -241,6 +280,7 @@ __device__ __forceinline__ void start_sample(path_t<ADJ>& P)
 // cd kd / pi * cos / pdf divided by the roulette probabilities, which are
 // held constant (unbiased for the gradient of the expected image):
 //   d I_c / d cd_{m,c} = I_c * (vertices with material m) / cd_{m,c}
+//     (leave-one-out products when a factor is zero: adj_add_cd)
 //   d I_c / d ce_{e,c} = T_c * ls_e.
 // Per-block FP32 partials in LDS; FP64 only in the final reduction.
 //
@@ -267,17 +307,10 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
     constexpr float INV_PI = 3.18309886183790691216444201928e-01f;
     if (hr.tri < 0) {
         // escaped: ambient light times throughput (pathtracing.inl:69-77)
-        const f3 e = mul3(mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]), P.tp);
+        const f3 amb = mk3(S.kp.ambient[0], S.kp.ambient[1], S.kp.ambient[2]);
+        const f3 e = mul3(amb, P.tp);
         P.I = add3(P.I, e);
-        if (ADJ) {
-            for (uint32_t j = 0; j < P.nmid; ++j) {
-                const vr_material* mj = S.mats + P.mids[j];
-                const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
-                atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
-                atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);
-                atomicAdd(&s_grad[slot + 2], P.wgt.z * e.z / mj->cd[2]);
-            }
-        }
+        if (ADJ) adj_add_cd(S, P, amb, e, s_grad);
         return true;
     }
     if (P.bounce == 0) P.rhit = true;
@@ -344,13 +377,7 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
             atomicAdd(&s_grad[es + 3], P.wgt.x * P.tp.x * ls);
             atomicAdd(&s_grad[es + 4], P.wgt.y * P.tp.y * ls);
             atomicAdd(&s_grad[es + 5], P.wgt.z * P.tp.z * ls);
-            for (uint32_t j = 0; j < P.nmid; ++j) {
-                const vr_material* mj = S.mats + P.mids[j];
-                const uint32_t slot = 6 * (S.grad_base + P.mids[j]);
-                atomicAdd(&s_grad[slot + 0], P.wgt.x * e.x / mj->cd[0]);
-                atomicAdd(&s_grad[slot + 1], P.wgt.y * e.y / mj->cd[1]);
-                atomicAdd(&s_grad[slot + 2], P.wgt.z * e.z / mj->cd[2]);
-            }
+            adj_add_cd(S, P, src, e, s_grad);
         }
     } else if (ADJ && !zero_pdf && P.nmid < ADJ_MAX_VERTICES) {
         P.mids[P.nmid++] = mid;
@@ -418,6 +445,28 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
     const float q = dot3(n, refl) / pdf;
     P.tp = mk3(P.tp.x * (src.x * q), P.tp.y * (src.y * q), P.tp.z * (src.z * q));
     if (zero_pdf) P.tp = mk3(0.0f, 0.0f, 0.0f);
+    if (ADJ && !emission && !zero_pdf) {
+        // the vertex's factors f_c = src_c * q: zero ones are counted, the
+        // others multiplied into nzp
+        const float f[3] = {src.x * q, src.y * q, src.z * q};
+        const float g = (m->kd * INV_PI) * q;
+        float nz[3] = {P.nzp.x, P.nzp.y, P.nzp.z};
+        float zg[3] = {P.zg.x, P.zg.y, P.zg.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (f[c] == 0.0f) {
+                if (P.zc[c] == 0) {
+                    P.zm[c] = mid;
+                    zg[c] = g;
+                }
+                P.zc[c] = P.zc[c] < 2u ? P.zc[c] + 1u : 2u;
+            } else {
+                nz[c] = nz[c] * f[c];
+            }
+        }
+        P.nzp = mk3(nz[0], nz[1], nz[2]);
+        P.zg = mk3(zg[0], zg[1], zg[2]);
+    }
     if (FROZEN) {
         P.tp0 = emission ? P.tp : mk3(P.tp0.x * (src0.x * q), P.tp0.y * (src0.y * q), P.tp0.z * (src0.z * q));
         if (zero_pdf) P.tp0 = mk3(0.0f, 0.0f, 0.0f);
@@ -432,6 +481,7 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
         if (terminate) active = false;
         P.tp = mk3(P.tp.x / prob, P.tp.y / prob, P.tp.z / prob);
         if (FROZEN) P.tp0 = mk3(P.tp0.x / prob, P.tp0.y / prob, P.tp0.z / prob);
+        if (ADJ) P.nzp = mk3(P.nzp.x / prob, P.nzp.y / prob, P.nzp.z / prob);
     }
     if (!active) return true;
     no = add3(pos, muls3(refl, S.kp.epsilon));
@@ -907,7 +957,7 @@ bool valid_shape(const vr_launch_shape& s)
 }
 
 // largest material table the adjoint keeps in LDS partials
-constexpr size_t ADJ_MAX_MATERIALS = 2048;
+constexpr size_t ADJ_MAX_MATERIALS = ADJ_MAX_MATERIALS_TOTAL;
 
 int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
                      const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,

@@ -46,6 +46,10 @@ int fail(int code, const std::string& msg);
 int ensure_device();
 
 struct bvh_state;
+// adjoint limits: path vertices kept per sample (num_bounces), materials in
+// the per-block LDS partials
+constexpr uint32_t ADJ_MAX_BOUNCES = 16;
+constexpr size_t ADJ_MAX_MATERIALS_TOTAL = 2048;
 // sah_cost of a BVH over its binary nodes (FP64 sums); synchronizes st
 int sah_cost_device(const bvh_state& s, float ci, float cl, float cp, double* out, hipStream_t st);
 // reference-layout node arrays on the host: sah_cost and tree rotations
@@ -563,4 +567,5 @@ int lbvh_build_host_boxes(const float* boxes, size_t n, int max_leaf_size, vr_bv
 // the state behind a vr_bvh handle
 struct vr_bvh_t {
     vr::bvh_state s;
+    hipEvent_t built = nullptr;  // recorded after the last build / rebuild / refit on its stream
 };
