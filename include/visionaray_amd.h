@@ -251,6 +251,17 @@ int vr_bvh_optimize_rotations_host(vr_bvh_node* h_nodes, size_t num_nodes, int* 
  * intersect<ClosestHit|AnyHit>, detail/bvh/intersect.inl:41-357). */
 int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
 
+/* vr_trace that allocates no device memory (no hipMalloc, hipFree or
+ * allocating library call in any call, the first included).  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Its work counter
+ * lives in a 256-byte file-scope device array resolved per device at vr_init
+ * (or at the first call on a device) instead of the library's counter ring.
+ * Limit: at most 2^31 rays per call (VR_ERR_INVALID_ARGUMENT beyond).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (other
+ * streams, other host threads) race; serialize them. */
+int vr_trace_noalloc(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream);
+
 /* Same, also counting per ray the traversal-node visits and triangle tests
  * (d_counts[2 * i] = nodes, d_counts[2 * i + 1] = triangles); for the
  * algorithmic-bytes model. */

@@ -60,6 +60,7 @@ SIGNATURES = {
     "vr_sah_build": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_lbvh_build_host": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_trace": (I, [P, I, P, SZ, P, P]),
+    "vr_trace_noalloc": (I, [P, I, P, SZ, P, P]),
     "vr_tlas_build": (I, [P, U32, P, U32, P, C.POINTER(P)]),
     "vr_tlas_closest_hit": (I, [P, P, SZ, P, P]),
     "vr_tlas_any_hit": (I, [P, P, SZ, P, P]),

@@ -19,6 +19,9 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st);
 int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hipStream_t st);
 int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
                 uint32_t* work, hipStream_t st);
+int trace_scratch_setup();
+int trace_batch_noalloc(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits,
+                        hipStream_t st);
 int render_pt(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
               const vr_camera& cam, const vr_sched_params& sp, float* accum, float* color,
               unsigned long long* ray_counts, uint32_t* work, bool counted, hipStream_t st);
@@ -179,7 +182,8 @@ int vr_init(int device)
     VR_HIP_CHECK(hipGetDeviceCount(&count));
     VR_CHECK(device >= 0 && device < count, VR_ERR_INVALID_ARGUMENT, "vr_init: no such device");
     VR_HIP_CHECK(hipSetDevice(device));
-    return ensure_device();
+    const int rc = ensure_device();
+    return rc ? rc : trace_scratch_setup();
 }
 
 const char* vr_last_error(void) { return g_error.c_str(); }
@@ -658,6 +662,12 @@ int vr_trace(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_
     int rc = ensure_device();
     if (rc) return rc;
     return trace_batch(bvh->s, any_hit != 0, d_rays, n, d_hits, nullptr, work_counter(), (hipStream_t)stream);
+}
+
+int vr_trace_noalloc(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, void* stream)
+{
+    VR_CHECK(bvh && (n == 0 || (d_rays && d_hits)), VR_ERR_INVALID_ARGUMENT, "vr_trace_noalloc: null argument");
+    return trace_batch_noalloc(bvh->s, any_hit != 0, d_rays, n, d_hits, (hipStream_t)stream);
 }
 
 int vr_trace_counted(vr_bvh bvh, int any_hit, const vr_ray* d_rays, size_t n, vr_hit* d_hits, uint32_t* d_counts,

@@ -12,10 +12,28 @@
 #include "vr_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace vr {
 
+// Scratch of vr_trace_noalloc: the work counter of its resident grid (the
+// other entry points take theirs from a counter ring the library allocates
+// on a device's first use).  A file-scope device array, so that entry point
+// allocates nothing; every call on a device shares that device's copy and
+// keeps the last call's contents (the counter is cleared by a fill kernel at
+// the start of each call), so calls that overlap on one GPU race.  256 bytes:
+// word 0 is the counter, the rest of the line stays unused so the counter
+// shares its cache line with nothing.
+constexpr uint32_t NOALLOC_SCRATCH_WORDS = 64;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
 namespace {
+
+// the device address of `scratch` on each device (the array's name is a
+// host-side stand-in: only these addresses may reach kernels and copies)
+constexpr int NOALLOC_MAX_DEVICES = 64;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
 
 constexpr int BLOCK = 256;
 constexpr uint32_t CHUNK = 64;
@@ -115,6 +133,40 @@ int launch_trace(const bvh_state& bs, const vr_ray* rays, uint32_t n, vr_hit* hi
 }
 
 }  // namespace
+
+int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
+                uint32_t* work, hipStream_t st);
+
+// Look up the current device's address of `scratch` once (vr_init, else the
+// first vr_trace_noalloc call on the device).
+int trace_scratch_setup()
+{
+    int dev = 0;
+    VR_HIP_CHECK(hipGetDevice(&dev));
+    VR_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, VR_ERR_UNSUPPORTED, "vr_trace_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev]) return VR_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    VR_CHECK(e == hipSuccess && p != nullptr, VR_ERR_HIP,
+             std::string("vr_trace_noalloc: hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    return VR_OK;
+}
+
+// vr_trace with the work counter in `scratch`: no device allocation.
+int trace_batch_noalloc(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits,
+                        hipStream_t st)
+{
+    VR_CHECK(n <= (1ull << 31), VR_ERR_INVALID_ARGUMENT,
+             "vr_trace_noalloc: more than 2^31 rays in one call (the scratch holds one 32-bit work counter)");
+    if (n == 0) return VR_OK;
+    const int rc = trace_scratch_setup();
+    if (rc) return rc;
+    int dev = 0;
+    VR_HIP_CHECK(hipGetDevice(&dev));
+    return trace_batch(bs, any_hit, rays, n, hits, nullptr, g_scratch_ptr[dev], st);
+}
 
 int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
                 uint32_t* work, hipStream_t st)
