@@ -203,6 +203,22 @@ int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream);
  * the handle's last build / rebuild / refit (a wait on that work's stream up
  * to it, not on the whole device). */
 int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info);
+
+/* Device view of a BVH for kernels written against the device API
+ * (visionaray_amd_device.hpp): tree.ref() (bvh.h:919-931), the POD
+ * index_bvh_ref_t (bvh.h:495-569) kernels take by value.  Non-owning: valid
+ * while the handle lives and is not rebuilt. */
+typedef struct vr_bvh_ref {
+    const void* nodes;   /* 128-byte 4-wide traversal nodes */
+    const void* prims;   /* 48-byte triangles in traversal order */
+    const void* pairs;   /* 64-byte binary node pairs */
+    uint32_t num_prims;
+    uint32_t root;       /* entry of the root node */
+    uint32_t num_nodes;  /* traversal nodes (= pair records) */
+    uint32_t reserved;
+} vr_bvh_ref;
+/* VR_ERR_UNSUPPORTED for a tree deeper than the traversal stack. */
+int vr_bvh_get_ref(vr_bvh bvh, vr_bvh_ref* out);
 /* Reference-layout nodes (tree.nodes(), bvh.h:910) into d_nodes[num_nodes]. */
 int vr_bvh_nodes(vr_bvh bvh, vr_bvh_node* d_nodes, void* stream);
 /* tree.indices() (bvh.h:913) into d_indices[num_prims]. */

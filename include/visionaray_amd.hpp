@@ -132,6 +132,13 @@ public:
         check(vr_bvh_get_info(h_, &i));
         return i;
     }
+    // tree.ref(): the device view kernels take (visionaray_amd_device.hpp)
+    vr_bvh_ref ref() const
+    {
+        vr_bvh_ref r{};
+        check(vr_bvh_get_ref(h_, &r));
+        return r;
+    }
     size_t num_primitives() const { return info().num_prims; }
     size_t num_nodes() const { return info().num_nodes; }
     // tree.nodes() / tree.indices() copied to the host
@@ -405,6 +412,11 @@ public:
                            stream));
         ++frame_id_;
     }
+    // Any kernel functor (a user kernel): one thread per pixel under the
+    // jittered-blend sampler.  Defined in visionaray_amd_device.hpp (HIP code).
+    template <typename K>
+    void frame(K kernel, sched_params sp, void* stream = nullptr);
+
     unsigned frame_id() const { return frame_id_; }
 
     // Path-replay adjoint of the last frame (no reference counterpart):

@@ -53,6 +53,7 @@ SIGNATURES = {
     "vr_bvh_from_nodes": (I, [P, SZ, P, SZ, P, C.POINTER(P)]),
     "vr_bvh_refit": (I, [P, P, P]),
     "vr_bvh_get_info": (I, [P, P]),
+    "vr_bvh_get_ref": (I, [P, P]),
     "vr_bvh_nodes": (I, [P, P, P]),
     "vr_bvh_indices": (I, [P, P, P]),
     "vr_bvh_destroy": (I, [P]),

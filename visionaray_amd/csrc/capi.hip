@@ -529,6 +529,22 @@ int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info)
     return VR_OK;
 }
 
+int vr_bvh_get_ref(vr_bvh bvh, vr_bvh_ref* out)
+{
+    VR_CHECK(bvh && out, VR_ERR_INVALID_ARGUMENT, "vr_bvh_get_ref: null argument");
+    const bvh_state& s = bvh->s;
+    VR_CHECK(stack_fits(s), VR_ERR_UNSUPPORTED, "vr_bvh_get_ref: BVH deeper than the traversal stack");
+    const bvh_view v = s.view();
+    out->nodes = v.quads;
+    out->prims = v.tris;
+    out->pairs = v.pairs;
+    out->num_prims = v.num_prims;
+    out->root = v.root;
+    out->num_nodes = v.num_quads;
+    out->reserved = 0;
+    return VR_OK;
+}
+
 int vr_bvh_nodes(vr_bvh bvh, vr_bvh_node* d_nodes, void* stream)
 {
     VR_CHECK(bvh && d_nodes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_nodes: null argument");

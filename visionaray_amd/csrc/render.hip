@@ -23,90 +23,6 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-// make_random_seed (make_random_seed.h:23-37)
-__device__ inline uint32_t make_random_seed(int32_t val0, int32_t val1)
-{
-    uint32_t v0 = (uint32_t)val0, v1 = (uint32_t)val1, s0 = 0;
-#pragma unroll
-    for (uint32_t n = 0; n < 4; ++n) {
-        s0 += 0x9E3779B9u;
-        v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xC8013EA4u);
-        v1 += ((v0 << 4) + 0xAd90777Du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7E95761Eu);
-    }
-    return v0;
-}
-
-// random_generator<float> as built for the host (random_generator.h:29-62):
-// minstd_rand0 (x <- 16807 x mod 2^31 - 1; seed mod 2^31 - 1, zero -> 1) with
-// uniform_real_distribution<float>(0, 1) through generate_canonical<float, 24>,
-// one engine draw per sample: float(x - 1) / 2^31, clamped below 1.
-struct rng_t {
-    uint32_t x;
-};
-__device__ inline void rng_seed(rng_t& g, uint32_t s)
-{
-    const uint32_t x = s % 2147483647u;
-    g.x = x == 0 ? 1u : x;
-}
-__device__ inline float rng_next(rng_t& g)
-{
-    const uint64_t t = (uint64_t)g.x * 16807u;
-    uint32_t r = (uint32_t)(t & 0x7FFFFFFFu) + (uint32_t)(t >> 31);  // 2^31 == 1 mod 2^31 - 1
-    if (r >= 0x7FFFFFFFu) r -= 0x7FFFFFFFu;
-    g.x = r;
-    const float f = (float)(r - 1u) * 4.656612873077392578125e-10f;  // exact 2^-31 scaling
-    return f >= 1.0f ? 0x1.fffffep-1f : f;
-}
-// two draws for one argument list, in the order the host compiler evaluated
-// the reference's `f(gen.next(), gen.next())`: arg_order 1 = right first
-__device__ inline void rng_pair(rng_t& g, uint32_t arg_order, float& first_arg, float& second_arg)
-{
-    const float a = rng_next(g);
-    const float b = rng_next(g);
-    first_arg = arg_order ? b : a;
-    second_arg = arg_order ? a : b;
-}
-
-// make_primary_ray with jitter (detail/sched_common.h:209-233) and
-// pinhole_camera::primary_ray (detail/pinhole_camera.inl:103-121), image region
-// (0,0)-(1,1)
-__device__ inline void primary_ray(const vr_camera& cam, rng_t& g, uint32_t arg_order, int x, int y, int w, int h,
-                                   f3& o, f3& d)
-{
-    float a, b;
-    rng_pair(g, arg_order, a, b);
-    const float jx = a - 0.5f, jy = b - 0.5f;
-    const float fx = (float)x + jx, fy = (float)y + jy;
-    float sx = (fx + 0.5f) / (float)w;
-    float sy = (fy + 0.5f) / (float)h;
-    sx = (1.0f - sx) * 0.0f + sx * 1.0f;
-    sy = (1.0f - sy) * 0.0f + sy * 1.0f;
-    const float u = 2.0f * sx - 1.0f;
-    const float v = 2.0f * sy - 1.0f;
-    const f3 U = mk3(cam.U[0], cam.U[1], cam.U[2]);
-    const f3 V = mk3(cam.V[0], cam.V[1], cam.V[2]);
-    const f3 W = mk3(cam.W[0], cam.W[1], cam.W[2]);
-    d = normalize3(add3(add3(muls3(U, u), muls3(V, v)), W));
-    o = mk3(cam.eye[0], cam.eye[1], cam.eye[2]);
-}
-
-// cosine_sample_hemisphere (sampling.h:117-127)
-__device__ inline f3 cosine_hemisphere(float u1, float u2)
-{
-    const float r = sqrtf(u1);
-    const float theta = 6.28318530717958623199592693709e+00f * u2;
-    float sn, cs;
-    sincosf(theta, &sn, &cs);
-    return mk3(r * cs, r * sn, sqrtf(1.0f - u1));
-}
-
-// make_orthonormal_basis (math/detail/vector3.inl:356-366; brdf.h:62-66)
-__device__ inline void onb(f3 w, f3& u, f3& v)
-{
-    v = fabsf(w.x) > fabsf(w.y) ? normalize3(mk3(-w.z, 0.0f, w.x)) : normalize3(mk3(0.0f, w.z, -w.y));
-    u = cross3(v, w);
-}
-
 __device__ inline f3 ldtri3(float4 q) { return mk3(q.x, q.y, q.z); }
 
 __device__ inline uint64_t lane_mask_lt_count(uint64_t m)
@@ -160,28 +76,6 @@ __device__ inline void write_pixel(const scene_desc& S, uint32_t px, float4 c)
                                  c.w * sf + old.w * df);
     *acc = r;
     reinterpret_cast<float4*>(S.color)[px] = r;
-}
-
-// Pixel order of the work queue: tile x tile squares row by row (tile <= 1:
-// scanlines).  The queue runs over whole tiles; indices in the padding of a
-// partial tile map to no pixel.
-__host__ __device__ inline uint32_t padded_pixels(uint32_t w, uint32_t h, uint32_t tile)
-{
-    if (tile <= 1) return w * h;
-    return ((w + tile - 1) / tile) * ((h + tile - 1) / tile) * tile * tile;
-}
-__device__ inline bool tile_pixel(uint32_t idx, uint32_t w, uint32_t h, uint32_t tile, int& x, int& y)
-{
-    if (tile <= 1) {
-        x = (int)(idx % w);
-        y = (int)(idx / w);
-        return true;
-    }
-    const uint32_t tiles_x = (w + tile - 1) / tile, tt = tile * tile;
-    const uint32_t t = idx / tt, o = idx % tt;
-    x = (int)((t % tiles_x) * tile + o % tile);
-    y = (int)((t / tiles_x) * tile + o / tile);
-    return (uint32_t)x < w && (uint32_t)y < h;
 }
 
 // kernel modes; PT_NEE (a light list: next-event estimation with MIS) is
@@ -821,7 +715,7 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
         f3 o, d;
         primary_ray(A.cam, g, A.arg_order, x, y, (int)w, (int)h, o, d);
         float4 c = make_float4(A.bg[0], A.bg[1], A.bg[2], 1.0f);
-        const hit_t hr = traverse<false, false, LDS_STACK, BLOCK>(A.b, o, d, 0.0f, FLT_MAX, st, nullptr);
+        const hit_t hr = traverse<false, false, LDS_STACK, BLOCK, TRAV_LANE>(A.b, o, d, 0.0f, FLT_MAX, st, nullptr);
         if (hr.tri >= 0) {
             const f3 pos = add3(o, muls3(d, hr.t));
             const float4* tq = reinterpret_cast<const float4*>(A.b.tris + hr.tri);
@@ -836,7 +730,7 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
                 const f3 sp = cosine_hemisphere(a1, a2);
                 const f3 dir = normalize3(add3(add3(smul3(sp.x, u), smul3(sp.y, v)), smul3(sp.z, n)));
                 const f3 ao_o = add3(pos, muls3(dir, 1e-3f));
-                const hit_t ah = traverse<true, false, LDS_STACK, BLOCK>(A.b, ao_o, dir, 0.0f, A.radius, st, nullptr);
+                const hit_t ah = traverse<true, false, LDS_STACK, BLOCK, TRAV_LANE>(A.b, ao_o, dir, 0.0f, A.radius, st, nullptr);
                 if (ah.tri >= 0) ao = ao + inc;
             }
             ao = 1.0f - ao;
@@ -943,7 +837,7 @@ pt_args single_args(const bvh_state& bs, const vr_material* mats, size_t num_mat
 
 // argument evaluation order of the reference host build: GCC evaluates
 // f(gen.next(), gen.next()) right to left (1)
-uint32_t g_arg_order = 1;
+uint32_t g_arg_order = REFERENCE_ARG_ORDER;
 
 // current launch shape of the frame kernel (vr_set_launch_shape / autotuner)
 vr_launch_shape g_shape = {256, 16, 1, VR_SCHEDULE_PERSISTENT};

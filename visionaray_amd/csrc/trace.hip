@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
             }
         }
         if (__ballot(!done) == 0) break;
-        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK>(T, b, st, spill, &c)) {
+        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK, COUNT ? TRAV_LANE : TRAV_VOTED>(T, b, st, spill, &c)) {
             float4 o;
             if (T.r.tri >= 0) {
                 const float prim = b.tris[T.r.tri].a.w;  // prim_id bits

@@ -1,9 +1,7 @@
-// vr_internal.h -- device data layouts, exact-arithmetic helpers and the BVH
-// traversal shared by every kernel of the library.
-//
-// All HIP code of the library is compiled with -ffp-contract=off: each float
-// operation below rounds exactly as written, in the order the reference writes
-// it, so results are bit-identical to the reference's CPU arithmetic.
+// vr_internal.h -- host-side state and helpers shared by the library's
+// sources: error handling, the state behind a vr_bvh handle, launch helpers.
+// The device layouts, exact arithmetic and the traversal are in the public
+// visionaray_amd_core.h (user kernels compile the same code).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,6 +11,7 @@
 #include <string>
 
 #include "../../include/visionaray_amd.h"
+#include "../../include/visionaray_amd_core.h"
 
 namespace vr {
 
@@ -58,70 +57,6 @@ int optimize_rotations_host(vr_bvh_node* nodes, size_t n, int* count);
 // SAH-driven tree rotations of a freshly built LBVH (before its collapse)
 int lbvh_rotate(bvh_state& s, uint32_t leaf_max, uint32_t* rotations, hipStream_t st);
 
-// ---------------------------------------------------------------------------
-// device layouts
-// ---------------------------------------------------------------------------
-
-// Traversal node: the two children of one inner node in one 64-byte record,
-// so an inner visit is a single cache-line fetch of four 16-byte loads.
-//   f[0..5]   child 0 box  lo.x lo.y lo.z hi.x hi.y hi.z
-//   f[6..11]  child 1 box
-//   e[0], e[1] child entries (see below); e[2] parent pair (-1 at the root
-//   pair); e[3] for an LBVH the far end j of the pair's leaf range
-//   [min(index, j), max(index, j)], unused otherwise.
-// Child entry: bit 31 clear -> inner node, the index of its pair record;
-// bit 31 set -> leaf covering triangles [first, first + count) with
-// first = entry & 0x03FFFFFF and count = ((entry >> 26) & 31) + 1.
-struct alignas(64) pair_node {
-    float f[12];
-    uint32_t e[4];
-};
-static_assert(sizeof(pair_node) == 64, "pair_node is one 64-byte line");
-
-constexpr uint32_t LEAF_BIT = 0x80000000u;
-constexpr uint32_t LEAF_FIRST_MASK = 0x03FFFFFFu;
-constexpr uint32_t MAX_LEAF_FIRST = LEAF_FIRST_MASK;
-constexpr uint32_t MAX_LEAF_COUNT = 32;
-
-__host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count)
-{
-    return LEAF_BIT | ((count - 1u) << 26) | first;
-}
-
-// Traversal node: up to four children in one 128-byte line (eight 16-byte
-// loads), boxes as structure of arrays so each load carries one coordinate
-// of all four children.  It is a collapse of the binary tree (pair records):
-// every child box is the box of a binary node, the same exact union.
-//   lo_x .. hi_z  child c's box component in lane c; an unused slot has
-//                 lo = hi = +inf, a box no ray enters
-//   child[c]      entry as in pair_node (inner: quad index)
-//   meta.x        number of children (2..4); meta.y..w unused
-struct alignas(128) quad_node {
-    float4 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z;
-    uint4 child;
-    uint4 meta;
-};
-static_assert(sizeof(quad_node) == 128, "quad_node is one 128-byte line");
-
-// Triangle in traversal order, 48 bytes = three 16-byte loads:
-//   a = v1.xyz, prim_id    b = e1.xyz, geom_id    c = e2.xyz, original index
-struct alignas(16) tri_node {
-    float4 a;
-    float4 b;
-    float4 c;
-};
-static_assert(sizeof(tri_node) == 48, "tri_node layout");
-
-// Device view of a BVH, passed by value into kernels (the analogue of
-// index_bvh_ref_t, bvh.h:495-569).
-struct bvh_view {
-    const quad_node* quads;
-    const tri_node* tris;
-    const pair_node* pairs;  // the binary tree (traverse_binary)
-    uint32_t num_prims;
-    uint32_t root;  // entry of the root: 0 (quad 0 / pair 0) or a leaf entry; unused when num_prims == 0
-    uint32_t num_quads;  // also the number of pair records
-};
 
 // Host-side state behind a vr_bvh handle.
 struct bvh_state {
@@ -156,387 +91,6 @@ struct bvh_state {
     bvh_view view() const { return bvh_view{quads, tris, pairs, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
 };
 
-// ---------------------------------------------------------------------------
-// exact scalar helpers (reference semantics)
-// ---------------------------------------------------------------------------
-
-// math/detail/math.h:116-128 ternary min/max
-__host__ __device__ inline float tmin_f(float x, float y) { return x < y ? x : y; }
-__host__ __device__ inline float tmax_f(float x, float y) { return x < y ? y : x; }
-
-struct f3 {
-    float x, y, z;
-};
-__host__ __device__ inline f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
-__host__ __device__ inline f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__host__ __device__ inline f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__host__ __device__ inline f3 mul3(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
-__host__ __device__ inline f3 muls3(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
-__host__ __device__ inline f3 smul3(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
-__host__ __device__ inline f3 neg3(f3 a) { return f3{-a.x, -a.y, -a.z}; }
-__host__ __device__ inline f3 min3v(f3 a, f3 b) { return f3{tmin_f(a.x, b.x), tmin_f(a.y, b.y), tmin_f(a.z, b.z)}; }
-__host__ __device__ inline f3 max3v(f3 a, f3 b) { return f3{tmax_f(a.x, b.x), tmax_f(a.y, b.y), tmax_f(a.z, b.z)}; }
-// math/detail/vector3.inl:292-307
-__host__ __device__ inline f3 cross3(f3 u, f3 v)
-{
-    return f3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
-}
-__host__ __device__ inline float dot3(f3 u, f3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
-// normalize = v * (1 / sqrt(dot(v, v))), both correctly rounded
-// (math/detail/vector3.inl:332-335, math/detail/math.h:596-599)
-__host__ __device__ inline f3 normalize3(f3 v) { return muls3(v, 1.0f / sqrtf(dot3(v, v))); }
-// math/detail/vector3.inl:432-447
-__host__ __device__ inline float min_el3(f3 u)
-{
-    float r = u.x;
-    if (u.y < r) r = u.y;
-    if (u.z < r) r = u.z;
-    return r;
-}
-__host__ __device__ inline float max_el3(f3 u)
-{
-    float r = u.x;
-    if (u.y > r) r = u.y;
-    if (u.z > r) r = u.z;
-    return r;
-}
-// math/detail/vector.inl:679-686
-__host__ __device__ inline f3 faceforward3(f3 n, f3 i, f3 nref) { return dot3(nref, i) < 0.0f ? neg3(n) : n; }
-
-// ---------------------------------------------------------------------------
-// ray / triangle: Moller-Trumbore, two-sided, inclusive edges
-// (math/intersect.h:176-229); returns hit and writes t, b1, b2
-// ---------------------------------------------------------------------------
-__device__ inline bool intersect_tri(f3 ori, f3 dir, f3 v1, f3 e1, f3 e2, float& t, float& b1, float& b2)
-{
-    f3 s1 = cross3(dir, e2);
-    float div = dot3(s1, e1);
-    if (!(div != 0.0f)) return false;
-    float inv_div = 1.0f / div;
-    f3 d = sub3(ori, v1);
-    b1 = dot3(d, s1) * inv_div;
-    if (!(b1 >= 0.0f && b1 <= 1.0f)) return false;
-    f3 s2 = cross3(d, e1);
-    b2 = dot3(dir, s2) * inv_div;
-    if (!(b2 >= 0.0f && b1 + b2 <= 1.0f)) return false;
-    t = dot3(e2, s2) * inv_div;
-    return true;
-}
-
-struct hit_t {
-    float t;
-    float u;
-    float v;
-    int32_t tri;  // traversal position of the hit triangle, -1 = miss
-};
-
-// ---------------------------------------------------------------------------
-// BVH traversal (detail/bvh/intersect.inl:166-356) over the 4-wide collapse.
-//
-// The root box is not tested.  At a node every child box is slab tested
-// exactly as the reference tests a binary node's children: tnear =
-// max(tmin, max_el(min(tlo, thi))) and tfar = min(tmax, min_el(max(tlo,
-// thi))) with tlo = (lo - ori) * inv_dir, and the child is entered when
-// tfar >= tnear and tnear < t_closest.  Entered children are visited nearest
-// first (a sorting network on tnear); the others are deferred on the stack,
-// farthest deepest.  Primitives are accepted with is_closer (update_if.h:
-// 68-107): hit && t < t_closest && tmin <= t <= tmax.  The closest hit is the
-// reference's; only which of several triangles hit at exactly the same t is
-// reported can differ (the reference keeps the first found in its binary
-// near-first order).
-//
-// The stack keeps its top LDS_DEPTH entries in LDS, one column per lane
-// (stack[k * STRIDE], so the 64 lanes of a wave touch 64 consecutive words);
-// deeper entries go to a per-lane private array that only rays on unusually
-// deep paths touch.  A node defers at most three children, so MAX_STACK
-// entries cover trees of depth (MAX_STACK - 1) / 3 (checked on the host).
-// ---------------------------------------------------------------------------
-
-struct trace_counters {
-    uint32_t nodes;
-    uint32_t tris;
-};
-
-// a Karras tree has at most 65 levels (the common prefix of a node's range,
-// 30 code bits + 32 index bits, grows strictly from parent to child), so its
-// collapse at most 64 quad levels: 3 * 64 + 1 entries always suffice
-constexpr int MAX_STACK = 196;
-
-// Slab test of one child box; returns tnear, or +inf when the child is not
-// entered.  The slab distances round exactly as the reference's; the min/max
-// reductions use the hardware min/max (v_min3/v_max3), which agree with the
-// reference's ternary forms on every non-NaN input up to the sign of a zero,
-// and a signed zero changes no comparison below.
-__device__ __forceinline__ float slab_enter(float lx, float ly, float lz, float hx, float hy, float hz, f3 ori,
-                                            f3 inv, float tmin, float tmax, float tclosest)
-{
-    const float x0 = (lx - ori.x) * inv.x, x1 = (hx - ori.x) * inv.x;
-    const float y0 = (ly - ori.y) * inv.y, y1 = (hy - ori.y) * inv.y;
-    const float z0 = (lz - ori.z) * inv.z, z1 = (hz - ori.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-    return (tf >= tn && tn < tclosest) ? tn : __builtin_inff();
-}
-
-__device__ __forceinline__ void cswap(float& ka, uint32_t& ea, float& kb, uint32_t& eb)
-{
-    const bool s = kb < ka;
-    const float k = s ? kb : ka;
-    kb = s ? ka : kb;
-    ka = k;
-    const uint32_t e = s ? eb : ea;
-    eb = s ? ea : eb;
-    ea = e;
-}
-
-// One query in flight.  A query advances one iteration at a time
-// (trav_iter) so that a lane can hold its query while other lanes of the
-// wave shade or start new ones; traverse() runs one query to the end.
-struct trav_state {
-    f3 ori, dir, inv;
-    float tmin, tmax;
-    uint32_t cur;     // entry being visited
-    int sp;           // deferred entries on the stack
-    uint32_t budget;  // visits left: a ray visits each node at most once
-    uint32_t lf, ln;  // trav_iter: triangles [lf, lf + ln) of a leaf still to test
-    hit_t r;
-};
-
-__device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax)
-{
-    T.ori = ori;
-    T.dir = dir;
-    T.inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
-                dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
-    T.tmin = tmin;
-    T.tmax = tmax;
-    T.cur = b.root;
-    T.sp = 0;
-    T.budget = 2u * b.num_prims + 2u;
-    T.lf = 0;
-    T.ln = 0;
-    T.r.t = FLT_MAX;
-    T.r.u = 0.0f;
-    T.r.v = 0.0f;
-    T.r.tri = -1;
-}
-
-// entry of a finished query (never a valid leaf: first + count would exceed
-// any primitive count)
-constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
-
-// node iterations are chosen while at least this many times as many lanes
-// have node work as have a triangle to test (a node visit costs about twice a
-// triangle test)
-// persistent ray batches: idle lanes take new rays once this many of their
-// wave wait (fewer refills, each serving more lanes); measured best on the
-// large scene's bounce rays: 32 for the one-level trace, 16 for the
-// two-level one
-#ifndef VR_TRACE_REFILL_LANES
-#define VR_TRACE_REFILL_LANES 32
-#endif
-#ifndef VR_TLAS_REFILL_LANES
-#define VR_TLAS_REFILL_LANES 16
-#endif
-#ifndef VR_TRAV_NODE_BIAS
-#define VR_TRAV_NODE_BIAS 2.0f
-#endif
-
-// One iteration of the query, either one node visit or one triangle test,
-// whichever more lanes of the wave can do (the wave votes).  A lane keeps
-// one leaf's triangle range and, after taking it, goes on descending (the
-// leaf's triangles wait for a triangle iteration), so lanes that meet leaves
-// at different depths stay busy; only the order in which triangles are
-// tested changes (Aila and Laine 2009, speculative traversal).  Returns true
-// when this lane's query is finished (T.r is its result).
-// any_rt: this query is an any-hit query (a shadow ray) although the kernel
-// traces closest hits otherwise.
-template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
-__device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
-                                          uint32_t* __restrict__ spill, trace_counters* cnt, bool any_rt = false)
-{
-    if (b.num_prims == 0) return true;
-    auto push = [&](uint32_t e) {
-        if (T.sp < LDS_DEPTH) stack[T.sp * STRIDE] = e;
-        else spill[T.sp - LDS_DEPTH] = e;
-        ++T.sp;
-    };
-    auto pop = [&]() -> uint32_t {
-        if (T.sp == 0) return TRAV_DONE;
-        --T.sp;
-        return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
-    };
-    // a leaf at cur goes to the free triangle range
-    if (T.ln == 0 && (T.cur & LEAF_BIT) && T.cur != TRAV_DONE) {
-        T.lf = T.cur & LEAF_FIRST_MASK;
-        T.ln = ((T.cur >> 26) & 31u) + 1u;
-        if (T.lf + T.ln > b.num_prims) return true;  // corrupt entry
-        T.cur = pop();
-    }
-    const bool node_work = !(T.cur & LEAF_BIT);
-    if (!node_work && T.ln == 0) return true;  // cur is TRAV_DONE: finished
-    const uint32_t n_node = (uint32_t)__popcll(__ballot(node_work));
-    const uint32_t n_tri = (uint32_t)__popcll(__ballot(T.ln > 0));
-    if ((float)n_node >= VR_TRAV_NODE_BIAS * (float)n_tri) {
-        if (!node_work) return false;
-        if (--T.budget == 0 || T.cur >= b.num_quads) return true;
-        const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
-        const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
-        const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
-        if (COUNT) cnt->nodes++;
-        const float tc = T.r.t;
-        float k0 = slab_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T.ori, T.inv, T.tmin, T.tmax, tc);
-        float k1 = slab_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, T.ori, T.inv, T.tmin, T.tmax, tc);
-        float k2 = slab_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, T.ori, T.inv, T.tmin, T.tmax, tc);
-        float k3 = slab_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, T.ori, T.inv, T.tmin, T.tmax, tc);
-        uint32_t e0 = ch.x, e1 = ch.y, e2 = ch.z, e3 = ch.w;
-        cswap(k0, e0, k1, e1);
-        cswap(k2, e2, k3, e3);
-        cswap(k0, e0, k2, e2);
-        cswap(k1, e1, k3, e3);
-        cswap(k1, e1, k2, e2);
-        if (!(k0 < __builtin_inff())) {
-            T.cur = pop();
-        } else {
-            // deferred children, farthest deepest: e_k goes to sp + n - k
-            const int n = (k1 < __builtin_inff()) + (k2 < __builtin_inff()) + (k3 < __builtin_inff());
-            if (T.sp + 3 <= LDS_DEPTH) {
-                if (n >= 1) stack[(T.sp + n - 1) * STRIDE] = e1;
-                if (n >= 2) stack[(T.sp + n - 2) * STRIDE] = e2;
-                if (n >= 3) stack[T.sp * STRIDE] = e3;
-                T.sp += n;
-            } else {
-                if (n >= 3) push(e3);
-                if (n >= 2) push(e2);
-                if (n >= 1) push(e1);
-            }
-            T.cur = e0;
-        }
-        return false;
-    }
-    if (T.ln == 0) return false;
-    const uint32_t k = T.lf;
-    ++T.lf;
-    --T.ln;
-    const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
-    const float4 ta = tp[0], tb = tp[1], tq = tp[2];
-    if (COUNT) cnt->tris++;
-    float t, u, v;
-    if (intersect_tri(T.ori, T.dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
-        if (t < T.r.t && t >= T.tmin && t <= T.tmax) {
-            T.r.t = t;
-            T.r.u = u;
-            T.r.v = v;
-            T.r.tri = (int32_t)k;
-            if (ANY_HIT || any_rt) return true;
-        }
-    }
-    return false;
-}
-
-template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE>
-__device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
-                                          uint32_t* __restrict__ stack, trace_counters* cnt)
-{
-    trav_state T;
-    uint32_t spill[MAX_STACK - LDS_DEPTH];
-    trav_begin(T, b, ori, dir, tmin, tmax);
-    while (!trav_iter<ANY_HIT, COUNT, LDS_DEPTH, STRIDE>(T, b, stack, spill, cnt)) {
-    }
-    return T.r;
-}
-
-// The reference's traversal of the binary tree as written
-// (detail/bvh/intersect.inl:166-356, device order): at a pair both child
-// boxes are slab tested with the reference's exact arithmetic, the nearer
-// entered child (tnear0 < tnear1 ? 0 : 1) is visited first and the other is
-// deferred; triangles are accepted with is_closer.  Used for the
-// reference-scheme comparison launch, not by the fast paths.
-template <bool ANY_HIT, int LDS_DEPTH, int STRIDE>
-__device__ __forceinline__ hit_t traverse_binary(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
-                                                 uint32_t* __restrict__ stack)
-{
-    hit_t r;
-    r.t = FLT_MAX;
-    r.u = 0.0f;
-    r.v = 0.0f;
-    r.tri = -1;
-    if (b.num_prims == 0) return r;
-    const f3 inv = mk3(dir.x != 0.0f ? 1.0f / dir.x : FLT_MAX, dir.y != 0.0f ? 1.0f / dir.y : FLT_MAX,
-                       dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
-    int sp = 0;
-    uint32_t cur = b.root;
-    uint32_t spill[MAX_STACK - LDS_DEPTH];
-    auto push = [&](uint32_t e) {
-        if (sp < LDS_DEPTH) stack[sp * STRIDE] = e;
-        else spill[sp - LDS_DEPTH] = e;
-        ++sp;
-    };
-    auto pop = [&]() -> uint32_t {
-        --sp;
-        return sp < LDS_DEPTH ? stack[sp * STRIDE] : spill[sp - LDS_DEPTH];
-    };
-    // a ray visits each node at most once; a longer walk means a corrupt tree
-    uint32_t budget = 2u * b.num_prims + 2u;
-    for (;;) {
-        while (!(cur & LEAF_BIT)) {
-            if (--budget == 0 || cur >= b.num_quads) return r;
-            const float4* p = reinterpret_cast<const float4*>(b.pairs + cur);
-            const float4 a0 = p[0], q1 = p[1], q2 = p[2];
-            const uint4 q3 = *reinterpret_cast<const uint4*>(p + 3);
-            float tn0, tf0, tn1, tf1;
-            {
-                const f3 tlo = mul3(sub3(mk3(a0.x, a0.y, a0.z), ori), inv);
-                const f3 thi = mul3(sub3(mk3(a0.w, q1.x, q1.y), ori), inv);
-                tn0 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf0 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            {
-                const f3 tlo = mul3(sub3(mk3(q1.z, q1.w, q2.x), ori), inv);
-                const f3 thi = mul3(sub3(mk3(q2.y, q2.z, q2.w), ori), inv);
-                tn1 = tmax_f(tmin, max_el3(min3v(tlo, thi)));
-                tf1 = tmin_f(tmax, min_el3(max3v(tlo, thi)));
-            }
-            const bool h0 = tf0 >= tn0 && tn0 < r.t;
-            const bool h1 = tf1 >= tn1 && tn1 < r.t;
-            if (h0 && h1) {
-                const bool near1 = !(tn0 < tn1);
-                if (sp >= MAX_STACK) return r;  // cannot happen for trees that pass stack_fits
-                push(near1 ? q3.x : q3.y);
-                cur = near1 ? q3.y : q3.x;
-            } else if (h0) {
-                cur = q3.x;
-            } else if (h1) {
-                cur = q3.y;
-            } else {
-                if (sp == 0) return r;
-                cur = pop();
-            }
-        }
-        const uint32_t first = cur & LEAF_FIRST_MASK;
-        const uint32_t count = ((cur >> 26) & 31u) + 1u;
-        if (first + count > b.num_prims) return r;  // corrupt entry
-        for (uint32_t k = first; k < first + count; ++k) {
-            const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
-            const float4 ta = tp[0], tb = tp[1], tq = tp[2];
-            float t, u, v;
-            if (intersect_tri(ori, dir, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v)) {
-                if (t < r.t && t >= tmin && t <= tmax) {
-                    r.t = t;
-                    r.u = u;
-                    r.v = v;
-                    r.tri = (int32_t)k;
-                    if (ANY_HIT) return r;
-                }
-            }
-        }
-        if (sp == 0) return r;
-        cur = pop();
-    }
-}
-
-// LDS part of the traversal stack (entries per lane)
-constexpr int LDS_STACK = 16;
 
 // ---------------------------------------------------------------------------
 // shared host helpers
