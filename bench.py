@@ -30,6 +30,15 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "incoherent Mrays/s @4M tris; LBVH build ms; adjoint vs FD speedup"
 HBM_PEAK_GBS = 8000.0
+# MI355X vector ALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz, one 32-bit lane
+# operation per lane per cycle (a wave64 instruction issues over 4 cycles):
+# 39.3 T lane-operations/s, i.e. one wave64 VALU instruction per CU per cycle
+CUS, CLOCK_HZ = 256, 2.4e9
+VALU_PEAK_LANE_OPS = CUS * 4 * 16 * CLOCK_HZ
+# hardware-counter summary of the path-tracing kernel (rocprofv3 --pmc over
+# `bench.py --profile-pass`) and the sources whose hash it was collected with
+COUNTERS_JSON = ROOT / "profiles" / "pathtrace_c1_counters.json"
+KERNEL_SOURCES = ("visionaray_amd/csrc/render.hip", "include/visionaray_amd_core.h")
 # algorithmic bytes of one query: ray in, hit out, one 128-byte line per
 # traversal-node visit, 48 bytes per triangle test
 RAY_IN, HIT_OUT, NODE_BYTES, TRI_BYTES = 32, 16, 128, 48
@@ -37,6 +46,63 @@ RAY_IN, HIT_OUT, NODE_BYTES, TRI_BYTES = 32, 16, 128, 48
 
 def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_source_sha() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update((ROOT / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def roofline(kernel_ms: float, alg_bytes: float, per_ray: dict) -> dict:
+    """Roofline of the frame kernel.  Bound: VALU issue (the counters show the
+    vector ALU busy most of the kernel's time, with a fraction of its lanes
+    active, and HBM far below its peak).  achieved = active-lane VALU
+    operations per second = SQ_INSTS_VALU x 64 x lane utilisation / kernel
+    time, against 39.3 T lane-ops/s; traffic = HBM bytes per launch from
+    FETCH_SIZE / WRITE_SIZE scaled by what they read for kernels of known
+    traffic (a 128-byte line gather for reads, a copy for writes).  Counts come
+    from the committed counter summary, the time from this run's HIP events."""
+    out = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3), "unit": "Tlane-op/s",
+           "frac": None, "traffic": None, "kernel": "k_pathtrace", "kernel_ms": round(kernel_ms, 4)}
+    alg_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    hbm = {"algorithmic_bytes_per_launch": int(alg_bytes), "algorithmic_gbs": round(alg_gbs, 1),
+           "algorithmic_frac_of_peak": round(alg_gbs / HBM_PEAK_GBS, 4), "per_ray": per_ray,
+           "note": "algorithmic bytes count every 128-byte node and 48-byte triangle fetch; L2 / MALL serve "
+                   "most of them, so they exceed what HBM moves"}
+    try:
+        prof = json.loads(COUNTERS_JSON.read_text())
+    except (OSError, ValueError):
+        out["hbm"] = hbm
+        out["error"] = f"no counter summary at {COUNTERS_JSON.relative_to(ROOT)}"
+        return out
+    cur = kernel_source_sha()
+    stale = prof.get("kernel_source_sha") != cur
+    c = prof["counters_per_launch"]
+    t = kernel_ms * 1e-3
+    util = c["SQ_THREAD_CYCLES_VALU"] / (c["SQ_ACTIVE_INST_VALU"] * 64.0)
+    lane_ops = c["SQ_INSTS_VALU"] * 64.0 * util
+    issue = c["SQ_INSTS_VALU"] / (CUS * t * CLOCK_HZ)
+    traffic = prof["hbm_bytes_per_launch"]
+    out["achieved"] = round(lane_ops / t / 1e12, 3)
+    out["frac"] = round(lane_ops / t / VALU_PEAK_LANE_OPS, 4)
+    out["traffic"] = int(traffic)
+    out["valu"] = {"issue_frac": round(issue, 4), "lane_utilization": round(util, 4),
+                   "insts_valu_per_launch": c["SQ_INSTS_VALU"], "insts_salu_per_launch": c.get("SQ_INSTS_SALU"),
+                   "note": "issue_frac = wave64 VALU instructions per CU per cycle (peak 1); frac = issue_frac x "
+                           "lane_utilization"}
+    hbm.update({"measured_bytes_per_launch": int(traffic), "measured_gbs": round(traffic / t / 1e9, 1),
+                "measured_frac_of_peak": round(traffic / t / 1e9 / HBM_PEAK_GBS, 4),
+                "calibration": prof.get("calibration")})
+    out["hbm"] = hbm
+    out["profile"] = {"file": str(COUNTERS_JSON.relative_to(ROOT)), "kernel": prof.get("kernel"),
+                      "kernel_source_sha": prof.get("kernel_source_sha"), "current_source_sha": cur,
+                      "stale": stale}
+    if stale:
+        out["profile"]["note"] = "the kernel sources changed since the counters were collected"
+    return out
 
 
 def main():
@@ -69,6 +135,9 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--no-extras", action="store_true",
                     help="c1: skip the reference-scheme frame and the heightfield-only run")
+    ap.add_argument("--no-c2", action="store_true", help="c1: skip the batched-scenes measurement (C2)")
+    ap.add_argument("--no-c3", action="store_true", help="c1: skip the adjoint-vs-FD measurement (C3)")
+    ap.add_argument("--c2-steps", type=int, default=2, help="c1: timed frames of the C2 measurement")
     ap.add_argument("--trace-spp", type=int, default=4,
                     help="c1: bounce rays per pixel of the incoherent closest-hit batch (vr_trace); 0 skips it")
     args = ap.parse_args()
@@ -88,10 +157,9 @@ def main():
         if args.out:
             Path(args.out).write_text(json.dumps(line) + "\n")
         return
+    import bench_workloads as BW
     if args.config != "c1":
-        from visionaray_amd import workloads
-        line = {"c2": workloads.bench_batched, "c3": workloads.bench_adjoint, "c4": workloads.bench_dynamic}[
-            args.config](args, V, S, log)
+        line = {"c2": BW.bench_batched, "c3": BW.bench_adjoint, "c4": BW.bench_dynamic}[args.config](args, V, S, log)
         print(json.dumps(line), flush=True)
         if args.out:
             Path(args.out).write_text(json.dumps(line) + "\n")
@@ -179,14 +247,6 @@ def main():
     # ---- a batch of incoherent closest-hit queries (vr_trace) ----
     closest = trace_batch_bench(args, V, tris, bvh, cam, stream) if args.trace_spp > 0 else None
 
-    traffic, counters = None, None
-    tfile = ROOT / "profiles" / "pathtrace_c1_traffic.json"
-    if tfile.exists():
-        try:
-            prof = json.loads(tfile.read_text())
-            traffic, counters = prof.get("bytes_per_launch"), prof.get("counters")
-        except (OSError, ValueError):
-            traffic = None
 
     # ---- CPU baseline: the reference's own tiled_sched path ----
     # ---- the same frame on the reference's GPU scheme, and the @4M point ----
@@ -197,9 +257,26 @@ def main():
         extras["lbvh_rotated"] = rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms)
         extras["heightfield_only"] = heightfield_only(args, V, S, stream)
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream)
+        cpu, parity = cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream)
+
+    # ---- the metric's third part (C3) and the batched launch (C2) ----
+    adjoint = None
+    if not args.no_c3:
+        c3 = BW.bench_adjoint(args, V, S, log, quick=True)
+        adjoint = {k: c3[k] for k in ("value", "unit", "adjoint_ms", "forward_loss_ms", "fd_params_timed",
+                                      "fd_ms_timed", "fd_ms_all_params", "cosine_vs_plain_fd",
+                                      "cosine_at_config_spp")}
+        adjoint["workload"] = c3["config"]["workload"]
+        adjoint["params"] = c3["config"]["params"]
+    batched = None
+    if not args.no_c2:
+        c2 = BW.bench_batched(args, V, S, log, steps=args.c2_steps)
+        batched = {k: c2[k] for k in ("speedup", "speedup_vs_persistent_per_scene", "per_scene_plain_launches",
+                                      "per_scene_persistent_launches", "batched_graph", "rays_per_frame")}
+        batched["workload"] = c2["config"]["workload"]
+        batched["batch_shape"] = c2["config"]["batch_shape"]
 
     result = {
         "metric": METRIC,
@@ -225,26 +302,13 @@ def main():
         "lbvh_depth": info["depth"],
         "total_mrays_per_s": round(total_mrays, 3),
         "rays_per_frame": {"primary": int(prim_per), "secondary": int(sec_per)},
-        "adjoint_vs_fd_speedup": None,
+        "adjoint_vs_fd_speedup": adjoint,
+        "batched": batched,
+        "parity": parity,
         "closest_hit_batch": closest,
         **extras,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic,
-            "kernel": "k_pathtrace",
-            "kernel_ms": round(kernel_ms, 4),
-            "algorithmic_bytes_per_launch": int(alg_bytes),
-            "per_ray": {"nodes": round(st[2] / rays_frame, 3), "triangles": round(st[3] / rays_frame, 3)},
-            "hbm_traffic_gbs": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
-            "counters": counters,
-            "note": "algorithmic bytes count every node/triangle fetch as HBM traffic; L2/MALL serve most, so "
-                    "achieved can exceed the HBM peak. Measured HBM bytes are `traffic`; the kernel is bound by "
-                    "VALU issue of divergent wave64 code (counters.valu_busy_pct)",
-        },
+        "roofline": roofline(kernel_ms, alg_bytes, {"nodes": round(st[2] / rays_frame, 3),
+                                                    "triangles": round(st[3] / rays_frame, 3)}),
         "cpu_baseline": cpu,
     }
     line = json.dumps(result)
@@ -506,9 +570,11 @@ def instanced_batch(args, V, d_rays, n, flat_prim, flat_ms, stream, reps=3):
 
 
 def profile_pass(args, V, S, tris, d_tris, d_mats, stream):
-    """One LBVH build, one path-traced frame, one copy kernel of known size:
-    the dispatches a hardware-counter run reads (FETCH_SIZE / WRITE_SIZE of the
-    copy calibrate the counters against bytes moved)."""
+    """One LBVH build, one counted and one plain path-traced frame, a copy of
+    known size and a gather of known size (every 128-byte line of 4 GiB once,
+    random order): the dispatches a hardware-counter run reads.  FETCH_SIZE /
+    WRITE_SIZE of the copy and the gather calibrate the counters against bytes
+    moved, for streaming and for line-gather access."""
     bvh = V.lbvh_builder().build(d_tris, keep_nodes=False, stream=stream)
     eye, center, up, fovy, _ = S.config1_camera()
     cam = V.pinhole_camera(eye, center, up, fovy, args.width / args.height)
@@ -525,23 +591,47 @@ def profile_pass(args, V, S, tris, d_tris, d_mats, stream):
     dst = V.DeviceArray(nbytes // 4, np.float32)
     V.check(V.lib().vr_device_copy(dst.ptr, src.ptr, nbytes, stream.handle))
     stream.synchronize()
+    del src, dst
+    # gather calibration: every 128-byte line of a 4 GiB buffer exactly once,
+    # in random order (16x the L2 + MALL capacity, so no line is reused)
+    gbytes = 4 << 30
+    nlines = gbytes // 128
+    gsrc = V.DeviceArray(gbytes // 4, np.uint32)
+    V.check(V.lib().vr_memset(gsrc.ptr, 0xA5, gbytes))
+    lines = V.DeviceArray.from_host(np.random.default_rng(0).permutation(nlines).astype(np.uint32))
+    gout = V.DeviceArray(1, np.uint32, zero=True)
+    V.check(V.lib().vr_device_gather_lines(gsrc.ptr, gbytes, lines.ptr, nlines, gout.ptr, stream.handle))
+    stream.synchronize()
     st = stats.to_host().astype(np.float64)
     rays = st[0] + st[1]
     alg = rays * (RAY_IN + HIT_OUT) + st[2] * NODE_BYTES + st[3] * TRI_BYTES
-    print(json.dumps({"profile_pass": True, "copy_bytes": nbytes, "pathtrace_algorithmic_bytes": int(alg),
+    print(json.dumps({"profile_pass": True, "copy_bytes": nbytes, "gather_bytes": gbytes + 4 * nlines,
+                      "gather_lines": nlines, "pathtrace_algorithmic_bytes": int(alg),
                       "rays": int(rays), "nodes": int(st[2]), "triangles": int(st[3])}), flush=True)
 
 
 def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
-    """Time the reference's tiled_sched + pathtracing::kernel (compiled from
-    the reference headers into oracle/_ref) on a bounded sample: the same
-    scene, camera and depth at --cpu-spp samples per pixel.  Rays/s uses the
-    query count of the GPU run of the identical sample (same per-pixel seeds)."""
+    """The leg that runs the oracle/ code, as baseline and checker only.
+
+    Baseline: the reference's tiled_sched + pathtracing::kernel (compiled from
+    the reference headers into oracle/_ref) timed on a bounded sample, the same
+    scene, camera and depth at --cpu-spp samples per pixel; rays/s uses the
+    query count of the GPU run of the identical sample (same per-pixel seeds).
+
+    Parity: the GPU frame (16 spp, depth 8, frame 0) against the oracle's
+    restatement of the same frame on four bands of four rows, per pixel within
+    4 sigma (sigma from four more GPU frames), with the global relative RMSE.
+    Returns (cpu_baseline, parity)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     try:
         import oracle as O
     except Exception as e:  # pragma: no cover
-        return {"value": None, "error": f"oracle unavailable: {e}"}
+        return {"value": None, "error": f"oracle unavailable: {e}"}, None
+    parity = None
+    try:
+        parity = frame_parity(args, tris, mats, V, S, bvh, d_mats, kp, stream, O)
+    except Exception as e:  # the checker must not take the measurement down
+        parity = {"error": str(e)}
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     w, h = args.width, args.height
     eye, center, up, fovy, _ = S.config1_camera()
@@ -561,7 +651,7 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
                               threads=threads)
         log(f"cpu baseline: reference tiled_sched, {threads} threads, {w}x{h} at {args.cpu_spp} spp")
         t0 = time.time()
-        r = O.ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=3, timeout=300, stall=60.0)
+        r = O.ref_bench(tris, mats, cam_in, params, packet=4, build="lbvh", frames=5, timeout=300, stall=60.0)
         log(f"cpu baseline done in {time.time() - t0:.1f}s: frame {r['frame_s_median']:.3f}s, "
             f"build {r['build_s']:.2f}s")
         frame_s = r["frame_s_median"]
@@ -572,15 +662,83 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
             "kind": "reference",
             "sample": f"same scene and camera, {w}x{h} at {args.cpu_spp} spp, depth {int(kpv['num_bounces'])}; "
                       f"tiled_sched<basic_ray<simd::float4>> + pathtracing::kernel over the reference's CPU "
-                      f"lbvh_builder tree (leaf size 4); median of 3 frames after 1 warm-up; rays counted "
+                      f"lbvh_builder tree (leaf size 4); median of 5 frames after 1 warm-up; rays counted "
                       f"on the GPU run of the identical sample",
             "frame_s": round(frame_s, 4),
             "total_mrays_per_s": round((n_primary + n_secondary) / frame_s / 1e6, 4),
             "cpu_lbvh_build_s": round(r["build_s"], 3),
             "binary": O.ref_binary().name,
+            "host": host_info(),
             **({"stopped": r["stopped"], "timed_frames": r["frames"]} if "stopped" in r else {}),
-        }
-    return {"value": None, "error": "oracle/_ref not built"}
+        }, parity
+    return {"value": None, "error": "oracle/_ref not built"}, parity
+
+
+def host_info():
+    """CPU model, core count and compiler of the host running the baseline."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        import subprocess
+        out = subprocess.run(["g++", "--version"], capture_output=True, text=True, timeout=10).stdout
+        info["compiler"] = out.splitlines()[0] if out else None
+    except Exception:
+        info["compiler"] = None
+    return info
+
+
+PARITY_BANDS = [(300, 304), (540, 544), (760, 764), (1060, 1064)]
+
+
+def frame_parity(args, tris, mats, V, S, bvh, d_mats, kp, stream, O):
+    """GPU frame 0 against the oracle's frame 0 on PARITY_BANDS rows."""
+    t0 = time.time()
+    w, h = args.width, args.height
+    eye, center, up, fovy, _ = S.config1_camera()
+    cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+    rt = V.render_target(w, h)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=args.spp), cam, rt)
+    kern = V.pathtracing.kernel(bvh, d_mats, kp)
+    rows = [r for a, b in PARITY_BANDS if b <= h for r in range(a, b)]
+    sched.frame(kern, sp, stream=stream)
+    stream.synchronize()
+    got = rt.color_host()[rows]
+    frames = []
+    for _ in range(4):
+        sched.frame(kern, sp, stream=stream)
+        stream.synchronize()
+        frames.append(rt.color_host()[rows])
+    sigma = np.stack(frames).std(axis=0)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    kpv = kp[0]
+    p = O.render_params(w, h, spp=args.spp, num_bounces=int(kpv["num_bounces"]), epsilon=float(kpv["epsilon"]),
+                        bg=tuple(kpv["bg"]), ambient=tuple(kpv["ambient"]))
+    cam_o = cam.astype(O.CAMERA_DTYPE)
+    want = []
+    for a, b in PARITY_BANDS:
+        if b <= h:
+            img, _ = O.render_pt(nodes, tris, idx, mats, cam_o, p, rows=(a, b))
+            want.append(img[a:b])
+    want = np.concatenate(want)
+    exact = float(np.mean(np.all(got == want, axis=-1)))
+    diff = np.abs(got[..., :3].astype(np.float64) - want[..., :3])
+    within = float(np.mean(np.all(diff <= 4.0 * sigma[..., :3] + 1e-6, axis=-1)))
+    rel = float(np.sqrt(np.mean(diff ** 2)) / max(np.sqrt(np.mean(want[..., :3].astype(np.float64) ** 2)), 1e-300))
+    out = {"pixels": int(got.shape[0] * got.shape[1]), "rows": [list(b) for b in PARITY_BANDS],
+           "exact_pixel_fraction": round(exact, 5), "within_4_sigma": round(within, 5),
+           "global_rel_rmse": float(f"{rel:.4e}"),
+           "against": "the oracle's restatement of the reference's path tracer, frame 0, same seeds",
+           "check_s": round(time.time() - t0, 2)}
+    log(f"parity: {out['pixels']} pixels of frame 0: exact {exact:.4f}, within 4 sigma {within:.4f}, "
+        f"global rel RMSE {rel:.3e}")
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -1,5 +1,6 @@
-"""Measurement procedures for the batched, differentiable and dynamic
-configurations (bench.py --config c2 | c3 | c4).  Each returns one JSON-able
+"""Measurement procedures of bench.py for the batched, differentiable and
+dynamic configurations (C2, C3, C4: --config c2 | c3 | c4; the default run
+also reports C2 and C3 in the headline line).  Each returns one JSON-able
 dict; times come from HIP events on the stream the kernels run on."""
 from __future__ import annotations
 
@@ -16,7 +17,10 @@ def _events(V):
 # c2: 64 scenes, one persistent launch (graph replay) vs one launch per scene
 # ---------------------------------------------------------------------------
 
-def bench_batched(args, V, S, log):
+def bench_batched(args, V, S, log, steps=None):
+    """C2: 64 scenes of 1k-2M triangles in one persistent hipGraph-captured
+    launch, against per-scene launches in the reference's scheme and on the
+    persistent scheduler; `steps` timed frames (default --steps)."""
     count = args.scenes or 64
     V.init(0)
     st = V.Stream()
@@ -36,7 +40,7 @@ def bench_batched(args, V, S, log):
         total_tris += sc["tris"].size
     st.synchronize()
     log(f"c2: {count} scenes, {total_tris} triangles, prepared in {time.time() - t0:.1f}s")
-    steps = max(1, args.steps)
+    steps = max(1, args.steps if steps is None else steps)
 
     def per_scene(shape):
         """one frame() launch per scene under a launch shape; (ms, rays) per step"""
@@ -112,13 +116,20 @@ def bench_batched(args, V, S, log):
 # c3: adjoint gradient vs central finite differences
 # ---------------------------------------------------------------------------
 
-def bench_adjoint(args, V, S, log):
-    import sys
-    from pathlib import Path
+def bench_adjoint(args, V, S, log, quick=False):
+    """C3: 1.31 M triangles, 256 materials (792 parameters), 512 x 512, 64 spp,
+    depth 4, L2 loss to a perturbed-albedo target.  The speedup compares one
+    adjoint (forward + loss + backward, HIP events) with central differences of
+    all 792 parameters at h = 1e-3 (2 x 792 forward frames + losses), timed on
+    `fd_timed` parameters and extrapolated.  Agreement is checked against the
+    plain frame's central differences with common random numbers on a
+    parameter subset; see the returned keys for the spp of each check.
+    quick: the subset sizes of the default bench run."""
     V.init(0)
     st = V.Stream()
     width = height = 512
-    spp, depth = 64, 4
+    spp, depth, hstep = 64, 4, 1e-3
+    check_spp = 256
     tris, mats = S.config3_scene()
     rng = np.random.default_rng(11)
     target_m = mats.copy()
@@ -127,137 +138,118 @@ def bench_adjoint(args, V, S, log):
     eye, center, up, fovy, _ = S.config0_camera()
     cam = V.pinhole_camera(eye, center, up, fovy, 1.0)
     kp = V.make_kernel_params(num_bounces=depth, epsilon=1e-4)
-    rt = V.render_target(width, height)
-    sp = V.make_sched_params(V.jittered_blend_type(spp=spp), cam, rt)
-    d_mats = V.DeviceArray.from_host(mats)
-    kern = V.pathtracing.kernel(bvh, d_mats, kp)
-    # target image
-    sched = V.hip_sched()
-    sched.frame(V.pathtracing.kernel(bvh, V.DeviceArray.from_host(target_m), kp), sp, stream=st)
-    target = V.DeviceArray(width * height * 4, np.float32)
-    st.synchronize()
-    target.upload(rt.color.to_host())
-    dl = V.DeviceArray(width * height * 4, np.float32)
-    grad = V.DeviceArray(6 * mats.size, np.float64, zero=True)
-    loss = V.DeviceArray(1, np.float64, zero=True)
     lib = V.lib()
-
-    d_rr = V.DeviceArray.from_host(mats)  # the roulette table held at the unperturbed point
-
-    def forward_loss(kernel, frozen=False):
-        sched.frame_id = 0
-        if frozen:
-            sched.frame_frozen_rr(kernel, d_rr, sp, stream=st)
-        else:
-            sched.frame(kernel, sp, stream=st)
-        V.check(lib.vr_memset(loss.ptr, 0, 8))
-        V.check(lib.vr_image_l2_loss(rt.color.ptr, target.ptr, width * height, dl.ptr, loss.ptr, st.handle))
-
-    # adjoint: forward + loss + backward
-    for _ in range(2):
-        V.check(lib.vr_memset(grad.ptr, 0, grad.nbytes))
-        forward_loss(kern)
-        sched.backward(kern, sp, dl, grad, stream=st, frame_id=0)
-    st.synchronize()
-    V.check(lib.vr_memset(grad.ptr, 0, grad.nbytes))
-    e0, e1 = _events(V)
-    e0.record(st)
-    forward_loss(kern)
-    e_mid = V.Event()
-    e_mid.record(st)
-    sched.backward(kern, sp, dl, grad, stream=st, frame_id=0)
-    e1.record(st)
-    adjoint_ms = e0.elapsed_ms(e1)
-    forward_ms = e0.elapsed_ms(e_mid)
-    L0 = float(loss.to_host()[0])
-    g = grad.to_host().reshape(-1, 6)
-    ga = np.concatenate([g[:, 0:3].ravel(), g[mats["type"] == 1, 3:6].ravel()])
-    log(f"c3: loss {L0:.6f}; adjoint (forward + loss + backward) {adjoint_ms:.2f} ms, forward {forward_ms:.2f} ms")
-
-    # central differences with common random numbers, h = 1e-3.  The check
-    # differentiates the frozen-roulette estimator (vr_render_pt_frozen_rr:
-    # roulette probabilities from the throughput at the unperturbed albedos,
-    # so every roulette decision is common to both sides); its expectation is
-    # the image's for any albedos, and its derivative is what the adjoint
-    # computes.  A subset is also differenced with the plain frame, whose
-    # roulette decisions flip under the perturbation (noise ~ 1/sqrt(h)).
-    hstep = 1e-3
     params = [(m, "cd", c) for m in range(mats.size) for c in range(3)] + \
              [(m, "ce", c) for m in np.nonzero(mats["type"] == 1)[0] for c in range(3)]
-    nfd = len(params) if not args.fd_params else min(args.fd_params, len(params))
+    d_mats = V.DeviceArray.from_host(mats)
     d_pert = V.DeviceArray(mats.size, mats.dtype)
+    d_rr = V.DeviceArray.from_host(mats)  # roulette table held at the unperturbed point
+    kern = V.pathtracing.kernel(bvh, d_mats, kp)
     kpert = V.pathtracing.kernel(bvh, d_pert, kp)
 
-    def central(i, frozen):
-        m, key, c = params[i]
-        vals = []
-        for sgn in (1.0, -1.0):
-            mm = mats.copy()
-            mm[key][m, c] += np.float32(sgn * hstep)
-            d_pert.upload(mm)
-            forward_loss(kpert, frozen)
-            vals.append(float(loss.to_host()[0]))
-        return (vals[0] - vals[1]) / (2 * hstep)
+    class Frame:
+        """render target, target image and loss buffers at one spp"""
 
-    fd = np.zeros(len(params))
-    t0 = time.perf_counter()
-    e0, e1 = _events(V)
-    e0.record(st)
-    for i in range(nfd):
-        fd[i] = central(i, True)
-        if i % 100 == 0:
-            log(f"c3: finite differences {i}/{nfd}")
-    e1.record(st)
-    fd_ms = e0.elapsed_ms(e1)
-    fd_wall = time.perf_counter() - t0
-    fd_ms_all = fd_ms * len(params) / nfd
+        def __init__(self, n_spp):
+            self.rt = V.render_target(width, height)
+            self.sp = V.make_sched_params(V.jittered_blend_type(spp=n_spp), cam, self.rt)
+            self.sched = V.hip_sched()
+            self.sched.frame(V.pathtracing.kernel(bvh, V.DeviceArray.from_host(target_m), kp), self.sp, stream=st)
+            st.synchronize()
+            self.target = V.DeviceArray.from_host(self.rt.color.to_host())
+            self.dl = V.DeviceArray(width * height * 4, np.float32)
+            self.loss = V.DeviceArray(1, np.float64, zero=True)
+            self.grad = V.DeviceArray(6 * mats.size, np.float64, zero=True)
+
+        def forward_loss(self, kernel, frozen=False):
+            self.sched.frame_id = 0
+            if frozen:
+                self.sched.frame_frozen_rr(kernel, d_rr, self.sp, stream=st)
+            else:
+                self.sched.frame(kernel, self.sp, stream=st)
+            V.check(lib.vr_memset(self.loss.ptr, 0, 8))
+            V.check(lib.vr_image_l2_loss(self.rt.color.ptr, self.target.ptr, width * height, self.dl.ptr,
+                                         self.loss.ptr, st.handle))
+
+        def adjoint(self):
+            V.check(lib.vr_memset(self.grad.ptr, 0, self.grad.nbytes))
+            self.forward_loss(kern)
+            self.sched.backward(kern, self.sp, self.dl, self.grad, stream=st, frame_id=0)
+
+        def gradient(self):
+            self.adjoint()
+            st.synchronize()
+            g = self.grad.to_host().reshape(-1, 6)
+            return np.concatenate([g[:, 0:3].ravel(), g[mats["type"] == 1, 3:6].ravel()])
+
+        def central(self, i, frozen=False):
+            m, key, c = params[i]
+            vals = []
+            for sgn in (1.0, -1.0):
+                mm = mats.copy()
+                mm[key][m, c] += np.float32(sgn * hstep)
+                d_pert.upload(mm)
+                self.forward_loss(kpert, frozen)
+                st.synchronize()
+                vals.append(float(self.loss.to_host()[0]))
+            return (vals[0] - vals[1]) / (2 * hstep)
 
     def cosine(x, y):
         return float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y) + 1e-300))
 
-    cos = cosine(ga[:nfd], fd[:nfd])
+    F = Frame(spp)
+    # ---- timed: the adjoint, then central differences (enqueued back to back)
+    for _ in range(2):
+        F.adjoint()
+    st.synchronize()
+    e0, e_mid, e1 = V.Event(), V.Event(), V.Event()
+    e0.record(st)
+    V.check(lib.vr_memset(F.grad.ptr, 0, F.grad.nbytes))
+    F.forward_loss(kern)
+    e_mid.record(st)
+    F.sched.backward(kern, F.sp, F.dl, F.grad, stream=st, frame_id=0)
+    e1.record(st)
+    adjoint_ms = e0.elapsed_ms(e1)
+    forward_ms = e0.elapsed_ms(e_mid)
+    L0 = float(F.loss.to_host()[0])
+    ga = F.gradient()
+    nfd = len(params) if not args.fd_params else min(args.fd_params, len(params))
+    if quick and not args.fd_params:
+        nfd = 24
+    timed = np.linspace(0, len(params) - 1, nfd).astype(int)
+    uploads = []
+    for i in timed:  # the perturbed tables, uploaded between frames as the FD loop does
+        for sgn in (1.0, -1.0):
+            m, key, c = params[i]
+            mm = mats.copy()
+            mm[key][m, c] += np.float32(sgn * hstep)
+            uploads.append(mm)
+    e0, e1 = V.Event(), V.Event()
+    e0.record(st)
+    for mm in uploads:
+        d_pert.upload(mm)
+        F.forward_loss(kpert)
+    e1.record(st)
+    st.synchronize()
+    fd_ms = e0.elapsed_ms(e1)
+    fd_ms_all = fd_ms * len(params) / nfd
     speedup = fd_ms_all / adjoint_ms
-    log(f"c3: FD {nfd} params in {fd_ms:.0f} ms (wall {fd_wall:.1f}s) -> all {fd_ms_all:.0f} ms; "
-        f"speedup x{speedup:.1f}, cosine {cos:.6f} (frozen-roulette FD)")
-    sub = np.arange(0, nfd, max(1, nfd // 96))
-    fd_plain = np.array([central(int(i), False) for i in sub])
+    log(f"c3: adjoint {adjoint_ms:.2f} ms (forward + loss {forward_ms:.2f} ms); FD of {nfd} params "
+        f"{fd_ms:.0f} ms -> all {len(params)}: {fd_ms_all:.0f} ms; speedup x{speedup:.1f}")
+
+    # ---- agreement: every 16th albedo parameter and every emission parameter
+    sub = np.array(list(range(0, 768, 16 if quick else 4)) + list(range(768, len(params))))
+    fd_plain = np.array([F.central(int(i)) for i in sub])
+    fd_frozen = np.array([F.central(int(i), frozen=True) for i in sub])
     cos_plain = cosine(ga[sub], fd_plain)
-    cos_sub = cosine(ga[sub], fd[sub])
-    log(f"c3: on {sub.size} params: cosine {cos_plain:.4f} vs FD of the plain frame (roulette decisions flip), "
-        f"{cos_sub:.6f} vs frozen-roulette FD")
-
-    # FP64 CPU adjoint on a band of rows (the image gradient masked to it)
-    fp64 = None
-    try:
-        sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
-        import oracle as O
-        rows = (240, 248)
-        img = rt.color.to_host().reshape(height, width, 4)
-        forward_loss(kern)
-        st.synchronize()
-        img = rt.color.to_host().reshape(height, width, 4)
-        dlh = dl.to_host().reshape(height, width, 4).copy()
-        mask = np.zeros_like(dlh)
-        mask[rows[0]:rows[1]] = dlh[rows[0]:rows[1]]
-        d_mask = V.DeviceArray.from_host(mask.reshape(-1))
-        gm = V.DeviceArray(6 * mats.size, np.float64, zero=True)
-        sched.backward(kern, sp, d_mask, gm, stream=st, frame_id=0)
-        st.synchronize()
-        gpu_band = gm.to_host().reshape(-1, 6)
-        nodes_ref = None
-        t1 = time.time()
-        nodes_ref, idx, _, _ = O.lbvh_build(tris)
-        p = O.render_params(width, height, spp=spp, num_bounces=depth, epsilon=1e-4)
-        gcd, gce = O.render_pt_adjoint(nodes_ref, tris, idx, mats, cam.astype(O.CAMERA_DTYPE), p, mask,
-                                       rows=rows)
-        want = np.concatenate([gcd, gce], axis=1)
-        rel = float(np.linalg.norm(gpu_band - want) / max(np.linalg.norm(want), 1e-300))
-        cos64 = float((gpu_band.ravel() @ want.ravel()) / (np.linalg.norm(gpu_band) * np.linalg.norm(want) + 1e-300))
-        fp64 = {"rows": list(rows), "rel_err": rel, "cosine": cos64, "cpu_s": round(time.time() - t1, 2)}
-        log(f"c3: FP64 CPU adjoint on rows {rows}: rel err {rel:.2e}, cosine {cos64:.6f}")
-    except Exception as e:  # the oracle is optional for this leg
-        fp64 = {"error": str(e)}
-
+    cos_frozen = cosine(ga[sub], fd_frozen)
+    log(f"c3: {sub.size} params at {spp} spp: cosine {cos_plain:.5f} vs plain CRN FD, {cos_frozen:.6f} vs "
+        "frozen-roulette FD")
+    G = Frame(check_spp)
+    ga_check = G.gradient()
+    fd_check = np.array([G.central(int(i)) for i in sub])
+    cos_check = cosine(ga_check[sub], fd_check)
+    log(f"c3: {sub.size} params at {check_spp} spp: cosine {cos_check:.5f} vs plain CRN FD")
     return {
         "metric": "adjoint vs FD speedup",
         "value": round(speedup, 2), "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 2,
@@ -266,15 +258,24 @@ def bench_adjoint(args, V, S, log):
         "config": {"workload": f"C3 differentiable: room + level-8 icosphere, {tris.size} triangles, "
                                f"{mats.size} materials ({int((mats['type'] == 1).sum())} emissive), "
                                f"{width}x{height}, {spp} spp, depth {depth}, L2 loss to perturbed-albedo target",
-                   "params": len(params), "fd_params_timed": nfd, "h": hstep},
-        "adjoint_ms": round(adjoint_ms, 4), "forward_ms": round(forward_ms, 4),
-        "fd_ms_all_params": round(fd_ms_all, 2), "cosine_vs_fd": round(cos, 6),
-        "fd_estimator": "central, h=1e-3, common random numbers and common roulette decisions "
-                        "(vr_render_pt_frozen_rr at the unperturbed albedos)",
-        "plain_frame_fd_subset": {"params": int(sub.size), "cosine": round(cos_plain, 6),
-                                  "cosine_frozen_same_subset": round(cos_sub, 6)},
+                   "params": len(params), "h": hstep},
+        "adjoint_ms": round(adjoint_ms, 4), "forward_loss_ms": round(forward_ms, 4),
+        "fd_params_timed": int(nfd), "fd_ms_timed": round(fd_ms, 2), "fd_ms_all_params": round(fd_ms_all, 2),
+        "cosine_vs_plain_fd": {
+            "value": round(cos_check, 6), "spp": check_spp, "params": int(sub.size), "h": hstep,
+            "fd": "central differences of the plain frame's L2 loss, common random numbers (frame 0 seeds)",
+            "adjoint": f"the same loss at {check_spp} spp, frame 0",
+        },
+        "cosine_at_config_spp": {
+            "spp": spp, "params": int(sub.size),
+            "vs_plain_fd": round(cos_plain, 6),
+            "vs_frozen_roulette_fd": round(cos_frozen, 6),
+            "note": "the L2 loss holds each pixel's variance, which Russian roulette makes depend on the albedos "
+                    "through the roulette probabilities; the adjoint holds those probabilities constant (it is "
+                    "the exact derivative of the frozen-roulette estimator), the plain differences include that "
+                    "term, which falls as 1/spp",
+        },
         "loss": L0,
-        "fp64_cpu_adjoint": fp64,
     }
 
 

@@ -162,6 +162,14 @@ int vr_event_elapsed_ms(void* start, void* stop, float* ms);
  * multiple of 16): graph-capturable, and a kernel of known HBM traffic for
  * calibrating hardware counters. */
 int vr_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* Reads the 128-byte lines d_lines[0..n) of d_src (src_bytes long, a
+ * multiple of 128; the caller keeps every index below src_bytes / 128), one
+ * line per thread in eight 16-byte loads -- the access pattern of a traversal
+ * lane reading a 4-wide node -- and xors what it read into *d_out.  With
+ * distinct lines its HBM traffic is known (n * 128 + n * 4 bytes): it
+ * calibrates hardware traffic counters for gathers.  No reference counterpart. */
+int vr_device_gather_lines(const void* d_src, size_t src_bytes, const uint32_t* d_lines, size_t n, uint32_t* d_out,
+                           void* stream);
 
 /* ---- camera ------------------------------------------------------------- */
 /* pinhole_camera::perspective + look_at + begin_frame

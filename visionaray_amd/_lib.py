@@ -45,6 +45,7 @@ SIGNATURES = {
     "vr_event_record": (I, [P, P]),
     "vr_event_elapsed_ms": (I, [P, P, C.POINTER(F)]),
     "vr_device_copy": (I, [P, P, SZ, P]),
+    "vr_device_gather_lines": (I, [P, SZ, P, SZ, P, P]),
     "vr_camera_look_at": (I, [P, P, P, F, F, P]),
     "vr_lbvh_build": (I, [P, SZ, U32, P, C.POINTER(P)]),
     "vr_lbvh_rebuild": (I, [P, P, SZ, P]),

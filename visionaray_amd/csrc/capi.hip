@@ -35,6 +35,8 @@ int sah_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh
                    uint32_t* indices);
 int lbvh_build_host(const vr_triangle* prims, size_t n, int max_leaf_size, vr_bvh_node* nodes, size_t* num_nodes,
                     uint32_t* indices);
+int device_gather_lines(const void* src, size_t src_bytes, const uint32_t* lines, size_t n, uint32_t* out,
+                        hipStream_t st);
 int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 int render_pt_adjoint(const bvh_state& bs, const vr_material* mats, size_t num_mats, const vr_kernel_params& kp,
                       const vr_camera& cam, const vr_sched_params& sp, const float* dl, double* grad,
@@ -298,6 +300,13 @@ int vr_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream)
 {
     VR_CHECK(bytes == 0 || (d_dst && d_src), VR_ERR_INVALID_ARGUMENT, "vr_device_copy: null pointer");
     return device_copy(d_dst, d_src, bytes, (hipStream_t)stream);
+}
+
+int vr_device_gather_lines(const void* d_src, size_t src_bytes, const uint32_t* d_lines, size_t n, uint32_t* d_out,
+                           void* stream)
+{
+    VR_CHECK(n == 0 || (d_src && d_lines && d_out), VR_ERR_INVALID_ARGUMENT, "vr_device_gather_lines: null pointer");
+    return device_gather_lines(d_src, src_bytes, d_lines, n, d_out, (hipStream_t)stream);
 }
 
 // pinhole_camera::look_at + perspective + begin_frame

@@ -785,6 +785,34 @@ __global__ __launch_bounds__(BLOCK) void k_copy16(uint4* __restrict__ dst, const
     for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) dst[i] = src[i];
 }
 
+// Counter calibration for gathers: thread i reads the 128-byte line
+// lines[i] of src with eight 16-byte loads, as a traversal lane reads one
+// 4-wide node, and the xor of everything read goes to *out (one atomic per
+// block), so a permutation of distinct lines moves a known number of bytes.
+__global__ __launch_bounds__(BLOCK) void k_gather_lines(const uint4* __restrict__ src,
+                                                        const uint32_t* __restrict__ lines, size_t n,
+                                                        uint32_t* __restrict__ out)
+{
+    uint32_t acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const uint4* p = src + (size_t)lines[i] * 8u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint4 v = p[k];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc ^= __shfl_xor(acc, off);
+    __shared__ uint32_t part[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int q = 0; q < BLOCK / 64; ++q) t ^= part[q];
+        atomicXor(out, t);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1022,6 +1050,17 @@ int device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
     const unsigned grid = std::max(1u, std::min(div_up(n, BLOCK), 65536u));
     hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(BLOCK), 0, st, static_cast<uint4*>(dst),
                        static_cast<const uint4*>(src), n);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+int device_gather_lines(const void* src, size_t src_bytes, const uint32_t* lines, size_t n, uint32_t* out,
+                        hipStream_t st)
+{
+    VR_CHECK(src_bytes % 128 == 0, VR_ERR_INVALID_ARGUMENT, "vr_device_gather_lines: src_bytes must be whole lines");
+    if (n == 0) return VR_OK;
+    const unsigned grid = std::max(1u, std::min(div_up(n, BLOCK), 65536u));
+    hipLaunchKernelGGL(k_gather_lines, dim3(grid), dim3(BLOCK), 0, st, static_cast<const uint4*>(src), lines, n, out);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
