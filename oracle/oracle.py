@@ -240,16 +240,22 @@ def render_ao(nodes, tris, indices, cam, params, rows=None) -> np.ndarray:
     return col
 
 
-def render_pt(nodes, tris, indices, mats, cam, params, rows=None, lights=None):
+def render_pt(nodes, tris, indices, mats, cam, params, rows=None, lights=None, accum=None):
     """-> (color image, [primary rays, secondary rays]); `lights` (LIGHT_DTYPE)
-    turns on next-event estimation (shadow rays are counted as secondary)."""
+    turns on next-event estimation (shadow rays are counted as secondary);
+    `accum` ((h, w, 4) float32, updated in place) is the render target's
+    accumulation buffer the frame blends into (zeros when None)."""
     nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
     tris = np.ascontiguousarray(tris, TRI_DTYPE)
     indices = np.ascontiguousarray(indices, np.uint32)
     mats = np.ascontiguousarray(mats, MAT_DTYPE)
     cam = np.ascontiguousarray(cam, CAMERA_DTYPE)
     w, h = int(params["width"][0]), int(params["height"][0])
-    acc = np.zeros((h, w, 4), np.float32)
+    if accum is None:
+        acc = np.zeros((h, w, 4), np.float32)
+    else:
+        acc = accum
+        assert acc.dtype == np.float32 and acc.shape == (h, w, 4) and acc.flags["C_CONTIGUOUS"]
     col = np.zeros((h, w, 4), np.float32)
     counts = np.zeros(2, np.uint64)
     y0, y1 = rows if rows else (0, h)

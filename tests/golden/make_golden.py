@@ -90,8 +90,25 @@ def main():
     np.savez_compressed(OUT / "images.npz", cam_in=cam_in, ao_params=ao_p, ao=ao, mats=mats, tris_m=tris_m,
                         pt_params=pt_p, pt=pt)
     nee()
+    progressive()
     quality()
     instances()
+
+
+def progressive():
+    """Progressive accumulation (the viewer's per-frame blend): three frames
+    of pathtracing::kernel on one render target under the jittered-blend
+    sampler with sfactor = dfactor = 0.5, frames 0, 1, 2 seeded by the
+    scheduler's frame counter (sched_common.h:427-526, pixel_access.h:1040-1131)."""
+    g = np.load(OUT / "images.npz")
+    tris_m, mats, cam_in = g["tris_m"], g["mats"], g["cam_in"]
+    n_m, i_m, _, _ = O.lbvh_build(tris_m)
+    w = h = 48
+    p = O.ref_params(w, h, spp=2, frame_id=2, num_bounces=6, epsilon=1e-4, bg=(0.05, 0.05, 0.05),
+                     ambient=(0.6, 0.7, 0.9), sfactor=0.5, dfactor=0.5, threads=4)
+    img = O.ref_pt(n_m, tris_m, i_m, mats, cam_in, p)
+    np.savez_compressed(OUT / "progressive.npz", params=p, frames=np.int32(3), sfactor=np.float32(0.5),
+                        dfactor=np.float32(0.5), pt=img)
 
 
 def quality():

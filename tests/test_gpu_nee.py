@@ -73,9 +73,9 @@ def test_nee_matches_reference_image(V):
     img, (npri, nsec), _ = nee_frame(V, bvh, g["mats"], g["lights"], kp, cam, w, h, spp)
     assert npri == w * h * spp
     frames = np.stack([nee_frame(V, bvh, g["mats"], g["lights"], kp, cam, w, h, spp, frame_id=f)[0]
-                       for f in range(1, 17)])
+                       for f in range(1, 33)])
     exact, within, rel = stats_compare(img, g["pt"], frames.std(axis=0), "nee golden")
-    assert exact >= 0.3 and within >= 0.99
+    assert exact >= 0.3 and within >= 0.998
     assert mean_agrees(img, g["pt"], frames, "nee golden")
 
 
@@ -94,9 +94,13 @@ def test_nee_matches_oracle(V, O, spp):
     assert npri == counts[0] == w * h * spp
     assert abs(nsec - int(counts[1])) <= 0.01 * counts[1]
     assert st[2] > 0 and st[3] > 0
-    frames = np.stack([nee_frame(V, bvh, mats, lights, kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 9)])
+    frames = np.stack([nee_frame(V, bvh, mats, lights, kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 33)])
     exact, within, rel = stats_compare(img, want, frames.std(axis=0), f"nee oracle spp={spp}")
-    assert exact >= 0.3 and within >= 0.99
+    # at 1 spp a pixel's value is heavy-tailed (mostly unlit, now and then a
+    # light sample many sigma above the mean), so a shadow ray that a one-ulp
+    # difference in sin/cos flips moves it by many sigma: 4 sigma holds for
+    # fewer of those pixels than it would for a Gaussian estimate
+    assert exact >= 0.3 and within >= (0.995 if spp == 1 else 0.998)
     assert mean_agrees(img, want, frames, f"nee oracle spp={spp}")
 
 

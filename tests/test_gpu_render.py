@@ -76,10 +76,45 @@ def test_pt_matches_reference_image(V, O):
                               ambient_color=(*p[9:12], 1.0))
     img, (npri, nsec), _ = gpu_frame(V, bvh, g["mats"], kp, cam, w, h, spp)
     assert npri == w * h * spp
-    frames = np.stack([gpu_frame(V, bvh, g["mats"], kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 17)])
+    frames = np.stack([gpu_frame(V, bvh, g["mats"], kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 33)])
     sigma = frames.std(axis=0)
     exact, within, rel = stats_compare(img, g["pt"], sigma, "pt golden")
-    assert exact >= 0.8 and within >= 0.995 and rel < 0.05
+    assert exact >= 0.8 and within >= 0.998 and rel < 0.05
+
+
+def test_progressive_accumulation(V, O):
+    """Three frames on one render target with sfactor = dfactor = 0.5
+    (jittered_blend_type; pixel_access::blend, pixel_access.h:1040-1131):
+    equal bit for bit to the blend recomputed from the GPU's own single
+    frames (accum_f = c_f * sf + accum_{f-1} * df, float32), and against the
+    reference's progressive frames (tests/golden/progressive.npz) bit-identical
+    where the device's sin/cos agree, within 4 sigma elsewhere."""
+    g = load_golden("images")
+    pg = load_golden("progressive")
+    p = pg["params"]
+    w, h, spp = int(p[0]), int(p[1]), int(p[2])
+    sf, df = np.float32(pg["sfactor"]), np.float32(pg["dfactor"])
+    bvh = V.lbvh_builder().build(g["tris_m"])
+    cam = golden_camera(O, g["cam_in"])
+    kp = V.make_kernel_params(num_bounces=int(p[4]), epsilon=float(p[5]), bg_color=(*p[6:9], 1.0),
+                              ambient_color=(*p[9:12], 1.0))
+    kern = V.pathtracing.kernel(bvh, V.DeviceArray.from_host(g["mats"]), kp)
+    rt = V.render_target(w, h)
+    sched = V.hip_sched()
+    sp = V.make_sched_params(V.jittered_blend_type(spp=spp, sfactor=float(sf), dfactor=float(df)), cam, rt)
+    for _ in range(int(pg["frames"])):
+        sched.frame(kern, sp)
+    got = rt.color_host()
+    singles = [gpu_frame(V, bvh, g["mats"], kp, cam, w, h, spp, frame_id=f)[0] for f in range(int(pg["frames"]))]
+    acc = np.zeros_like(singles[0])
+    for c in singles:
+        acc = (c * sf).astype(np.float32) + (acc * df).astype(np.float32)
+    assert np.array_equal(got, acc)
+    frames = np.stack([gpu_frame(V, bvh, g["mats"], kp, cam, w, h, spp, frame_id=f)[0] for f in range(3, 35)])
+    weights = np.array([sf * df * df, sf * df, sf], np.float64)
+    sigma = frames.std(axis=0) * np.sqrt(np.sum(weights ** 2))
+    exact, within, rel = stats_compare(got, pg["pt"], sigma, "progressive golden")
+    assert exact >= 0.8 and within >= 0.998 and rel < 0.05
 
 
 @pytest.mark.parametrize("spp", [1, 8])
@@ -103,9 +138,9 @@ def test_pt_matches_oracle_c0(V, O, spp):
     assert npri == counts[0] == w * h * spp
     assert abs(nsec - int(counts[1])) <= 0.01 * counts[1]
     assert st[0] == npri and st[1] == nsec and st[2] > 0 and st[3] > 0
-    frames = np.stack([gpu_frame(V, bvh, mats, kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 9)])
+    frames = np.stack([gpu_frame(V, bvh, mats, kp, cam, w, h, spp, frame_id=f)[0] for f in range(1, 33)])
     exact, within, rel = stats_compare(img, want, frames.std(axis=0), f"pt oracle spp={spp}")
-    assert exact >= 0.8 and within >= 0.99 and rel < 0.05
+    assert exact >= 0.8 and within >= 0.998 and rel < 0.05
 
 
 def _quad(z, half, geom_id=0):

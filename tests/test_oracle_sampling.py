@@ -90,6 +90,48 @@ def test_pt_image_matches_reference(O):
     assert counts[0] == int(p[0]) * int(p[1]) * int(p[2])
 
 
+def progressive_oracle(O, frames_sf_df=None):
+    """The oracle's three progressive frames on one accumulation buffer;
+    returns (final color, the three single frames at sfactor 1, dfactor 0)."""
+    g = load_golden("images")
+    pg = load_golden("progressive")
+    p = pg["params"]
+    tris = g["tris_m"]
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    cam = camera_from_golden(O, g["cam_in"])
+    w, h = int(p[0]), int(p[1])
+    sf, df = frames_sf_df if frames_sf_df else (float(p[12]), float(p[13]))
+    acc = np.zeros((h, w, 4), np.float32)
+    singles = []
+    img = None
+    for f in range(int(pg["frames"])):
+        kw = dict(spp=int(p[2]), num_bounces=int(p[4]), epsilon=p[5], bg=p[6:9], ambient=p[9:12])
+        params = O.render_params(w, h, frame_id=f, sfactor=sf, dfactor=df, **kw)
+        img, _ = O.render_pt(nodes, tris, idx, g["mats"], cam, params, accum=acc)
+        singles.append(O.render_pt(nodes, tris, idx, g["mats"], cam, O.render_params(w, h, frame_id=f, **kw))[0])
+    return img, singles
+
+
+def test_progressive_accumulation_matches_reference(O):
+    """Three frames blended into one accumulation buffer with sfactor =
+    dfactor = 0.5 (jittered_blend_type, sched_common.h:427-526;
+    pixel_access::blend, pixel_access.h:1040-1131): bit-exact against the
+    reference's tiled_sched frames 0, 1, 2, and equal to the blend recomputed
+    from the single frames, accum_f = c_f * sf + accum_{f-1} * df (float32,
+    each operation rounded)."""
+    pg = load_golden("progressive")
+    img, singles = progressive_oracle(O)
+    assert np.array_equal(img, pg["pt"])
+    acc = np.zeros_like(singles[0])
+    sf, df = np.float32(pg["sfactor"]), np.float32(pg["dfactor"])
+    for c in singles:
+        acc = (c * sf).astype(np.float32) + (acc * df).astype(np.float32)
+    assert np.array_equal(acc, pg["pt"])
+    # a dropped accumulation term (dfactor 0) is caught
+    last_only, _ = progressive_oracle(O, (0.5, 0.0))
+    assert not np.array_equal(last_only, pg["pt"])
+
+
 def camera_from_golden(O, cam_in):
     """The camera basis the reference computed for these inputs."""
     cams = load_golden("camera")

@@ -6,11 +6,14 @@ fallback for any entry point.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libvisionaray_amd.so"
+# VR_LIBRARY selects another build of the same library (e.g. one with the host
+# code under AddressSanitizer); default: the in-tree build
+LIB_PATH = Path(os.environ.get("VR_LIBRARY") or Path(__file__).resolve().parent / "libvisionaray_amd.so")
 
 VR_OK = 0
 VR_ERR_INVALID_ARGUMENT = 1
