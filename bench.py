@@ -655,6 +655,7 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
         log(f"cpu baseline done in {time.time() - t0:.1f}s: frame {r['frame_s_median']:.3f}s, "
             f"build {r['build_s']:.2f}s")
         frame_s = r["frame_s_median"]
+        extra = cpu_baseline_variants(args, tris, mats, V, S, bvh, d_mats, kp, stream, O, cam_in, threads)
         return {
             "value": round(n_secondary / frame_s / 1e6, 4),
             "unit": "Mrays/s",
@@ -669,9 +670,58 @@ def cpu_baseline(args, tris, mats, V, S, bvh, d_mats, kp, stream):
             "cpu_lbvh_build_s": round(r["build_s"], 3),
             "binary": O.ref_binary().name,
             "host": host_info(),
+            **extra,
             **({"stopped": r["stopped"], "timed_frames": r["frames"]} if "stopped" in r else {}),
         }, parity
     return {"value": None, "error": "oracle/_ref not built"}, parity
+
+
+def cpu_baseline_variants(args, tris, mats, V, S, bvh, d_mats, kp, stream, O, cam_in, threads):
+    """The reference's CPU path with scalar rays (tiled_sched<basic_ray<float>>)
+    on the same scene at 1 spp, and its binned_sah_builder on the config-0
+    scene (single-threaded, as written)."""
+    out = {}
+    w, h = args.width, args.height
+    kpv = kp[0]
+    try:
+        eye, center, up, fovy, _ = S.config1_camera()
+        cam = V.pinhole_camera(eye, center, up, fovy, w / h)
+        rt = V.render_target(w, h)
+        sched = V.hip_sched()
+        sched.frame(V.pathtracing.kernel(bvh, d_mats, kp), V.make_sched_params(V.jittered_blend_type(spp=1), cam, rt),
+                    stream=stream)
+        stream.synchronize()
+        _, sec = sched.rays()
+        params = O.ref_params(w, h, spp=1, num_bounces=int(kpv["num_bounces"]), epsilon=float(kpv["epsilon"]),
+                              bg=tuple(kpv["bg"]), ambient=tuple(kpv["ambient"]), threads=threads)
+        r = O.ref_bench(tris, mats, cam_in, params, packet=1, build="lbvh", frames=5, timeout=300, stall=60.0)
+        out["scalar_rays"] = {"value": round(sec / r["frame_s_median"] / 1e6, 4), "unit": "Mrays/s",
+                              "sample": f"same scene and camera, {w}x{h} at 1 spp; tiled_sched<basic_ray<float>>; "
+                                        "median of 5 frames after 1 warm-up",
+                              "frame_s": round(r["frame_s_median"], 4)}
+        log(f"cpu baseline, scalar rays: frame {r['frame_s_median']:.3f}s at 1 spp "
+            f"({out['scalar_rays']['value']:.2f} incoherent Mrays/s)")
+    except Exception as e:
+        out["scalar_rays"] = {"error": str(e)}
+    try:
+        c0 = S.config0_scene(level=5)
+        c0m = S.lambert_materials(1, seed=3)
+        c0["geom_id"] = 0
+        cin = O.camera_input(*S.config0_camera(64, 64))
+        p0 = O.ref_params(64, 64, spp=1, num_bounces=2, epsilon=1e-4, threads=threads)
+        r0 = O.ref_bench(c0, c0m, cin, p0, packet=1, build="sah", frames=1, timeout=120, stall=60.0)
+        t0 = time.perf_counter()
+        V.sah_build(c0, 4)
+        ours = time.perf_counter() - t0
+        out["c0_sah_build"] = {"reference_s": round(r0["build_s"], 4), "this_build_host_s": round(ours, 4),
+                               "triangles": int(c0.size),
+                               "note": "binned_sah_builder, 16 bins, no spatial splits, leaf size 4; the "
+                                       "reference's single-threaded build against vr_sah_build (same tree)"}
+        log(f"cpu baseline, C0 SAH build of {c0.size} triangles: reference {r0['build_s'] * 1e3:.1f} ms, "
+            f"this build's host SAH {ours * 1e3:.1f} ms")
+    except Exception as e:
+        out["c0_sah_build"] = {"error": str(e)}
+    return out
 
 
 def host_info():

@@ -227,8 +227,17 @@ struct trav_state {
     int sp;           // deferred entries on the stack
     uint32_t budget;  // visits left: a ray visits each node at most once
     uint32_t lf, ln;  // trav_iter: triangles [lf, lf + ln) of a leaf still to test
+    // byte offsets in a quad_node of the near slab planes per axis: lo (0,
+    // 16, 32) where inv >= 0, hi (48, 64, 80) where inv < 0; the far planes
+    // are at near ^ 48, ^ 80, ^ 112
+    uint32_t nx, ny, nz;
     hit_t r;
 };
+
+// quad_node byte offsets: lo_x 0, lo_y 16, lo_z 32, hi_x 48, hi_y 64, hi_z 80
+constexpr uint32_t QN_FAR_X = 48u, QN_FAR_Y = 80u, QN_FAR_Z = 112u;
+// quads addressable with 32-bit byte offsets (node << 7)
+constexpr uint32_t QN_NEAR_FAR_MAX = 1u << 25;
 
 __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax)
 {
@@ -238,6 +247,9 @@ __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 
                 dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
     T.tmin = tmin;
     T.tmax = tmax;
+    T.nx = T.inv.x < 0.0f ? 48u : 0u;
+    T.ny = T.inv.y < 0.0f ? 64u : 16u;
+    T.nz = T.inv.z < 0.0f ? 80u : 32u;
     T.cur = b.root;
     T.sp = 0;
     T.budget = 2u * b.num_prims + 2u;
@@ -267,7 +279,7 @@ constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
 #define VR_TLAS_REFILL_LANES 16
 #endif
 #ifndef VR_TRAV_NODE_BIAS
-#define VR_TRAV_NODE_BIAS 2.0f
+#define VR_TRAV_NODE_BIAS 2u
 #endif
 
 // One iteration of the query, either one node visit or one triangle test,
@@ -284,10 +296,26 @@ constexpr uint32_t TRAV_DONE = 0xFFFFFFFFu;
 // Moller-Trumbore on the triangle's three 16-byte records.  Any type with
 // these two members replaces them (the intersectors of the device API).
 struct default_isect {
+    static constexpr bool near_far = true;  // box_nf below is the same test as box
     __device__ __forceinline__ float box(float lx, float ly, float lz, float hx, float hy, float hz,
                                          const trav_state& T, float tclosest) const
     {
         return slab_enter(lx, ly, lz, hx, hy, hz, T.ori, T.inv, T.tmin, T.tmax, tclosest);
+    }
+    // The same slab test given each axis' near and far plane (the planes the
+    // ray's direction sign selects): with inv >= 0, (lo - o) * inv <=
+    // (hi - o) * inv (subtraction and multiplication round monotonically), so
+    // min and max of the two distances are the near and far ones exactly;
+    // with inv < 0 the other way round.
+    __device__ __forceinline__ float box_nf(float nx, float ny, float nz, float fx, float fy, float fz,
+                                            const trav_state& T, float tclosest) const
+    {
+        const float x0 = (nx - T.ori.x) * T.inv.x, x1 = (fx - T.ori.x) * T.inv.x;
+        const float y0 = (ny - T.ori.y) * T.inv.y, y1 = (fy - T.ori.y) * T.inv.y;
+        const float z0 = (nz - T.ori.z) * T.inv.z, z1 = (fz - T.ori.z) * T.inv.z;
+        const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, T.tmin));
+        const float tf = fminf(fminf(x1, y1), fminf(z1, T.tmax));
+        return (tf >= tn && tn < tclosest) ? tn : __builtin_inff();
     }
     __device__ __forceinline__ bool tri(const trav_state& T, float4 a, float4 b, float4 c, float& t, float& u,
                                         float& v) const
@@ -320,7 +348,17 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     auto pop = [&]() -> uint32_t {
         if (T.sp == 0) return TRAV_DONE;
         --T.sp;
-        return T.sp < LDS_DEPTH ? stack[T.sp * STRIDE] : spill[T.sp - LDS_DEPTH];
+        // read the LDS column unconditionally (clamped) and the private
+        // overflow only when needed, so the two reads never merge into one
+        // generic-address load
+        uint32_t e = stack[(T.sp < LDS_DEPTH ? T.sp : LDS_DEPTH - 1) * STRIDE];
+        if (T.sp >= LDS_DEPTH) {
+            e = spill[T.sp - LDS_DEPTH];
+            // an empty asm on the value (no instruction): keeps the compiler
+            // from sinking both reads into one generic-address load
+            asm volatile("" : "+v"(e));
+        }
+        return e;
     };
     // a leaf at cur goes to the free triangle range
     if (T.ln == 0 && (T.cur & LEAF_BIT) && T.cur != TRAV_DONE) {
@@ -335,22 +373,52 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     if (ORDER == TRAV_VOTED) {
         const uint32_t n_node = (uint32_t)__popcll(__ballot(node_work));
         const uint32_t n_tri = (uint32_t)__popcll(__ballot(T.ln > 0));
-        do_node = (float)n_node >= VR_TRAV_NODE_BIAS * (float)n_tri;
+        do_node = n_node >= VR_TRAV_NODE_BIAS * n_tri;
     } else {
         do_node = T.ln == 0;
     }
     if (do_node) {
         if (!node_work) return false;
         if (--T.budget == 0 || T.cur >= b.num_quads) return true;
-        const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
-        const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
-        const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
         if (COUNT) cnt->nodes++;
         const float tc = T.r.t;
-        float k0 = is.box(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T, tc);
-        float k1 = is.box(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, T, tc);
-        float k2 = is.box(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, T, tc);
-        float k3 = is.box(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, T, tc);
+        float k0, k1, k2, k3;
+        uint4 ch;
+        bool near_far = false;
+        if constexpr (ISECT::near_far) near_far = b.num_quads <= QN_NEAR_FAR_MAX;
+        if (near_far) {
+            // per-lane byte offsets from the node's base: with a wave-uniform
+            // BVH base the loads take the base in scalar registers and one
+            // 32-bit add each
+            const char* qb = reinterpret_cast<const char*>(b.quads);
+            const uint32_t vo = T.cur << 7;
+            const uint32_t ax = vo | T.nx, ay = vo | T.ny, az = vo | T.nz;
+            const float4 nx = *reinterpret_cast<const float4*>(qb + ax);
+            const float4 ny = *reinterpret_cast<const float4*>(qb + ay);
+            const float4 nz = *reinterpret_cast<const float4*>(qb + az);
+            const float4 fx = *reinterpret_cast<const float4*>(qb + (ax ^ QN_FAR_X));
+            const float4 fy = *reinterpret_cast<const float4*>(qb + (ay ^ QN_FAR_Y));
+            const float4 fz = *reinterpret_cast<const float4*>(qb + (az ^ QN_FAR_Z));
+            ch = *reinterpret_cast<const uint4*>(qb + vo + 96u);
+            // used here: the child entries are loaded with the planes (one
+            // memory round trip per visit), not sunk below the box tests
+            asm volatile("" : "+v"(ch.x), "+v"(ch.y), "+v"(ch.z), "+v"(ch.w));
+            if constexpr (ISECT::near_far) {
+                k0 = is.box_nf(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, T, tc);
+                k1 = is.box_nf(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, T, tc);
+                k2 = is.box_nf(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, T, tc);
+                k3 = is.box_nf(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, T, tc);
+            }
+        } else {
+            const float4* p = reinterpret_cast<const float4*>(b.quads + T.cur);
+            const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+            ch = *reinterpret_cast<const uint4*>(p + 6);
+            asm volatile("" : "+v"(ch.x), "+v"(ch.y), "+v"(ch.z), "+v"(ch.w));
+            k0 = is.box(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, T, tc);
+            k1 = is.box(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, T, tc);
+            k2 = is.box(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, T, tc);
+            k3 = is.box(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, T, tc);
+        }
         uint32_t e0 = ch.x, e1 = ch.y, e2 = ch.z, e3 = ch.w;
         cswap(k0, e0, k1, e1);
         cswap(k2, e2, k3, e3);
@@ -381,7 +449,11 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     ++T.lf;
     --T.ln;
     const float4* tp = reinterpret_cast<const float4*>(b.tris + k);
-    const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+    float4 ta = tp[0];
+    const float4 tb = tp[1], tq = tp[2];
+    // v1 is used only after the determinant test: pinned here so that the
+    // three records arrive in one memory round trip
+    asm volatile("" : "+v"(ta.x), "+v"(ta.y), "+v"(ta.z));
     if (COUNT) cnt->tris++;
     float t, u, v;
     if (is.tri(T, ta, tb, tq, t, u, v)) {

@@ -182,6 +182,7 @@ namespace detail {
 // default intersector exactly the library's own slab decision
 template <typename I>
 struct isect_adapter {
+    static constexpr bool near_far = false;  // the user sees whole boxes
     I* user;
     __device__ float box(float lx, float ly, float lz, float hx, float hy, float hz, const vr::trav_state& T,
                          float tclosest) const

@@ -60,9 +60,10 @@ def test_ao_matches_reference_image(V, O):
     want = g["ao"]
     exact = np.mean(np.all(img == want, axis=-1))
     print(f"[ao] exact pixels {exact:.4f}")
-    assert exact >= 0.97
-    # a differing pixel differs by whole AO samples (1/8 per sample and spp)
-    assert np.abs(img - want).max() <= 0.25 + 1e-6
+    # the AO kernel traverses in an order that depends on the ray and the tree
+    # alone (the device API's), and resolves this image's exact ties as the
+    # reference does: bit-exact
+    assert np.array_equal(img, want)
 
 
 def test_pt_matches_reference_image(V, O):
