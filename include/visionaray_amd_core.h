@@ -227,10 +227,10 @@ struct trav_state {
     int sp;           // deferred entries on the stack
     uint32_t budget;  // visits left: a ray visits each node at most once
     uint32_t lf, ln;  // trav_iter: triangles [lf, lf + ln) of a leaf still to test
-    // byte offsets in a quad_node of the near slab planes per axis: lo (0,
-    // 16, 32) where inv >= 0, hi (48, 64, 80) where inv < 0; the far planes
-    // are at near ^ 48, ^ 80, ^ 112
-    uint32_t nx, ny, nz;
+    // byte offsets in a quad_node of the near slab planes per axis, one byte
+    // each (x, y, z from the low byte): lo (0, 16, 32) where inv >= 0, hi
+    // (48, 64, 80) where inv < 0; the far planes are at near ^ 48, ^ 80, ^ 112
+    uint32_t near;
     hit_t r;
 };
 
@@ -247,9 +247,7 @@ __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 
                 dir.z != 0.0f ? 1.0f / dir.z : FLT_MAX);
     T.tmin = tmin;
     T.tmax = tmax;
-    T.nx = T.inv.x < 0.0f ? 48u : 0u;
-    T.ny = T.inv.y < 0.0f ? 64u : 16u;
-    T.nz = T.inv.z < 0.0f ? 80u : 32u;
+    T.near = (T.inv.x < 0.0f ? 48u : 0u) | ((T.inv.y < 0.0f ? 64u : 16u) << 8) | ((T.inv.z < 0.0f ? 80u : 32u) << 16);
     T.cur = b.root;
     T.sp = 0;
     T.budget = 2u * b.num_prims + 2u;
@@ -392,7 +390,7 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
             // 32-bit add each
             const char* qb = reinterpret_cast<const char*>(b.quads);
             const uint32_t vo = T.cur << 7;
-            const uint32_t ax = vo | T.nx, ay = vo | T.ny, az = vo | T.nz;
+            const uint32_t ax = vo | (T.near & 0xFFu), ay = vo | ((T.near >> 8) & 0xFFu), az = vo | (T.near >> 16);
             const float4 nx = *reinterpret_cast<const float4*>(qb + ax);
             const float4 ny = *reinterpret_cast<const float4*>(qb + ay);
             const float4 nz = *reinterpret_cast<const float4*>(qb + az);

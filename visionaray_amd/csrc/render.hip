@@ -384,16 +384,24 @@ __device__ __forceinline__ bool shade(const scene_desc& S, const pt_args& A,
     return P.bounce >= S.kp.num_bounces;
 }
 
-// A path ended: add its sample (the background on a primary miss); returns
-// true when the pixel's last sample is done.
+// The pixel's sum of samples: every sample adds alpha 1 (exact in float up
+// to 2^24 samples), so the alpha sum is the sample count.
+__device__ __forceinline__ float4 sample_sum(const scene_desc& S, float4 sum)
+{
+    sum.w = (float)S.sp.spp;
+    return sum;
+}
+
+// A path ended: add its rgb sample (the background on a primary miss);
+// returns true when the pixel's last sample is done.
 template <bool ADJ>
 __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 {
     if (!ADJ) {
         if (P.rhit) {
-            P.sum.x += P.I.x; P.sum.y += P.I.y; P.sum.z += P.I.z; P.sum.w += 1.0f;
+            P.sum.x += P.I.x; P.sum.y += P.I.y; P.sum.z += P.I.z;
         } else {
-            P.sum.x += S.kp.bg[0]; P.sum.y += S.kp.bg[1]; P.sum.z += S.kp.bg[2]; P.sum.w += 1.0f;
+            P.sum.x += S.kp.bg[0]; P.sum.y += S.kp.bg[1]; P.sum.z += S.kp.bg[2];
         }
     }
     ++P.s;
@@ -462,6 +470,8 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
     P.sum = make_float4(0, 0, 0, 0);
     P.wgt = mk3(0, 0, 0);
     start_sample(P);
+    // queries started by the wave (wave-uniform: counted with ballots where
+    // queries start, so no lane holds a counter)
     unsigned long long n_primary = 0, n_secondary = 0;
     trace_counters tc{0, 0};
     unsigned long long n_nodes = 0, n_tris = 0;
@@ -473,6 +483,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
     bvh_view bv{};  // BATCH: the BVH of the lane's scene, kept in registers
 
     for (;;) {
+        const bool was_tracing = tracing;
         // 1. shade the finished queries; each such lane starts its next query
         //    or finishes its sample or pixel
         if (pending) {
@@ -506,7 +517,6 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                         Q.cd = nd;
                         trav_begin(T, S.b, so, sd, S.kp.epsilon, smax);
                         tracing = true;
-                        ++n_secondary;
                         shadow_now = true;
                     }
                 }
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
             if (!shadow_now) {
                 if (ended) {
                     if (end_sample(S, P)) {
-                        if (!ADJ) write_pixel(S, (uint32_t)px, P.sum);
+                        if (!ADJ) write_pixel(S, (uint32_t)px, sample_sum(S, P.sum));
                         px = -1;
                     } else {
                         const uint32_t w = S.sp.width, h = S.sp.height;
@@ -526,8 +536,6 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                 if (px >= 0) {
                     trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
                     tracing = true;
-                    if (P.bounce == 0) ++n_primary;
-                    else ++n_secondary;
                 }
             }
         }
@@ -576,7 +584,6 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                         }
                         trav_begin(T, S.b, o, d, 0.0f, FLT_MAX);
                         tracing = true;
-                        ++n_primary;
                     }
                 } else {
                     done = true;
@@ -584,6 +591,15 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
             }
         }
         if (__ballot(!done) == 0) break;
+        if (A.ray_counts) {
+            // every lane that holds a query now and did not before this round
+            // started one: primary if its path is at bounce 0 (a shadow ray
+            // is secondary)
+            const uint64_t started = __ballot(tracing && !was_tracing);
+            const uint64_t prim = __ballot(tracing && !was_tracing && P.bounce == 0 && !(NEE && Q.shadow));
+            n_primary += (unsigned long long)__popcll(prim);
+            n_secondary += (unsigned long long)__popcll(started & ~prim);
+        }
 
         // 3. advance the queries in flight until VR_PT_REGEN_LANES lanes wait
         //    to be shaded (or none is in flight): a lane whose query ends
@@ -599,11 +615,10 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         }
     }
     if (A.ray_counts) {
-        // wave reduction, one atomic pair per wave
-        for (int off = 32; off >= 1; off >>= 1) {
-            n_primary += __shfl_xor(n_primary, off);
-            n_secondary += __shfl_xor(n_secondary, off);
-            if (COUNT) {
+        // wave reduction, one atomic pair per wave (the query counts are
+        // already the wave's)
+        if (COUNT) {
+            for (int off = 32; off >= 1; off >>= 1) {
                 n_nodes += __shfl_xor(n_nodes, off);
                 n_tris += __shfl_xor(n_tris, off);
             }
@@ -668,7 +683,7 @@ __global__ __launch_bounds__(BLK) void k_pathtrace_plain(pt_args A)
             }
             end_sample(S, P);
         }
-        write_pixel(S, px, P.sum);
+        write_pixel(S, px, sample_sum(S, P.sum));
     }
     if (A.ray_counts) {
         for (int off = 32; off >= 1; off >>= 1) {
