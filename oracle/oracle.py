@@ -71,6 +71,9 @@ def lib():
         l.vro_query_log.argtypes = [P, SZ]
         l.vro_query_log_count.restype = SZ
         l.vro_render_pt_adjoint.argtypes = [P, P, P, SZ, P, P, P, P, P, P, C.c_int, C.c_int]
+        l.vro_collapse_wide.restype = SZ
+        l.vro_collapse_wide.argtypes = [P, SZ, C.c_int, P, P, P]
+        l.vro_compress_wide.argtypes = [P, P, SZ, C.c_int, P]
         _lib = l
     return _lib
 
@@ -149,6 +152,36 @@ def trace(nodes, tris, indices, rays, any_hit=False) -> np.ndarray:
     hits = np.zeros(rays.size, HIT_DTYPE)
     lib().vro_trace(_p(nodes), _p(tris), _p(indices), tris.size, _p(rays), rays.size, int(any_hit), _p(hits))
     return hits
+
+
+def collapse_wide(nodes, width: int):
+    """bvh_collapser into bvh_multi_node<width> -> (bounds (num, 6, width)
+    float32: minx miny minz maxx maxy maxz per slot, children (num, width)
+    int64, deepest collapse stack)"""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    n = max(nodes.size, 1)
+    b = np.zeros((n, 6, width), np.float32)
+    c = np.zeros((n, width), np.int64)
+    ms = np.zeros(1, np.uint64)
+    num = lib().vro_collapse_wide(_p(nodes), nodes.size, width, _p(b), _p(c), _p(ms))
+    return b[:num].copy(), c[:num].copy(), int(ms[0])
+
+
+def compress_wide(bounds, children, width: int) -> np.ndarray:
+    """bvh_compressed_node<width>::init of each node, reference bytes
+    (num, 64 | 112) uint8"""
+    bounds = np.ascontiguousarray(bounds, np.float32)
+    children = np.ascontiguousarray(children, np.int64)
+    stride = 64 if width == 4 else 112
+    out = np.zeros((max(len(children), 1), stride), np.uint8)
+    lib().vro_compress_wide(_p(bounds), _p(children), len(children), width, _p(out))
+    return out[: len(children)]
+
+
+def wide_leaf(child: int):
+    """decode_leaf of an encoded leaf child -> (first, count)"""
+    a = ~int(child) & 0xFFFFFFFFFFFFFFFF
+    return a & 0xFFFFFFFFFFFF, a >> 48
 
 
 def trace_brute(tris, rays, any_hit=False) -> np.ndarray:
@@ -505,6 +538,42 @@ def ref_rotate(nodes, tris, indices):
               t.put("tris.bin", tris), tris.size, t.put("idx.bin", np.asarray(indices, np.uint32)),
               t.path("out.bin"), t.path("count.bin")])
         return np.fromfile(t.path("out.bin"), NODE_DTYPE), int(np.fromfile(t.path("count.bin"), np.int32)[0])
+    finally:
+        t.close()
+
+
+def ref_wide(nodes, tris, indices, width: int):
+    """The reference's bvh_collapser (detail/bvh/collapse.h:19-169) into
+    index_bvh<width> and bvh_compressed_node<width>::init (bvh.h:352-400) of
+    every wide node -> (bounds, children, compressed bytes) as collapse_wide
+    and compress_wide give them."""
+    t = _Tmp()
+    try:
+        _run(["wide", t.put("nodes.bin", np.ascontiguousarray(nodes, NODE_DTYPE)), len(nodes),
+              t.put("tris.bin", tris), tris.size, t.put("idx.bin", np.asarray(indices, np.uint32)), width,
+              t.path("wide.bin"), t.path("count.bin"), t.path("comp.bin")])
+        num = int(np.fromfile(t.path("count.bin"), np.uint64)[0])
+        raw = np.fromfile(t.path("wide.bin"), np.uint8).reshape(num, 32 * width)
+        bounds = raw[:, : 24 * width].copy().view(np.float32).reshape(num, 6, width)
+        children = raw[:, 24 * width:].copy().view(np.int64).reshape(num, width)
+        comp = np.fromfile(t.path("comp.bin"), np.uint8).reshape(num, 64 if width == 4 else 112)
+        return bounds, children, comp
+    finally:
+        t.close()
+
+
+def ref_trace_wide(nodes, tris, indices, width: int, rays, compressed=False, binary=None):
+    """Closest hits of the reference's wide traversal over its collapsed tree:
+    intersect_ray1_bvhN (detail/bvh/intersect_ray1_bvhN.inl:166-410) or, for
+    compressed nodes, intersect_ray1_bvhN_compressed.  width 8 runs the AVX
+    build of the driver (the reference's 8-lane SIMD types)."""
+    t = _Tmp()
+    try:
+        _run(["tracew", t.put("nodes.bin", np.ascontiguousarray(nodes, NODE_DTYPE)), len(nodes),
+              t.put("tris.bin", tris), tris.size, t.put("idx.bin", np.asarray(indices, np.uint32)), width,
+              int(compressed), t.put("rays.bin", rays), rays.size, t.path("out.bin")],
+             binary=binary or (REF_BIN_V3 if width == 8 else REF_BIN))
+        return np.fromfile(t.path("out.bin"), HIT_DTYPE)
     finally:
         t.close()
 

@@ -27,6 +27,13 @@
 //                                       level (triangles BLASOF[b]..BLASOF[b+1]), index_bvh_inst_t per
 //                                       instance ({blas, inst_id, mat4x3 as 12 floats}), lbvh_builder over
 //                                       the instances, closest_hit -> {prim_id, t, u, v, inst_id, geom_id}
+//   wide    NODES NN TRIS NP IDX W OUTNODES OUTCOUNT OUTCOMP
+//                                       bvh_collapser into index_bvh4/8 (W), the wide nodes
+//                                       as stored and bvh_compressed_node<W>::init of each
+//   tracew  NODES NN TRIS NP IDX W COMP RAYS NR OUT
+//                                       intersect_ray1_bvhN (COMP 0) or
+//                                       intersect_ray1_bvhN_compressed (COMP 1) closest hits
+//                                       over the collapsed tree
 //   bench   ...                         CPU baseline timing (see below)
 
 #include <algorithm>
@@ -54,6 +61,14 @@
 #include <visionaray/scheduler.h>
 #include <visionaray/traverse.h>
 #include <visionaray/get_normal.h>
+#include <visionaray/detail/bvh/collapse.h>
+
+// the reference's 8-wide traversal needs its 8-lane SIMD types (AVX)
+#if defined(__AVX__)
+#define WIDE8_TRAVERSAL 1
+#else
+#define WIDE8_TRAVERSAL 0
+#endif
 #include <visionaray/detail/bvh/optimize.h>
 #include <visionaray/detail/bvh/statistics.h>
 #include <visionaray/detail/thread_pool.h>
@@ -384,6 +399,55 @@ int main(int argc, char** argv)
             else out[i] = to_out(closest_hit(rays[i], bvhs.data(), bvhs.data() + 1));
         }
         save(a[8], out.data(), out.size());
+        return 0;
+    }
+    if ((mode == "wide" && na == 9) || (mode == "tracew" && na == 10)) {
+        auto tree = load_tree(a);
+        const int W = std::atoi(a[5]);
+        auto run = [&](auto wide, auto comp) {
+            using WT = decltype(wide);
+            using CT = decltype(comp);
+            thread_pool pool(1);
+            bvh_collapser().collapse(tree, wide, pool);
+            comp.nodes().resize(wide.nodes().size());
+            for (size_t i = 0; i < wide.nodes().size(); ++i) comp.nodes()[i].init(wide.nodes()[i]);
+            comp.primitives() = wide.primitives();
+            comp.indices() = wide.indices();
+            if (mode == "wide") {
+                save(a[6], wide.nodes().data(), wide.nodes().size());
+                uint64_t count = wide.nodes().size();
+                save(a[7], &count, 1);
+                save(a[8], comp.nodes().data(), comp.nodes().size());
+                return;
+            }
+            const int use_comp = std::atoi(a[6]);
+            size_t nr = std::strtoull(a[8], nullptr, 10);
+            auto rays = load<basic_ray<float>>(a[7], nr);
+            std::vector<hit_out> out(nr);
+            auto wref = wide.ref();
+            auto cref = comp.ref();
+            if constexpr (WT::Width == 4) {
+                for (size_t i = 0; i < nr; ++i)
+                    out[i] = use_comp ? to_out(intersect_ray1_bvhN_compressed(rays[i], cref))
+                                      : to_out(intersect_ray1_bvhN(rays[i], wref));
+            } else if constexpr (WIDE8_TRAVERSAL) {
+                // the reference's compressed 8-wide traversal has no x86
+                // build (its sign_extend for 8 bytes exists for NEON only)
+                if (use_comp) {
+                    std::fprintf(stderr, "tracew: no compressed 8-wide traversal on this ISA\n");
+                    std::exit(3);
+                }
+                for (size_t i = 0; i < nr; ++i) out[i] = to_out(intersect_ray1_bvhN(rays[i], wref));
+            } else {
+                std::fprintf(stderr, "tracew: 8-wide traversal needs the AVX build\n");
+                std::exit(3);
+            }
+            save(a[9], out.data(), out.size());
+            (void)sizeof(CT);
+        };
+        if (W == 4) run(index_bvh4<tri_t>{}, compressed_index_bvh4<tri_t>{});
+        else if (W == 8) run(index_bvh8<tri_t>{}, compressed_index_bvh8<tri_t>{});
+        else return 2;
         return 0;
     }
     if (mode == "brute" && na == 6) {

@@ -1028,3 +1028,203 @@ void vro_render_pt(const vro_node* nodes, const vro_tri* tris, const uint32_t* i
     vro_render_pt_lights(nodes, tris, indices, num_prims, mats, NULL, 0, cam, p, accum, color, y_begin, y_end,
                          ray_counts);
 }
+
+/* ---- wide trees: bvh_multi_node<W>, the collapse, bvh_compressed_node<W> ---- */
+
+/* bvh_multi_node::encode_leaf (bvh.h:242-245) */
+static int64_t wide_encode_leaf(uint64_t first, uint64_t num)
+{
+    return (int64_t)~(num << 48 | (first & 0xFFFFFFFFFFFFull));
+}
+
+typedef struct {
+    float b[6][8]; /* minx, miny, minz, maxx, maxy, maxz of each child slot */
+    int64_t c[8];  /* children; INT64_MAX = unused */
+} wide_tmp;
+
+/* bvh_multi_node::get_num_children (bvh.h:272-283) */
+static int wide_num_children(const wide_tmp* n, int W)
+{
+    for (int i = 0; i < W; ++i)
+        if (n->c[i] == INT64_MAX) return i;
+    return W;
+}
+
+static void wide_set_box(wide_tmp* n, int slot, const float lo[3], const float hi[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        n->b[k][slot] = lo[k];
+        n->b[3 + k][slot] = hi[k];
+    }
+}
+
+/* bvh_multi_node::init (bvh.h:183-240): the multi-node of binary node id */
+static void wide_init(wide_tmp* m, size_t id, const vro_node* nodes, int W)
+{
+    const vro_node* n = &nodes[id];
+    for (int i = 0; i < W; ++i) {
+        m->c[i] = INT64_MAX;
+        for (int k = 0; k < 3; ++k) {
+            m->b[k][i] = FLT_MAX;
+            m->b[3 + k][i] = -FLT_MAX;
+        }
+    }
+    if (n->num_prims == 0) {
+        for (int i = 0; i < 2; ++i) {
+            const vro_node* c = &nodes[n->first + i];
+            m->c[i] = (int64_t)(n->first + i);
+            wide_set_box(m, i, c->lo, c->hi);
+            if (c->num_prims != 0) m->c[i] = wide_encode_leaf(c->first, c->num_prims);
+        }
+    } else if (id == 0) {
+        /* a root that is a leaf (bvh.h:221-234) */
+        wide_set_box(m, 0, n->lo, n->hi);
+        m->c[0] = wide_encode_leaf(n->first, n->num_prims);
+    }
+}
+
+/* surface_area (math/detail/aabb.inl:201-224): 2 (sx sy + sy sz + sz sx) */
+static float wide_child_area(const wide_tmp* n, int i)
+{
+    const float sx = n->b[3][i] - n->b[0][i], sy = n->b[4][i] - n->b[1][i], sz = n->b[5][i] - n->b[2][i];
+    return 2.0f * (sx * sy + sy * sz + sz * sx);
+}
+
+/* bvh_multi_node::collapse_child (bvh.h:256-269) */
+static void wide_collapse_child(wide_tmp* n, wide_tmp* child, int dest, int src)
+{
+    n->c[dest] = child->c[src];
+    for (int k = 0; k < 6; ++k) n->b[k][dest] = child->b[k][src];
+    if (src != 0) child->c[src] = INT64_MAX;
+}
+
+size_t vro_collapse_wide(const vro_node* nodes, size_t num_nodes, int W, float* out_bounds, int64_t* out_children,
+                         size_t* max_stack)
+{
+    if (num_nodes == 0 || (W != 4 && W != 8)) return 0;
+    wide_tmp* m = (wide_tmp*)malloc(num_nodes * sizeof(wide_tmp));
+    for (size_t i = 0; i < num_nodes; ++i) wide_init(&m[i], i, nodes, W);
+    /* depth-first from the root (collapse.h:37-118); the reference's stack
+     * holds 64 entries, this one as many as the tree needs (max_stack
+     * reports the deepest it got) */
+    size_t cap = 64, sp = 0, deepest = 0;
+    size_t* st = (size_t*)malloc(cap * sizeof(size_t));
+    st[sp++] = 0;
+    deepest = 1;
+    while (sp > 0) {
+        const size_t addr = st[--sp];
+        wide_tmp* node = &m[addr];
+        while (wide_num_children(node, W) < W) {
+            int best = -1;
+            float best_sa = 0.0f;
+            const int nc = wide_num_children(node, W);
+            for (int i = 0; i < nc; ++i) {
+                if (node->c[i] < 0) continue;
+                const wide_tmp* child = &m[node->c[i]];
+                const int cc = wide_num_children(child, W);
+                int inner = 0;
+                for (int c = 0; c < cc; ++c)
+                    if (child->c[c] > 0) ++inner;
+                /* leaves are never collapsed into the root */
+                if (inner == 0 && addr == 0) continue;
+                if (nc - 1 + cc <= W) {
+                    const float sa = wide_child_area(node, i);
+                    if (sa > best_sa) {
+                        best = i;
+                        best_sa = sa;
+                    }
+                }
+            }
+            if (best == -1) break;
+            wide_tmp* bc = &m[node->c[best]];
+            wide_collapse_child(node, bc, best, 0);
+            int dest = wide_num_children(node, W);
+            for (int i = 1; i < wide_num_children(bc, W); ++i) wide_collapse_child(node, bc, dest++, i);
+        }
+        const int nc = wide_num_children(node, W);
+        for (int i = 0; i < nc; ++i) {
+            if (node->c[i] > 0) {
+                if (sp == cap) {
+                    cap *= 2;
+                    st = (size_t*)realloc(st, cap * sizeof(size_t));
+                }
+                st[sp++] = (size_t)node->c[i];
+                if (sp > deepest) deepest = sp;
+            }
+        }
+    }
+    free(st);
+    if (max_stack) *max_stack = deepest;
+    /* drop empty nodes, renumber the inner children (collapse.h:120-150) */
+    int64_t* prefix = (int64_t*)malloc(num_nodes * sizeof(int64_t));
+    prefix[0] = 0;
+    for (size_t i = 1; i < num_nodes; ++i) prefix[i] = prefix[i - 1] + (m[i].c[0] == INT64_MAX ? 0 : 1);
+    size_t out = 0;
+    for (size_t i = 0; i < num_nodes; ++i) {
+        if (m[i].c[0] == INT64_MAX) continue;
+        const int nc = wide_num_children(&m[i], W);
+        for (int c = 0; c < W; ++c) {
+            int64_t e = m[i].c[c];
+            if (c < nc && e >= 0) e = prefix[e];
+            out_children[out * W + c] = e;
+            for (int k = 0; k < 6; ++k) out_bounds[(out * 6 + k) * W + c] = m[i].b[k][c];
+        }
+        ++out;
+    }
+    free(prefix);
+    free(m);
+    return out;
+}
+
+/* float -> int as x86's truncating conversion gives it (cvttss2si): values
+ * outside int32 and NaN give INT32_MIN; the reference stores such results
+ * into char / unsigned char, whose low byte this keeps */
+static int32_t cvt_trunc_i32(float x)
+{
+    if (!(x >= -2147483648.0f && x < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)x;
+}
+
+/* bvh_compressed_node<W>::init (bvh.h:352-400) in the reference layout:
+ * child_bounds (6 x W bytes), children (W x {int id; short num_prims},
+ * packed), origin (3 floats), e (3 chars); W = 4: 64 bytes, W = 8: 112 */
+void vro_compress_wide(const float* bounds, const int64_t* children, size_t num, int W, uint8_t* out)
+{
+    const size_t stride = W == 4 ? 64 : 112;
+    for (size_t n = 0; n < num; ++n) {
+        const float* b = bounds + n * 6 * (size_t)W;
+        const int64_t* c = children + n * (size_t)W;
+        uint8_t* o = out + n * stride;
+        memset(o, 0, stride);
+        /* get_bounds: the union of all W child boxes (aabb::insert) */
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int i = 0; i < W; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = tmin_f(lo[k], b[k * W + i]);
+                hi[k] = tmax_f(hi[k], b[(3 + k) * W + i]);
+            }
+        signed char e[3];
+        for (int k = 0; k < 3; ++k)
+            e[k] = (signed char)cvt_trunc_i32(ceilf(log2f((hi[k] - lo[k]) / (powf(2.0f, 8.0f) - 1.0f))));
+        for (int i = 0; i < W; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                o[k * W + i] = (uint8_t)cvt_trunc_i32(floorf((b[k * W + i] - lo[k]) / powf(2.0f, (float)e[k])));
+                o[(3 + k) * W + i] = (uint8_t)cvt_trunc_i32(ceilf((b[(3 + k) * W + i] - lo[k]) / powf(2.0f, (float)e[k])));
+            }
+            int32_t id;
+            int16_t np;
+            if (c[i] < 0) {
+                const uint64_t a = ~(uint64_t)c[i];
+                id = (int32_t)(a & 0xFFFFFFFFFFFFull);
+                np = (int16_t)(a >> 48);
+            } else {
+                id = (int32_t)c[i];
+                np = 0;
+            }
+            memcpy(o + 6 * W + 6 * i, &id, 4);
+            memcpy(o + 6 * W + 6 * i + 4, &np, 2);
+        }
+        memcpy(o + 12 * W, lo, 12);
+        memcpy(o + 12 * W + 12, e, 3);
+    }
+}

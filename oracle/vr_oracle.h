@@ -26,6 +26,10 @@
  *   make_random_seed    golden table from the reference header
  *   rng                 golden next() streams from the reference CPU engine
  *   render_ao/render_pt golden images from the reference tiled_sched path
+ *   collapse_wide       the reference's bvh_collapser run on the same binary
+ *                       trees (oracle/_ref), node for node, W = 4 and 8
+ *   compress_wide       the reference's bvh_compressed_node::init on the same
+ *                       wide nodes (oracle/_ref), byte for byte on used slots
  */
 #ifndef VR_ORACLE_H
 #define VR_ORACLE_H
@@ -175,6 +179,20 @@ size_t vro_query_log_count(void);
 void vro_render_pt_adjoint(const vro_node* nodes, const vro_tri* tris, const uint32_t* indices, size_t num_prims,
                            const vro_material* mats, const vro_camera* cam, const vro_render_params* p,
                            const float* dl_dimg, double* grad_cd, double* grad_ce, int y_begin, int y_end);
+
+/* ---- wide trees (bvh.h:167-426, detail/bvh/collapse.h:19-169) ---- */
+/* bvh_collapser::collapse of a binary tree (reference bvh_node layout) into
+ * bvh_multi_node<W>, W = 4 or 8, after the reference's removal of empty
+ * nodes; child bounds per node as minx[W] miny[W] minz[W] maxx[W] maxy[W]
+ * maxz[W] (out_bounds, 6 W floats) and children[W] (out_children, int64:
+ * >= 0 inner node, < 0 an encoded leaf ~(count << 48 | first), INT64_MAX
+ * unused).  Returns the number of nodes; max_stack the deepest the
+ * collapse's stack got (the reference's holds 64). */
+size_t vro_collapse_wide(const vro_node* nodes, size_t num_nodes, int W, float* out_bounds, int64_t* out_children,
+                         size_t* max_stack);
+/* bvh_compressed_node<W>::init of each node, in the reference's byte layout
+ * (W = 4: 64 bytes per node, W = 8: 112) */
+void vro_compress_wide(const float* bounds, const int64_t* children, size_t num, int W, uint8_t* out);
 
 #ifdef __cplusplus
 }

@@ -158,6 +158,30 @@ def instances():
                         top_nodes=top_nodes, top_indices=top_idx, boxes=boxes)
 
 
+def wide():
+    """bvh_collapser into 4- and 8-wide trees (detail/bvh/collapse.h:19-169),
+    bvh_compressed_node::init of every wide node (bvh.h:352-400), and the
+    closest hits of the reference's wide traversals (intersect_ray1_bvhN,
+    4-wide compressed too) over the GPU-layout LBVH and the reference's own
+    binned-SAH tree of the 344-triangle config-0 scene."""
+    tris = S.config0_scene(level=2)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    sah_nodes, sah_idx = O.ref_sah(tris)
+    rays = rays_for(2048, seed=21)
+    # rays from outside the scene too, some of them missing it
+    far = rays_for(1024, seed=22)
+    far["ori"] *= 4.0
+    rays = np.concatenate([rays, far])
+    out = dict(tris=tris, rays=rays, lbvh_nodes=nodes, lbvh_indices=idx, sah_nodes=sah_nodes, sah_indices=sah_idx)
+    for name, nd, ix in (("lbvh", nodes, idx), ("sah", sah_nodes, sah_idx)):
+        for w in (4, 8):
+            b, c, comp = O.ref_wide(nd, tris, ix, w)
+            out[f"{name}{w}_bounds"], out[f"{name}{w}_children"], out[f"{name}{w}_comp"] = b, c, comp
+            out[f"{name}{w}_hits"] = O.ref_trace_wide(nd, tris, ix, w, rays)
+        out[f"{name}4c_hits"] = O.ref_trace_wide(nd, tris, ix, 4, rays, compressed=True)
+    np.savez_compressed(OUT / "wide.npz", **out)
+
+
 if __name__ == "__main__":
     assert O.ref_available(), "oracle/_ref/vr_ref is not built (make -C oracle ref)"
     if len(sys.argv) > 1:
