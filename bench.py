@@ -245,7 +245,7 @@ def main():
         f"{alg_bytes / 1e9:.2f} GB algorithmic per frame -> {achieved:.0f} GB/s")
 
     # ---- a batch of incoherent closest-hit queries (vr_trace) ----
-    closest = trace_batch_bench(args, V, tris, bvh, cam, stream) if args.trace_spp > 0 else None
+    closest, d_rays = trace_batch_bench(args, V, tris, bvh, cam, stream) if args.trace_spp > 0 else (None, None)
 
 
     # ---- CPU baseline: the reference's own tiled_sched path ----
@@ -255,6 +255,7 @@ def main():
         extras["reference_scheme_frame"] = reference_scheme_frame(V, kernel, sp, stream, kernel_ms)
         extras["nee_frame"] = nee_frame(V, S, tris, mats, kernel, sp, stream, kernel_ms)
         extras["lbvh_rotated"] = rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms)
+        extras["wide_nodes"] = wide_nodes(V, bvh, kernel, sp, stream, kernel_ms, d_rays, closest)
         extras["heightfield_only"] = heightfield_only(args, V, S, stream)
 
     cpu, parity = None, None
@@ -375,6 +376,65 @@ def rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms, builds=3):
            "frame_speedup": round(kernel_ms / ms, 4)}
     log(f"rotated LBVH: build {out['build_ms_wall']:.2f} ms wall, SAH {sah_plain:.2f} -> {sah_rot:.2f}, frame "
         f"{ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} Mrays/s, x{out['frame_speedup']:.3f})")
+    return out
+
+
+def wide_nodes(V, bvh, kernel, sp, stream, kernel_ms, d_rays, closest, frames=3):
+    """The same BVH traversed through 8-wide nodes (vr_bvh_set_width: the
+    reference's bvh_collapser into bvh_multi_node<8>, and the 8-bit
+    compressed form): collapse time, the frame, node visits per ray and the
+    closest-hit batch, against the 4-wide traversal of the headline."""
+    import time as _t
+    out = {}
+    lib = V.lib()
+    for name, comp in (("oct_float", False), ("oct_compressed", True)):
+        stream.synchronize()
+        t0 = _t.perf_counter()
+        bvh.set_width(8, compressed=comp, stream=stream)
+        collapse_ms = (_t.perf_counter() - t0) * 1e3
+        sched = V.hip_sched()
+        sched.frame(kernel, sp, stream=stream)
+        stream.synchronize()
+        sched.reset_counters()
+        ms = []
+        for _ in range(frames):
+            e0, e1 = V.Event(), V.Event()
+            e0.record(stream)
+            sched.frame(kernel, sp, stream=stream)
+            e1.record(stream)
+            ms.append(e0.elapsed_ms(e1))
+        stream.synchronize()
+        sec = sched.rays()[1] / frames
+        stats = V.DeviceArray(4, np.uint64, zero=True)
+        sched.frame(kernel, sp, stream=stream, stats=stats)
+        stream.synchronize()
+        st = stats.to_host().astype(np.float64)
+        f_ms = float(np.median(ms))
+        rec = {"collapse_ms_wall": round(collapse_ms, 3), "frame_ms": round(f_ms, 3),
+               "incoherent_mrays_per_s": round(sec / (f_ms * 1e-3) / 1e6, 3),
+               "frame_speedup_vs_4wide": round(kernel_ms / f_ms, 4),
+               "per_ray": {"nodes": round(st[2] / (st[0] + st[1]), 3), "triangles": round(st[3] / (st[0] + st[1]), 3)}}
+        if d_rays is not None and closest is not None:
+            n = closest["rays"]
+            d_hits = V.DeviceArray(n * 4, np.float32)
+            V.check(lib.vr_trace(bvh.handle, 0, d_rays.ptr, n, d_hits.ptr, stream.handle))
+            tms = []
+            for _ in range(3):
+                e0, e1 = V.Event(), V.Event()
+                e0.record(stream)
+                V.check(lib.vr_trace(bvh.handle, 0, d_rays.ptr, n, d_hits.ptr, stream.handle))
+                e1.record(stream)
+                tms.append(e0.elapsed_ms(e1))
+            stream.synchronize()
+            t = float(np.median(tms))
+            rec["closest_hit_grays_per_s"] = round(n / (t * 1e-3) / 1e9, 4)
+        out[name] = rec
+        log(f"8-wide {name}: collapse {collapse_ms:.2f} ms wall, frame {f_ms:.2f} ms "
+            f"({rec['incoherent_mrays_per_s']:.1f} Mrays/s, x{rec['frame_speedup_vs_4wide']:.3f}), "
+            f"{rec['per_ray']['nodes']:.2f} node visits per ray, closest-hit "
+            f"{rec.get('closest_hit_grays_per_s', 0):.3f} Grays/s")
+    bvh.set_width(4, stream=stream)
+    stream.synchronize()
     return out
 
 
@@ -527,7 +587,7 @@ def trace_batch_bench(args, V, tris, bvh, cam, stream, reps=3):
         f"visits, {out['per_ray']['triangles']:.2f} triangles per ray; {out['hit_fraction']:.3f} hit")
     if not args.no_extras:
         out["instanced"] = instanced_batch(args, V, d_rays, rays.size, hits, t, stream)
-    return out
+    return out, d_rays
 
 
 def instanced_batch(args, V, d_rays, n, flat_prim, flat_ms, stream, reps=3):

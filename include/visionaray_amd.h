@@ -207,6 +207,27 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
  * moved vertices (same count and order as at build time). */
 int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream);
 
+/* vr_bvh_set_width flags */
+#define VR_WIDE_COMPRESSED 1u     /* 8-bit child planes, one 128-byte line per node */
+#define VR_WIDE_BINARY_LEAVES 2u  /* an LBVH's single-triangle leaves (the reference's collapse exactly) */
+/* Traversal width of a BVH: the reference's wide trees (bvh_collapser,
+ * detail/bvh/collapse.h:19-169, into index_bvh8 of bvh_multi_node<8>,
+ * bvh.h:167-312; with VR_WIDE_COMPRESSED the 8-bit form of
+ * bvh_compressed_node<8>, bvh.h:316-426, quantized outward so that a
+ * compressed tree finds exactly the hits of the uncompressed one).  Width 8
+ * collapses the binary tree into nodes of up to eight children, top down by
+ * the reference's rules; the handle keeps them through later rebuilds and
+ * refits, and vr_trace*, vr_render_pt and vr_render_pt_counted traverse them
+ * (a tree of more than 48 8-wide levels keeps the 4-wide traversal).  By
+ * default an LBVH's 8-wide leaves are the leaf ranges of its 4-wide nodes.
+ * Width 4 drops them.  Waits for `stream` (the 8-wide depth is read back). */
+int vr_bvh_set_width(vr_bvh bvh, int width, uint32_t flags, void* stream);
+/* The 8-wide nodes to the host: *bytes = their size (256-byte float nodes or
+ * 128-byte compressed ones, indexed by the pair index of their root binary
+ * node, node 0 the root; only nodes reachable from the root are written),
+ * *depth = 8-wide levels.  h_out may be null to ask for the size. */
+int vr_bvh_download_wide(vr_bvh bvh, void* h_out, size_t capacity, size_t* bytes, uint32_t* depth);
+
 /* Counts and depth.  The depth of an LBVH is read back from the device after
  * the handle's last build / rebuild / refit (a wait on that work's stream up
  * to it, not on the whole device). */

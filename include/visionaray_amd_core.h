@@ -63,6 +63,36 @@ struct alignas(128) quad_node {
 };
 static_assert(sizeof(quad_node) == 128, "quad_node is one 128-byte line");
 
+// 8-wide traversal node (bvh_multi_node<8>, bvh.h:167-312: the collapse of
+// the binary tree into nodes of up to eight children): two quad_node halves,
+// children 0-3 in h[0] and 4-7 in h[1], 256 bytes; h[0].meta.x = number of
+// children.  An inner child refers to the oct_node of its own pair index.
+struct alignas(128) oct_node {
+    quad_node h[2];
+};
+static_assert(sizeof(oct_node) == 256, "oct_node is two 128-byte lines");
+
+// The same node with 8-bit child planes (the idea of bvh_compressed_node<8>,
+// bvh.h:316-426), one 128-byte line.  Plane = fmaf(q, scale, org) in float;
+// the quantization rounds outward through exactly this decode, so every
+// decoded child box contains the child's exact box.
+//   org.xyz    origin (lower corner of the node's box); org.w = number of
+//              children (uint bits); slots >= that number are unused
+//   scale.xyz  power-of-two step per axis
+//   q          bytes 32..79: lo_x[8] lo_y[8] lo_z[8] hi_x[8] hi_y[8] hi_z[8]
+//   child      entries of slots 0-7 (as quad_node)
+struct alignas(128) octq_node {
+    float4 org;
+    float4 scale;
+    uint4 q[3];
+    uint4 child[2];
+    uint4 pad;
+};
+static_assert(sizeof(octq_node) == 128, "octq_node is one 128-byte line");
+
+// traversal node kinds of a BVH (bvh_view::wide holds the 8-wide ones)
+enum node_kind { NODE_QUAD = 0, NODE_OCT = 1, NODE_OCTQ = 2 };
+
 // Triangle in traversal order, 48 bytes = three 16-byte loads:
 //   a = v1.xyz, prim_id    b = e1.xyz, geom_id    c = e2.xyz, original index
 struct alignas(16) tri_node {
@@ -81,6 +111,7 @@ struct bvh_view {
     uint32_t num_prims;
     uint32_t root;  // entry of the root: 0 (quad 0 / pair 0) or a leaf entry; unused when num_prims == 0
     uint32_t num_quads;  // also the number of pair records
+    const void* wide;    // oct_node / octq_node per pair index, or null
 };
 
 
@@ -189,6 +220,11 @@ struct trace_counters {
 // 30 code bits + 32 index bits, grows strictly from parent to child), so its
 // collapse at most 64 quad levels: 3 * 64 + 1 entries always suffice
 constexpr int MAX_STACK = 196;
+// 8-wide nodes defer up to 7 children per level: trees of at most 48 binary
+// levels
+constexpr int MAX_STACK_WIDE = 7 * 48 + 1;
+template <int NODE>
+constexpr int max_stack_of() { return NODE == NODE_QUAD ? MAX_STACK : MAX_STACK_WIDE; }
 
 // Slab test of one child box; returns tnear, or +inf when the child is not
 // entered.  The slab distances round exactly as the reference's; the min/max
@@ -238,6 +274,12 @@ struct trav_state {
 constexpr uint32_t QN_FAR_X = 48u, QN_FAR_Y = 80u, QN_FAR_Z = 112u;
 // quads addressable with 32-bit byte offsets (node << 7)
 constexpr uint32_t QN_NEAR_FAR_MAX = 1u << 25;
+// oct_node: both halves as quad_node (node << 8, + 128 for the second half)
+constexpr uint32_t OCT_NEAR_FAR_MAX = 1u << 24;
+// octq_node byte offsets: lo_x 32, lo_y 40, lo_z 48, hi_x 56, hi_y 64,
+// hi_z 72; near plane 32 + 8 axis (+ 24 where inv < 0), half the quad_node
+// offset plus 32; the far plane at near ^ 24, ^ 104, ^ 120
+constexpr uint32_t OQ_FAR_X = 24u, OQ_FAR_Y = 104u, OQ_FAR_Z = 120u;
 
 __device__ __forceinline__ void trav_begin(trav_state& T, const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax)
 {
@@ -332,7 +374,103 @@ struct default_isect {
 // per-ray counts and results do not depend on the launch.
 enum trav_order { TRAV_VOTED = 0, TRAV_LANE = 1 };
 
-template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE, int ORDER = TRAV_VOTED, typename ISECT = default_isect>
+// The 8 (tnear, entry) pairs of a wide node's children in ascending tnear
+// (a 19-comparator network, depth 6; ties keep no particular order)
+__device__ __forceinline__ void sort8(float* k, uint32_t* e)
+{
+    cswap(k[0], e[0], k[2], e[2]);
+    cswap(k[1], e[1], k[3], e[3]);
+    cswap(k[4], e[4], k[6], e[6]);
+    cswap(k[5], e[5], k[7], e[7]);
+    cswap(k[0], e[0], k[4], e[4]);
+    cswap(k[1], e[1], k[5], e[5]);
+    cswap(k[2], e[2], k[6], e[6]);
+    cswap(k[3], e[3], k[7], e[7]);
+    cswap(k[0], e[0], k[1], e[1]);
+    cswap(k[2], e[2], k[3], e[3]);
+    cswap(k[4], e[4], k[5], e[5]);
+    cswap(k[6], e[6], k[7], e[7]);
+    cswap(k[2], e[2], k[4], e[4]);
+    cswap(k[3], e[3], k[5], e[5]);
+    cswap(k[1], e[1], k[4], e[4]);
+    cswap(k[3], e[3], k[6], e[6]);
+    cswap(k[1], e[1], k[2], e[2]);
+    cswap(k[3], e[3], k[4], e[4]);
+    cswap(k[5], e[5], k[6], e[6]);
+}
+
+// Slab tests of the eight children of wide node `node` (NODE_OCT or
+// NODE_OCTQ): tnear per slot (+inf: not entered) and the slots' entries.
+template <int NODE, typename ISECT>
+__device__ __forceinline__ void wide_children(const trav_state& T, const bvh_view& b, uint32_t node, float tc,
+                                              const ISECT& is, float* k, uint32_t* e)
+{
+    const char* nb = reinterpret_cast<const char*>(b.wide);
+    if constexpr (NODE == NODE_OCT) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // each half is a quad_node: the same near/far plane offsets
+            const uint32_t vo = (node << 8) + 128u * h;
+            const uint32_t ax = vo | (T.near & 0xFFu), ay = vo | ((T.near >> 8) & 0xFFu), az = vo | (T.near >> 16);
+            const float4 nx = *reinterpret_cast<const float4*>(nb + ax);
+            const float4 ny = *reinterpret_cast<const float4*>(nb + ay);
+            const float4 nz = *reinterpret_cast<const float4*>(nb + az);
+            const float4 fx = *reinterpret_cast<const float4*>(nb + (ax ^ QN_FAR_X));
+            const float4 fy = *reinterpret_cast<const float4*>(nb + (ay ^ QN_FAR_Y));
+            const float4 fz = *reinterpret_cast<const float4*>(nb + (az ^ QN_FAR_Z));
+            uint4 ch = *reinterpret_cast<const uint4*>(nb + vo + 96u);
+            asm volatile("" : "+v"(ch.x), "+v"(ch.y), "+v"(ch.z), "+v"(ch.w));
+            k[4 * h + 0] = is.box_nf(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, T, tc);
+            k[4 * h + 1] = is.box_nf(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, T, tc);
+            k[4 * h + 2] = is.box_nf(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, T, tc);
+            k[4 * h + 3] = is.box_nf(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, T, tc);
+            e[4 * h + 0] = ch.x;
+            e[4 * h + 1] = ch.y;
+            e[4 * h + 2] = ch.z;
+            e[4 * h + 3] = ch.w;
+        }
+    } else {
+        const uint32_t vo = node << 7;
+        const float4 org = *reinterpret_cast<const float4*>(nb + vo);
+        const float4 sc = *reinterpret_cast<const float4*>(nb + vo + 16u);
+        const uint32_t nq = ((T.near >> 1) & 0x7F7F7Fu) + 0x202020u;
+        const uint32_t ax = vo | (nq & 0xFFu), ay = vo | ((nq >> 8) & 0xFFu), az = vo | (nq >> 16);
+        const uint2 qnx = *reinterpret_cast<const uint2*>(nb + ax);
+        const uint2 qny = *reinterpret_cast<const uint2*>(nb + ay);
+        const uint2 qnz = *reinterpret_cast<const uint2*>(nb + az);
+        const uint2 qfx = *reinterpret_cast<const uint2*>(nb + (ax ^ OQ_FAR_X));
+        const uint2 qfy = *reinterpret_cast<const uint2*>(nb + (ay ^ OQ_FAR_Y));
+        const uint2 qfz = *reinterpret_cast<const uint2*>(nb + (az ^ OQ_FAR_Z));
+        uint4 c0 = *reinterpret_cast<const uint4*>(nb + vo + 80u);
+        uint4 c1 = *reinterpret_cast<const uint4*>(nb + vo + 96u);
+        asm volatile("" : "+v"(c0.x), "+v"(c0.y), "+v"(c0.z), "+v"(c0.w));
+        asm volatile("" : "+v"(c1.x), "+v"(c1.y), "+v"(c1.z), "+v"(c1.w));
+        const uint32_t cnt = __float_as_uint(org.w);
+        e[0] = c0.x;
+        e[1] = c0.y;
+        e[2] = c0.z;
+        e[3] = c0.w;
+        e[4] = c1.x;
+        e[5] = c1.y;
+        e[6] = c1.z;
+        e[7] = c1.w;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int sh = 8 * (i & 3);
+            auto plane = [&](uint2 w, float s, float o) {
+                const float q = (float)(((i < 4 ? w.x : w.y) >> sh) & 0xFFu);
+                return fmaf(q, s, o);  // exact q * s (a power of two), one rounding
+            };
+            const float t = is.box_nf(plane(qnx, sc.x, org.x), plane(qny, sc.y, org.y), plane(qnz, sc.z, org.z),
+                                      plane(qfx, sc.x, org.x), plane(qfy, sc.y, org.y), plane(qfz, sc.z, org.z),
+                                      T, tc);
+            k[i] = (uint32_t)i < cnt ? t : __builtin_inff();
+        }
+    }
+}
+
+template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE, int ORDER = TRAV_VOTED, typename ISECT = default_isect,
+          int NODE = NODE_QUAD>
 __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint32_t* __restrict__ stack,
                                           uint32_t* __restrict__ spill, trace_counters* cnt, bool any_rt = false,
                                           const ISECT& is = ISECT{})
@@ -380,6 +518,33 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
         if (--T.budget == 0 || T.cur >= b.num_quads) return true;
         if (COUNT) cnt->nodes++;
         const float tc = T.r.t;
+        if constexpr (NODE != NODE_QUAD) {
+            static_assert(ISECT::near_far, "wide nodes take the library's own box test");
+            float k[8];
+            uint32_t e[8];
+            wide_children<NODE>(T, b, T.cur, tc, is, k, e);
+            sort8(k, e);
+            if (!(k[0] < __builtin_inff())) {
+                T.cur = pop();
+            } else {
+                // deferred children, farthest deepest: e[j] goes to sp + n - j
+                int n = 0;
+#pragma unroll
+                for (int j = 1; j < 8; ++j) n += k[j] < __builtin_inff();
+                if (T.sp + 7 <= LDS_DEPTH) {
+#pragma unroll
+                    for (int j = 1; j < 8; ++j)
+                        if (n >= j) stack[(T.sp + n - j) * STRIDE] = e[j];
+                    T.sp += n;
+                } else {
+#pragma unroll
+                    for (int j = 7; j >= 1; --j)
+                        if (n >= j) push(e[j]);
+                }
+                T.cur = e[0];
+            }
+            return false;
+        }
         float k0, k1, k2, k3;
         uint4 ch;
         bool near_far = false;
@@ -466,15 +631,16 @@ __device__ __forceinline__ bool trav_iter(trav_state& T, const bvh_view& b, uint
     return false;
 }
 
-template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE, int ORDER = TRAV_VOTED, typename ISECT = default_isect>
+template <bool ANY_HIT, bool COUNT, int LDS_DEPTH, int STRIDE, int ORDER = TRAV_VOTED, typename ISECT = default_isect,
+          int NODE = NODE_QUAD>
 __device__ __forceinline__ hit_t traverse(const bvh_view& b, f3 ori, f3 dir, float tmin, float tmax,
                                           uint32_t* __restrict__ stack, trace_counters* cnt,
                                           const ISECT& is = ISECT{})
 {
     trav_state T;
-    uint32_t spill[MAX_STACK - LDS_DEPTH];
+    uint32_t spill[max_stack_of<NODE>() - LDS_DEPTH];
     trav_begin(T, b, ori, dir, tmin, tmax);
-    while (!trav_iter<ANY_HIT, COUNT, LDS_DEPTH, STRIDE, ORDER, ISECT>(T, b, stack, spill, cnt, false, is)) {
+    while (!trav_iter<ANY_HIT, COUNT, LDS_DEPTH, STRIDE, ORDER, ISECT, NODE>(T, b, stack, spill, cnt, false, is)) {
     }
     return T.r;
 }

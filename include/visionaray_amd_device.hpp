@@ -226,6 +226,7 @@ struct bvh_ref {
         view.num_prims = r.num_prims;
         view.root = r.root;
         view.num_quads = r.num_nodes;
+        view.wide = nullptr;  // user kernels traverse the 4-wide nodes
     }
     // tree.primitive(i) for a hit's primitive_list_index
     __device__ triangle primitive(unsigned i) const

@@ -78,6 +78,17 @@ def test_c1_bounce_rays_exact_hits(V, O, c1):
     anyh = V.any_hit(c1["bvh"], rays)
     assert np.array_equal(anyh["prim_id"] >= 0, want["prim_id"] >= 0)
     print(f"[c1] {rays.size} bounce rays: hits {np.mean(want['prim_id'] >= 0):.3f}, t bit-exact, ties {ties}")
+    # the same queries through the 8-wide nodes, float and compressed
+    try:
+        for compressed in (False, True):
+            c1["bvh"].set_width(8, compressed=compressed)
+            ties8 = O.check_hits(V.closest_hit(c1["bvh"], rays), want, c1["tris"], rays)
+            anyh = V.any_hit(c1["bvh"], rays)
+            assert np.array_equal(anyh["prim_id"] >= 0, want["prim_id"] >= 0)
+            print(f"[c1] 8-wide (compressed={compressed}): t bit-exact, ties {ties8}")
+            assert ties8 <= 1e-3 * rays.size
+    finally:
+        c1["bvh"].set_width(4)
 
 
 def test_c1_frame_bands_match_oracle(V, O, c1):

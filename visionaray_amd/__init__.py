@@ -23,7 +23,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BVH_INFO_DTYPE, CAMERA_DTYPE, HIT_DTYPE, KERNEL_PARAMS_DTYPE, NODE_DTYPE, RAY_DTYPE,
-                   SCHED_PARAMS_DTYPE, VR_BVH_KEEP_NODES, VRError, check, ptr)
+                   SCHED_PARAMS_DTYPE, VR_BVH_KEEP_NODES, VR_WIDE_BINARY_LEAVES, VR_WIDE_COMPRESSED, VRError, check,
+                   ptr)
 from .scenes import LIGHT_DTYPE, MAT_DTYPE, TRI_DTYPE
 
 __all__ = [
@@ -205,6 +206,23 @@ class BVH:
         p = _as_device(prims, TRI_DTYPE)
         check(lib().vr_bvh_refit(self.handle, p.ptr, _stream(stream)))
         self.prims = p
+
+    def set_width(self, width: int, compressed: bool = False, binary_leaves: bool = False, stream=None):
+        """vr_bvh_set_width: 8-wide traversal nodes (bvh_collapser into
+        bvh_multi_node<8>, detail/bvh/collapse.h:19-169; compressed: 8-bit
+        child planes as bvh_compressed_node<8>, bvh.h:316-426), or back to 4."""
+        flags = (VR_WIDE_COMPRESSED if compressed else 0) | (VR_WIDE_BINARY_LEAVES if binary_leaves else 0)
+        check(lib().vr_bvh_set_width(self.handle, int(width), flags, _stream(stream)))
+
+    def wide_nodes(self):
+        """(raw 8-wide node bytes (num_pairs, 256 | 128) uint8, 8-wide depth)"""
+        n, d = C.c_size_t(), C.c_uint32()
+        check(lib().vr_bvh_download_wide(self.handle, None, 0, C.byref(n), C.byref(d)))
+        out = np.zeros(n.value, np.uint8)
+        if n.value:
+            check(lib().vr_bvh_download_wide(self.handle, C.c_void_p(out.ctypes.data), n.value, C.byref(n),
+                                             C.byref(d)))
+        return out, int(d.value)
 
     def destroy(self):
         if self.handle and self.handle.value:

@@ -23,6 +23,8 @@ VR_ERR_NOT_INITIALIZED = 4
 VR_ERR_UNSUPPORTED = 5
 VR_BVH_KEEP_NODES = 1
 VR_BVH_ROTATE = 2
+VR_WIDE_COMPRESSED = 1
+VR_WIDE_BINARY_LEAVES = 2
 
 # every symbol the C header declares: name -> (restype, argtypes)
 P = C.c_void_p
@@ -56,6 +58,8 @@ SIGNATURES = {
     "vr_bvh_sorted_refs": (I, [P, P, P, P]),
     "vr_bvh_from_nodes": (I, [P, SZ, P, SZ, P, C.POINTER(P)]),
     "vr_bvh_refit": (I, [P, P, P]),
+    "vr_bvh_set_width": (I, [P, I, U32, P]),
+    "vr_bvh_download_wide": (I, [P, P, SZ, C.POINTER(SZ), C.POINTER(U32)]),
     "vr_bvh_get_info": (I, [P, P]),
     "vr_bvh_get_ref": (I, [P, P]),
     "vr_bvh_nodes": (I, [P, P, P]),

@@ -360,6 +360,7 @@ int vr_lbvh_rebuild(vr_bvh bvh, const vr_triangle* d_prims, size_t n, void* stre
     int rc = ensure_device();
     if (rc) return rc;
     rc = lbvh_build_into(bvh->s, d_prims, n, bvh->s.ref_nodes != nullptr, bvh->s.want_rotate, (hipStream_t)stream);
+    if (rc == VR_OK) rc = collapse_wide(bvh->s, (hipStream_t)stream);
     return rc ? rc : mark_built(bvh, (hipStream_t)stream);
     VR_TRY_END
 }
@@ -516,8 +517,50 @@ int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream)
 {
     VR_CHECK(bvh && (bvh->s.num_prims == 0 || d_prims), VR_ERR_INVALID_ARGUMENT, "vr_bvh_refit: null argument");
     VR_CHECK(bvh->s.is_lbvh, VR_ERR_UNSUPPORTED, "vr_bvh_refit: only LBVHs are refitted on the device");
-    const int rc = lbvh_refit(bvh->s, d_prims, (hipStream_t)stream);
+    int rc = lbvh_refit(bvh->s, d_prims, (hipStream_t)stream);
+    if (rc == VR_OK) rc = collapse_wide(bvh->s, (hipStream_t)stream);
     return rc ? rc : mark_built(bvh, (hipStream_t)stream);
+}
+
+int vr_bvh_set_width(vr_bvh bvh, int width, uint32_t flags, void* stream)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(bvh, VR_ERR_INVALID_ARGUMENT, "vr_bvh_set_width: null handle");
+    VR_CHECK(width == 4 || width == 8, VR_ERR_INVALID_ARGUMENT, "vr_bvh_set_width: width is 4 or 8");
+    VR_CHECK((flags & ~(VR_WIDE_COMPRESSED | VR_WIDE_BINARY_LEAVES)) == 0, VR_ERR_INVALID_ARGUMENT,
+             "vr_bvh_set_width: unknown flags");
+    bvh_state& s = bvh->s;
+    int cur = 0;
+    VR_HIP_CHECK(hipGetDevice(&cur));
+    VR_CHECK(cur == s.device, VR_ERR_INVALID_ARGUMENT, "vr_bvh_set_width: the BVH lives on another device");
+    if (width == 4) {
+        if (s.wide) VR_HIP_CHECK(hipFreeAsync(s.wide, (hipStream_t)stream));
+        s.wide = nullptr;
+        s.wide_bytes = 0;
+        s.wide_depth = 0;
+        s.wide_kind = NODE_QUAD;
+        return VR_OK;
+    }
+    s.wide_kind = (flags & VR_WIDE_COMPRESSED) ? NODE_OCTQ : NODE_OCT;
+    s.wide_binary_leaves = (flags & VR_WIDE_BINARY_LEAVES) != 0;
+    int rc = collapse_wide(s, (hipStream_t)stream);
+    if (rc == VR_OK) rc = mark_built(bvh, (hipStream_t)stream);
+    return rc;
+    VR_TRY_END
+}
+
+int vr_bvh_download_wide(vr_bvh bvh, void* h_out, size_t capacity, size_t* bytes, uint32_t* depth)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(bvh && bytes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_download_wide: null argument");
+    const bvh_state& s = bvh->s;
+    const size_t node_bytes = s.wide_kind == NODE_OCT ? sizeof(oct_node) : sizeof(octq_node);
+    *bytes = s.wide ? s.num_pairs * node_bytes : 0;
+    if (depth) *depth = s.wide_depth;
+    if (!h_out || *bytes == 0) return VR_OK;
+    VR_CHECK(capacity >= *bytes, VR_ERR_INVALID_ARGUMENT, "vr_bvh_download_wide: buffer too small");
+    return read_after_build(bvh, h_out, s.wide, *bytes);
+    VR_TRY_END
 }
 
 int vr_bvh_get_info(vr_bvh bvh, vr_bvh_info* info)
@@ -584,6 +627,7 @@ int vr_bvh_destroy_async(vr_bvh bvh, void* stream)
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(s.device);
+    if (s.wide) (void)hipFreeAsync(s.wide, (hipStream_t)stream);
     if (s.pooled) {
         // stream-ordered release: work enqueued on `stream` before this call
         // still sees the arrays; the caller orders other streams' use before it
@@ -618,6 +662,7 @@ int vr_bvh_destroy(vr_bvh bvh)
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(s.device);
         (void)hipDeviceSynchronize();
+        if (s.wide) (void)hipFreeAsync(s.wide, nullptr);
         if (s.arena) {
             (void)hipFreeAsync(s.arena, nullptr);
         } else {
@@ -626,6 +671,14 @@ int vr_bvh_destroy(vr_bvh bvh)
         }
         (void)hipSetDevice(cur);
     } else {
+        if (s.wide) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(s.device);
+            (void)hipDeviceSynchronize();
+            (void)hipFreeAsync(s.wide, nullptr);
+            (void)hipSetDevice(cur);
+        }
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }

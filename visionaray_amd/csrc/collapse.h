@@ -174,6 +174,60 @@ __device__ inline void make_quad(const Src& src, uint32_t p, uint32_t num_pairs,
     quads[p] = qn;
 }
 
+// The up to eight children of the 8-wide node rooted at pair p, by the
+// rules of the reference's bvh_collapser (detail/bvh/collapse.h:37-110):
+// open the inner child of largest surface area (strictly above zero, the
+// first of equal ones) while the node has fewer than eight children; the
+// root never opens a child whose two children are both leaves.  The opened
+// child's first child takes its slot and its second child is appended.
+// With leaf_max > 0 (an LBVH) a subtree of at most leaf_max triangles counts
+// as a leaf, as in the 4-wide collapse.  Returns the number of children.
+template <bool RANGES, typename Src>
+__device__ inline int make_wide8(const Src& src, uint32_t p, uint32_t num_pairs, uint32_t leaf_max, child_ref* c)
+{
+    const bool lbvh = leaf_max > 0;
+    auto openable = [&](const child_ref& x) {
+        return !(x.e & LEAF_BIT) && x.e < num_pairs && (!lbvh || x.n > leaf_max);
+    };
+    pair_children<RANGES>(src, num_pairs, p, lbvh, c[0], c[1]);
+    int nc = 2;
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+        if (nc != 2 + step) break;
+        int best = -1;
+        float best_a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 2 + step; ++k) {
+            if (!openable(c[k])) continue;
+            if (p == 0) {
+                child_ref a, b;
+                pair_children<RANGES>(src, num_pairs, c[k].e, lbvh, a, b);
+                if (!openable(a) && !openable(b)) continue;
+            }
+            f3 lo, hi;
+            src.box(c[k].loc, lo, hi);
+            const float a = half_area(lo, hi);
+            if (a > best_a) {
+                best_a = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        uint32_t eb = c[0].e;
+#pragma unroll
+        for (int k = 1; k < 2 + step; ++k)
+            if (k == best) eb = c[k].e;
+        child_ref a, b;
+        pair_children<RANGES>(src, num_pairs, eb, lbvh, a, b);
+#pragma unroll
+        for (int k = 0; k < 2 + step; ++k)
+            if (k == best) c[k] = a;
+        c[2 + step] = b;
+        ++nc;
+    }
+    return nc;
+}
+
 // a pair of at most leaf_max triangles is a leaf of its parent's quad, never
 // a quad itself (the root pair is kept: a tree that small still starts at
 // quad 0)

@@ -442,7 +442,7 @@ __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 #define VR_PT_REGEN_LANES 32
 #endif
 
-template <int MODE, bool BATCH, int BLK, int LDSD>
+template <int MODE, bool BATCH, int BLK, int LDSD, int NODE = NODE_QUAD>
 __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = (MODE & ~PT_NEE) == PT_COUNT;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
     // the lane's query: in flight (tracing) or finished and waiting to be
     // shaded (pending)
     trav_state T;
-    uint32_t spill[MAX_STACK - LDSD];
+    uint32_t spill[max_stack_of<NODE>() - LDSD];
     bool tracing = false, pending = false;
     bvh_view bv{};  // BATCH: the BVH of the lane's scene, kept in registers
 
@@ -608,7 +608,8 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
             if (__ballot(tracing) == 0) break;
             if (__popcll(__ballot(pending)) >= VR_PT_REGEN_LANES) break;
             if (tracing &&
-                trav_iter<false, COUNT, LDSD, BLK>(T, BATCH ? bv : A.single.b, st, spill, &tc, NEE && Q.shadow)) {
+                trav_iter<false, COUNT, LDSD, BLK, TRAV_VOTED, default_isect, NODE>(T, BATCH ? bv : A.single.b, st,
+                                                                                    spill, &tc, NEE && Q.shadow)) {
                 tracing = false;
                 pending = true;
             }
@@ -766,7 +767,7 @@ __global__ __launch_bounds__(BLOCK) void k_ao(ao_args A)
 
 // One compiled launch shape: a persistent grid of exactly the blocks the
 // device holds at once (occupancy x CUs), computed once per shape.
-template <int MODE, bool BATCH, int BLK, int LDSD>
+template <int MODE, bool BATCH, int BLK, int LDSD, int NODE = NODE_QUAD>
 int launch_variant(const pt_args& A, size_t dyn_lds, hipStream_t st)
 {
     static int grid = 0;
@@ -774,13 +775,13 @@ int launch_variant(const pt_args& A, size_t dyn_lds, hipStream_t st)
     if (grid == 0 || grid_lds != dyn_lds) {
         int dev = 0, per_cu = 0, cus = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
-        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE, BATCH, BLK, LDSD>, BLK,
-                                                                 dyn_lds));
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pathtrace<MODE, BATCH, BLK, LDSD, NODE>,
+                                                                 BLK, dyn_lds));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         grid = std::max(1, per_cu) * std::max(1, cus);
         grid_lds = dyn_lds;
     }
-    hipLaunchKernelGGL((k_pathtrace<MODE, BATCH, BLK, LDSD>), dim3(grid), dim3(BLK), dyn_lds, st, A);
+    hipLaunchKernelGGL((k_pathtrace<MODE, BATCH, BLK, LDSD, NODE>), dim3(grid), dim3(BLK), dyn_lds, st, A);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
@@ -894,6 +895,19 @@ int render_pt_shaped(const bvh_state& bs, const vr_material* mats, size_t num_ma
         single_args(bs, mats, num_mats, kp, cam, sp, accum, color, nullptr, ray_counts, work, nullptr, g_arg_order);
     A.tile = shape.tile;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(1), 0, st, work, nullptr, 0);
+    // a BVH with 8-wide nodes: the persistent kernel over them (default
+    // block size and stack depth)
+    if (shape.schedule != VR_SCHEDULE_PER_PIXEL) {
+        switch (traversal_kind(bs)) {
+        case NODE_OCT:
+            return counted ? launch_variant<PT_COUNT, false, 256, 16, NODE_OCT>(A, 0, st)
+                           : launch_variant<PT_RENDER, false, 256, 16, NODE_OCT>(A, 0, st);
+        case NODE_OCTQ:
+            return counted ? launch_variant<PT_COUNT, false, 256, 16, NODE_OCTQ>(A, 0, st)
+                           : launch_variant<PT_RENDER, false, 256, 16, NODE_OCTQ>(A, 0, st);
+        default: break;
+        }
+    }
     return counted ? launch_pt<PT_COUNT, false>(A, 0, shape, st) : launch_pt<PT_RENDER, false>(A, 0, shape, st);
 }
 

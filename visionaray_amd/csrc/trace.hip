@@ -38,14 +38,14 @@ std::mutex g_scratch_mutex;
 constexpr int BLOCK = 256;
 constexpr uint32_t CHUNK = 64;
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, int NODE>
 __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __restrict__ rays, uint32_t n,
                                                  vr_hit* __restrict__ hits, uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ work)
 {
     __shared__ uint32_t stack[LDS_STACK * BLOCK];
     uint32_t* st = stack + threadIdx.x;
-    uint32_t spill[MAX_STACK - LDS_STACK];
+    uint32_t spill[max_stack_of<NODE>() - LDS_STACK];
     trav_state T;
     trace_counters c{0, 0};
     const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
             }
         }
         if (__ballot(!done) == 0) break;
-        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK, COUNT ? TRAV_LANE : TRAV_VOTED>(T, b, st, spill, &c)) {
+        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK, COUNT ? TRAV_LANE : TRAV_VOTED, default_isect, NODE>(
+                           T, b, st, spill, &c)) {
             float4 o;
             if (T.r.tri >= 0) {
                 const float prim = b.tris[T.r.tri].a.w;  // prim_id bits
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
 __global__ void k_reset(uint32_t* work) { *work = 0u; }
 
 // a resident grid: the blocks the device holds at once
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, int NODE>
 int launch_trace(const bvh_state& bs, const vr_ray* rays, uint32_t n, vr_hit* hits, uint32_t* counts,
                  uint32_t* work, hipStream_t st)
 {
@@ -119,15 +120,15 @@ int launch_trace(const bvh_state& bs, const vr_ray* rays, uint32_t n, vr_hit* hi
     if (grid == 0) {
         int dev = 0, per_cu = 0, cus = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
-        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<ANY, COUNT>, BLOCK, 0));
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<ANY, COUNT, NODE>, BLOCK, 0));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         grid = std::max(1, per_cu) * std::max(1, cus);
     }
     // fewer rays than lanes: no more waves than there are 64-ray chunks
     const unsigned blocks = std::max(1u, std::min((unsigned)grid, div_up(n, BLOCK)));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, work);
-    hipLaunchKernelGGL((k_trace<ANY, COUNT>), dim3(blocks), dim3(BLOCK), 0, st, bs.view(), rays, n, hits, counts,
-                       work);
+    hipLaunchKernelGGL((k_trace<ANY, COUNT, NODE>), dim3(blocks), dim3(BLOCK), 0, st, bs.view(), rays, n, hits,
+                       counts, work);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
@@ -136,6 +137,19 @@ int launch_trace(const bvh_state& bs, const vr_ray* rays, uint32_t n, vr_hit* hi
 
 int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n, vr_hit* hits, uint32_t* counts,
                 uint32_t* work, hipStream_t st);
+
+namespace {
+template <int NODE>
+int trace_kind(const bvh_state& bs, bool any_hit, const vr_ray* rays, uint32_t m, vr_hit* hits, uint32_t* counts,
+               uint32_t* work, hipStream_t st)
+{
+    if (counts)
+        return any_hit ? launch_trace<true, true, NODE>(bs, rays, m, hits, counts, work, st)
+                       : launch_trace<false, true, NODE>(bs, rays, m, hits, counts, work, st);
+    return any_hit ? launch_trace<true, false, NODE>(bs, rays, m, hits, nullptr, work, st)
+                   : launch_trace<false, false, NODE>(bs, rays, m, hits, nullptr, work, st);
+}
+}  // namespace
 
 // Look up the current device's address of `scratch` once (vr_init, else the
 // first vr_trace_noalloc call on the device).
@@ -175,11 +189,11 @@ int trace_batch(const bvh_state& bs, bool any_hit, const vr_ray* rays, size_t n,
     VR_CHECK(n <= (1ull << 31), VR_ERR_INVALID_ARGUMENT, "trace: more than 2^31 rays in one call");
     VR_CHECK(stack_fits(bs), VR_ERR_UNSUPPORTED, "trace: BVH deeper than the traversal stack");
     const uint32_t m = (uint32_t)n;
-    if (counts)
-        return any_hit ? launch_trace<true, true>(bs, rays, m, hits, counts, work, st)
-                       : launch_trace<false, true>(bs, rays, m, hits, counts, work, st);
-    return any_hit ? launch_trace<true, false>(bs, rays, m, hits, nullptr, work, st)
-                   : launch_trace<false, false>(bs, rays, m, hits, nullptr, work, st);
+    switch (traversal_kind(bs)) {
+    case NODE_OCT: return trace_kind<NODE_OCT>(bs, any_hit, rays, m, hits, counts, work, st);
+    case NODE_OCTQ: return trace_kind<NODE_OCTQ>(bs, any_hit, rays, m, hits, counts, work, st);
+    default: return trace_kind<NODE_QUAD>(bs, any_hit, rays, m, hits, counts, work, st);
+    }
 }
 
 }  // namespace vr

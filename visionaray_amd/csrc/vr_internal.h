@@ -81,6 +81,13 @@ struct bvh_state {
     uint32_t* indices = nullptr;        // tree.indices()
     size_t num_ref_nodes = 0;
     float* root_box = nullptr;          // 6 floats
+    // 8-wide nodes (vr_bvh_set_width): NODE_OCT / NODE_OCTQ per pair index,
+    // remade after every build, rebuild and refit of the handle
+    int wide_kind = NODE_QUAD;
+    bool wide_binary_leaves = false;    // collapse over single-triangle leaves (the reference's rules)
+    void* wide = nullptr;
+    size_t wide_bytes = 0;
+    uint32_t wide_depth = 0;            // levels of 8-wide nodes
     bool rotated = false;               // LBVH after tree rotations (no refit, no reference layout)
     bool want_rotate = false;           // LBVH builds of this handle run the rotation pass
     bool pooled = false;                // arrays from the stream-ordered pool (hipMallocAsync)
@@ -88,7 +95,10 @@ struct bvh_state {
     void* scratch = nullptr;            // build scratch
     size_t scratch_bytes = 0;
 
-    bvh_view view() const { return bvh_view{quads, tris, pairs, (uint32_t)num_prims, root, (uint32_t)num_pairs}; }
+    bvh_view view() const
+    {
+        return bvh_view{quads, tris, pairs, (uint32_t)num_prims, root, (uint32_t)num_pairs, wide};
+    }
 };
 
 
@@ -100,6 +110,18 @@ inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b);
 
 // the traversal stack holds the deferred children of a tree of this depth
 inline bool stack_fits(const bvh_state& s) { return 3u * s.depth + 1u <= (uint32_t)MAX_STACK; }
+// the node kind the traversal kernels use: the 8-wide nodes when the handle
+// has them and its tree fits the wide stack and 32-bit node offsets, else
+// the 4-wide ones
+inline int traversal_kind(const bvh_state& s)
+{
+    if (s.wide_kind == NODE_QUAD || !s.wide || s.num_prims == 0 || s.num_pairs == 0) return NODE_QUAD;
+    if (s.wide_depth == 0 || 7u * s.wide_depth + 1u > (uint32_t)MAX_STACK_WIDE) return NODE_QUAD;
+    if (s.num_pairs > (s.wide_kind == NODE_OCT ? OCT_NEAR_FAR_MAX : QN_NEAR_FAR_MAX)) return NODE_QUAD;
+    return s.wide_kind;
+}
+// (re)make the 8-wide nodes of s.wide_kind from the pair records
+int collapse_wide(bvh_state& s, hipStream_t st);
 
 // Collapse the binary pair records of `s` into its 4-wide traversal nodes
 // (s.quads, s.root): quad p is the collapse rooted at pair p.  leaf_max > 0
