@@ -135,6 +135,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--no-extras", action="store_true",
                     help="c1: skip the reference-scheme frame and the heightfield-only run")
+    ap.add_argument("--build-only", action="store_true", help="c1: time the LBVH builds alone (--builds of them)")
     ap.add_argument("--no-c2", action="store_true", help="c1: skip the batched-scenes measurement (C2)")
     ap.add_argument("--no-c3", action="store_true", help="c1: skip the adjoint-vs-FD measurement (C3)")
     ap.add_argument("--c2-steps", type=int, default=2, help="c1: timed frames of the C2 measurement")
@@ -196,6 +197,12 @@ def main():
     info = bvh.info()
     lbvh_ms = float(np.median(build_ms))
     log(f"LBVH: {lbvh_ms:.3f} ms median of {len(build_ms)} (depth {info['depth']})")
+    if args.build_only:
+        print(json.dumps({"metric": "LBVH build ms", "value": round(lbvh_ms, 4), "unit": "ms",
+                          "higher_is_better": False, "builds": len(build_ms), "depth": info["depth"],
+                          "config": {"workload": f"C1 large scene, {tris.size} fp32 triangles, GPU LBVH"}}),
+              flush=True)
+        return
 
     # ---- path tracing ----
     eye, center, up, fovy, aspect = S.config1_camera()

@@ -180,8 +180,11 @@ int vr_camera_look_at(const float eye[3], const float center[3], const float up[
 /* ---- BVH construction ------------------------------------------------- */
 /* lbvh_builder::build(cuda_index_bvh<P>{}, P* prims, n)
  * (detail/bvh/lbvh.h:681-844): Morton codes, stable radix sort, Karras
- * hierarchy, bottom-up boxes.  d_prims stays owned by the caller and is
- * copied into the BVH.  n == 0 yields an empty BVH. */
+ * hierarchy, bottom-up boxes.  d_prims stays owned by the caller; the BVH
+ * keeps its triangles in its own traversal order, read from d_prims by the
+ * build's kernels, so d_prims must stay valid until the build's work on
+ * `stream` has run (the reference copies the primitives first, from the same
+ * stream-ordered buffer).  n == 0 yields an empty BVH. */
 int vr_lbvh_build(const vr_triangle* d_prims, size_t n, uint32_t flags, void* stream, vr_bvh* out);
 /* The same build into an existing LBVH handle (a new frame of moving
  * geometry): its device arrays are reused when n is unchanged, so no device
@@ -204,7 +207,8 @@ int vr_bvh_from_nodes(const vr_triangle* h_prims, size_t num_prims, const vr_bvh
                       size_t num_nodes, const uint32_t* h_indices, vr_bvh* out);
 
 /* bvh_refitter::refit (detail/bvh/refit.h:28-114): recompute every box for
- * moved vertices (same count and order as at build time). */
+ * moved vertices (same count and order as at build time); d_prims as for
+ * vr_lbvh_build. */
 int vr_bvh_refit(vr_bvh bvh, const vr_triangle* d_prims, void* stream);
 
 /* vr_bvh_set_width flags */

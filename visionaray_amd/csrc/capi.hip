@@ -614,6 +614,7 @@ int vr_bvh_indices(vr_bvh bvh, uint32_t* d_indices, void* stream)
     VR_CHECK(bvh && d_indices, VR_ERR_INVALID_ARGUMENT, "vr_bvh_indices: null argument");
     const bvh_state& s = bvh->s;
     if (s.num_prims == 0) return VR_OK;
+    if (s.is_lbvh) return lbvh_indices(d_indices, s.num_prims, (hipStream_t)stream);
     VR_HIP_CHECK(hipMemcpyAsync(d_indices, s.indices, s.num_prims * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return VR_OK;
 }

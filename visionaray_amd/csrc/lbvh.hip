@@ -586,8 +586,8 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
                                                            vr_bvh_node* __restrict__ ref_nodes,
                                                            float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
                                                            uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count,
-                                                           uint32_t* __restrict__ indices,
-                                                           quad_node* __restrict__ quads, uint32_t leaf_max)
+                                                           int slot_heights, quad_node* __restrict__ quads,
+                                                           uint32_t leaf_max)
 {
     __shared__ float s_box[CHUNK * 12];
     __shared__ uint32_t s_h[CHUNK * 2];
@@ -604,7 +604,6 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     const size_t j = c0 + tid;
     // the leaf and its climb; every thread then meets at the barrier below
     auto leaf_and_climb = [&]() {
-        if (indices) indices[j] = (uint32_t)j;  // index_bvh_t's primitive list is the identity
         // both per-leaf words before the triangle gather and its stores: a
         // wait for a load issued after the stores would wait for the stores too
         const int32_t id = sorted_ids[j];
@@ -644,7 +643,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
             if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
             store_box(&s_box[k * 12 + side * 6], lo, hi);
             s_h[2 * k + side] = h;
-            heights[2 * p + side] = h;  // every child slot's height (the rotation pass reads them)
+            if (slot_heights) heights[2 * p + side] = h;  // every child slot's height, for the rotation pass
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (atomicAdd(&s_flag[k], 1u) == 0u) return;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -880,18 +879,17 @@ int launch_morton(const vr_triangle* prims, size_t n, vr_triangle* dst, float* p
 
 // the bottom-up passes; a build has cleared the ready-list counters already
 // (k_morton) and writes the identity index list on the way (a refit keeps it)
-void launch_bottom_up(const bvh_state& bs, const build_scratch& s, vr_bvh_node* ref_nodes, bool build,
-                      hipStream_t st)
+void launch_bottom_up(const bvh_state& bs, const build_scratch& s, const vr_triangle* prims, vr_bvh_node* ref_nodes,
+                      bool build, hipStream_t st)
 {
     const size_t n = bs.num_prims;
     const size_t nb = (n + SUPER - 1) / SUPER;
     uint32_t* depth = reinterpret_cast<uint32_t*>(bs.root_box + 6);
     if (!build)
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
-    hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, bs.prims, n, bs.sorted_ids,
+    hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, prims, n, bs.sorted_ids,
                        s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
-                       s.keys0, s.counts, build ? reinterpret_cast<uint32_t*>(bs.indices) : nullptr, bs.quads,
-                       LBVH_LEAF_MAX);
+                       s.keys0, s.counts, (int)(build && bs.want_rotate), bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)CHUNK)
         hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags,
                            bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb,
@@ -936,9 +934,12 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         if (keep_nodes) bs.num_ref_nodes = 2 * n - 1;
         build_scratch plan = plan_scratch(n, nullptr);
         bs.scratch_bytes = plan.bytes;
+        // no copy of the primitives and no index list: the build reads the
+        // caller's triangles (valid until its work on the stream is done), a
+        // refit the new ones, and an LBVH's index list is the identity
         const size_t sizes[] = {np1 * sizeof(pair_node), np1 * sizeof(quad_node), n * sizeof(tri_node),
-                                n * sizeof(vr_triangle), n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
-                                n * 4, bs.num_ref_nodes * sizeof(vr_bvh_node), plan.bytes};
+                                0, n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
+                                0, bs.num_ref_nodes * sizeof(vr_bvh_node), plan.bytes};
         size_t total = 0;
         for (size_t b : sizes) total += align_up(b, 256);
         VR_HIP_CHECK(hipMallocAsync(&bs.arena, total, st));
@@ -968,7 +969,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
     const size_t nb = (n + SUPER - 1) / SUPER;
     const morton_extras extras{s.hist, tiles, s.leaf_parent, bs.flags, s.counts, (uint32_t)(nb + 1)};
-    int rc = launch_morton(d_prims, n, bs.prims, s.part, s.cb, s.keys0, extras, st);
+    int rc = launch_morton(d_prims, n, nullptr, s.part, s.cb, s.keys0, extras, st);
     if (rc) return rc;
 
     // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
@@ -999,7 +1000,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     // 5: boxes, traversal-order triangles, reference layout, index list,
     // and the 4-wide traversal nodes (each pair's quad as soon as its
     // subtree is complete)
-    launch_bottom_up(bs, s, keep_nodes ? bs.ref_nodes : nullptr, true, st);
+    launch_bottom_up(bs, s, d_prims, keep_nodes ? bs.ref_nodes : nullptr, true, st);
     VR_HIP_CHECK(hipGetLastError());
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
     bs.max_leaf = 1;
@@ -1029,7 +1030,6 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     if (n == 0) return VR_OK;
     VR_CHECK(!bs.rotated, VR_ERR_UNSUPPORTED, "refit: a rotated LBVH is rebuilt (vr_lbvh_rebuild), not refitted");
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
-    VR_HIP_CHECK(hipMemcpyAsync(bs.prims, d_prims, n * sizeof(vr_triangle), hipMemcpyDeviceToDevice, st));
     // the hierarchy (child entries, parent links, ranges, leaf parents) is
     // unchanged since the build; only the boxes are recomputed
     if (n > 1)
@@ -1037,7 +1037,24 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
                            bs.num_pairs, 0u);
     // same topology: the bottom-up pass makes the quads from the new boxes
     // (the greedy choices may move)
-    launch_bottom_up(bs, s, bs.ref_nodes, false, st);
+    launch_bottom_up(bs, s, d_prims, bs.ref_nodes, false, st);
+    VR_HIP_CHECK(hipGetLastError());
+    return VR_OK;
+}
+
+namespace {
+__global__ void k_iota(uint32_t* __restrict__ d, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) d[i] = (uint32_t)i;
+}
+}  // namespace
+
+// an LBVH's index list (tree.indices()): leaves name their primitives by
+// original index, so the list is the identity
+int lbvh_indices(uint32_t* d, size_t n, hipStream_t st)
+{
+    if (n == 0) return VR_OK;
+    hipLaunchKernelGGL(k_iota, dim3(std::max(1u, std::min(div_up(n, BLOCK), 4096u))), dim3(BLOCK), 0, st, d, n);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
