@@ -272,6 +272,15 @@ int vr_bvh_destroy_async(vr_bvh bvh, void* stream);
 int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes,
                  size_t* num_nodes, uint32_t* h_indices);
 
+/* The same binned SAH build on the device, into a BVH handle: the tree is
+ * the reference's node for node (the node array and index list of
+ * vr_bvh_nodes / vr_bvh_indices equal binned_sah_builder's, boxes up to the
+ * sign of a zero), built one level at a time.  d_prims as for
+ * vr_lbvh_build; max_leaf_size <= 0 means 4.  VR_ERR_UNSUPPORTED for a tree
+ * with a leaf of more than 32 primitives (the traversal's limit).  Waits for
+ * `stream` once per tree level (the level's node counts are read back). */
+int vr_sah_build_device(const vr_triangle* d_prims, size_t n, int max_leaf_size, void* stream, vr_bvh* out);
+
 /* Host LBVH build in the reference's CPU layout (lbvh_builder::build on an
  * index_bvh, detail/bvh/lbvh.h:500-656 driven by build_top_down): Morton
  * order, splits at the highest differing code bit, equal codes split at the

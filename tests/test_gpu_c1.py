@@ -91,6 +91,24 @@ def test_c1_bounce_rays_exact_hits(V, O, c1):
         c1["bvh"].set_width(4)
 
 
+def test_c1_device_sah_equals_host_sah(V, c1):
+    """the binned SAH tree of the full scene on the device, node for node the
+    host build's (the reference's tree)"""
+    import time
+    t0 = time.perf_counter()
+    want_nodes, want_idx = V.sah_build(c1["tris"])
+    t1 = time.perf_counter()
+    bvh = V.sah_build_device(c1["tris"])
+    t2 = time.perf_counter()
+    got = bvh.nodes()
+    assert got.size == want_nodes.size
+    for k in ("first", "num_prims", "axis", "sign", "lo", "hi"):
+        assert np.array_equal(got[k], want_nodes[k]), k
+    assert np.array_equal(bvh.indices(), want_idx)
+    print(f"[c1] binned SAH: {got.size} nodes node for node; host {t1 - t0:.2f} s, device {t2 - t1:.3f} s "
+          f"(with upload)")
+
+
 def test_c1_frame_bands_match_oracle(V, O, c1):
     d_mats = V.DeviceArray.from_host(c1["mats"])
     kern = V.pathtracing.kernel(c1["bvh"], d_mats, c1["kp"])

@@ -28,7 +28,7 @@ from ._lib import (BVH_INFO_DTYPE, CAMERA_DTYPE, HIT_DTYPE, KERNEL_PARAMS_DTYPE,
 from .scenes import LIGHT_DTYPE, MAT_DTYPE, TRI_DTYPE
 
 __all__ = [
-    "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build",
+    "init", "DeviceArray", "Stream", "Event", "lbvh_builder", "BVH", "bvh_from_nodes", "sah_build", "sah_build_device",
     "closest_hit", "any_hit", "trace", "sah_cost_host", "optimize_tree_rotations", "pinhole_camera", "make_kernel_params", "make_sched_params",
     "jittered_blend_type", "pathtracing", "ao_kernel", "hip_sched", "VRError", "TRI_DTYPE", "MAT_DTYPE",
     "RAY_DTYPE", "HIT_DTYPE", "NODE_DTYPE", "CAMERA_DTYPE", "LIGHT_DTYPE", "synchronize", "model",
@@ -279,6 +279,16 @@ def sah_build(prims: np.ndarray, max_leaf_size: int = 4):
     nn = C.c_size_t()
     check(lib().vr_sah_build(ptr(prims), n, max_leaf_size, ptr(nodes), C.byref(nn), ptr(idx)))
     return nodes[: nn.value].copy(), idx[:n].copy()
+
+
+def sah_build_device(prims, max_leaf_size: int = 4, stream=None) -> BVH:
+    """Binned SAH build on the device (vr_sah_build_device): the reference's
+    binned_sah_builder tree node for node, as a BVH handle (nodes() and
+    indices() give the reference layout)."""
+    p = _as_device(prims, TRI_DTYPE)
+    h = C.c_void_p()
+    check(lib().vr_sah_build_device(p.ptr, p.count, int(max_leaf_size), _stream(stream), C.byref(h)))
+    return BVH(h, p)
 
 
 def sah_cost_host(nodes: np.ndarray, ci: float = 1.2, cl: float = 0.0, cp: float = 1.0) -> float:

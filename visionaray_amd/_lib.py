@@ -67,6 +67,7 @@ SIGNATURES = {
     "vr_bvh_destroy": (I, [P]),
     "vr_bvh_destroy_async": (I, [P, P]),
     "vr_sah_build": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
+    "vr_sah_build_device": (I, [P, SZ, I, P, C.POINTER(P)]),
     "vr_lbvh_build_host": (I, [P, SZ, I, P, C.POINTER(SZ), P]),
     "vr_trace": (I, [P, I, P, SZ, P, P]),
     "vr_trace_noalloc": (I, [P, I, P, SZ, P, P]),

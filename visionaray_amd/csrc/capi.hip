@@ -688,6 +688,26 @@ int vr_bvh_destroy(vr_bvh bvh)
     return VR_OK;
 }
 
+int vr_sah_build_device(const vr_triangle* d_prims, size_t n, int max_leaf_size, void* stream, vr_bvh* out)
+{
+    VR_TRY_BEGIN
+    VR_CHECK(out, VR_ERR_INVALID_ARGUMENT, "vr_sah_build_device: null out handle");
+    VR_CHECK(n == 0 || d_prims, VR_ERR_INVALID_ARGUMENT, "vr_sah_build_device: null primitives");
+    int rc = ensure_device();
+    if (rc) return rc;
+    vr_bvh h = new vr_bvh_t;
+    h->s.device = current_device();
+    rc = sah_build_device(d_prims, n, max_leaf_size, h->s, (hipStream_t)stream);
+    if (rc == VR_OK) rc = mark_built(h, (hipStream_t)stream);
+    if (rc) {
+        vr_bvh_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return VR_OK;
+    VR_TRY_END
+}
+
 int vr_sah_build(const vr_triangle* h_prims, size_t n, int max_leaf_size, vr_bvh_node* h_nodes, size_t* num_nodes,
                  uint32_t* h_indices)
 {

@@ -128,6 +128,8 @@ int collapse_wide(bvh_state& s, hipStream_t st);
 // (LBVH only: pair e[3] holds the range end) turns inner subtrees of at most
 // leaf_max triangles into one leaf over their contiguous range.
 int collapse_quads(bvh_state& s, uint32_t leaf_max, hipStream_t st);
+// binned SAH build on the device into s (the reference's tree node for node)
+int sah_build_device(const vr_triangle* d_prims, size_t n, int max_leaf_size, bvh_state& s, hipStream_t st);
 // an LBVH's index list (the identity) into d[n]
 int lbvh_indices(uint32_t* d, size_t n, hipStream_t st);
 
