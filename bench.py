@@ -263,6 +263,7 @@ def main():
         extras["nee_frame"] = nee_frame(V, S, tris, mats, kernel, sp, stream, kernel_ms)
         extras["lbvh_rotated"] = rotated_lbvh(V, d_tris, bvh, kernel, sp, stream, kernel_ms)
         extras["wide_nodes"] = wide_nodes(V, bvh, kernel, sp, stream, kernel_ms, d_rays, closest)
+        extras["sah_device"] = sah_device(V, d_tris, bvh, kernel, sp, stream, kernel_ms)
         extras["heightfield_only"] = heightfield_only(args, V, S, stream)
 
     cpu, parity = None, None
@@ -442,6 +443,47 @@ def wide_nodes(V, bvh, kernel, sp, stream, kernel_ms, d_rays, closest, frames=3)
             f"{rec.get('closest_hit_grays_per_s', 0):.3f} Grays/s")
     bvh.set_width(4, stream=stream)
     stream.synchronize()
+    return out
+
+
+def sah_device(V, d_tris, bvh, kernel, sp, stream, kernel_ms, builds=3):
+    """The reference's binned SAH tree built on the device
+    (vr_sah_build_device, node for node binned_sah_builder's): build time,
+    sah_cost against the LBVH's, and the same frame on it."""
+    import time as _t
+    times, sb = [], None
+    for _ in range(builds):
+        if sb is not None:
+            sb.destroy()
+        stream.synchronize()
+        t0 = _t.perf_counter()
+        sb = V.sah_build_device(d_tris, stream=stream)
+        stream.synchronize()
+        times.append((_t.perf_counter() - t0) * 1e3)
+    cost_lbvh, cost_sah = bvh.sah_cost(stream=stream), sb.sah_cost(stream=stream)
+    k = V.pathtracing.kernel(sb, kernel.materials, kernel.params)
+    sched = V.hip_sched()
+    sched.frame(k, sp, stream=stream)
+    stream.synchronize()
+    sched.reset_counters()
+    ms = []
+    for _ in range(3):
+        e0, e1 = V.Event(), V.Event()
+        e0.record(stream)
+        sched.frame(k, sp, stream=stream)
+        e1.record(stream)
+        ms.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    f_ms = float(np.median(ms))
+    sec = sched.rays()[1] / len(ms)
+    info = sb.info()
+    sb.destroy()
+    out = {"build_ms_wall": round(float(np.median(times)), 3), "nodes": info["num_nodes"], "depth": info["depth"],
+           "sah_cost_lbvh": round(cost_lbvh, 3), "sah_cost_sah": round(cost_sah, 3), "frame_ms": round(f_ms, 3),
+           "incoherent_mrays_per_s": round(sec / (f_ms * 1e-3) / 1e6, 3),
+           "frame_speedup_vs_lbvh": round(kernel_ms / f_ms, 4)}
+    log(f"device binned SAH: build {out['build_ms_wall']:.1f} ms wall, SAH {cost_lbvh:.2f} -> {cost_sah:.2f}, "
+        f"frame {f_ms:.2f} ms ({out['incoherent_mrays_per_s']:.1f} Mrays/s, x{out['frame_speedup_vs_lbvh']:.3f})")
     return out
 
 

@@ -41,7 +41,10 @@ constexpr int BIN_WORDS = 13;  // prim lo xyz (min), prim hi xyz (max), cent lo 
 constexpr int NODE_BINS = NUM_BINS * BIN_WORDS;
 constexpr int BLOCK = 256;
 // nodes of at most this many references are split by one workgroup
-constexpr uint32_t LARGE = 2048;
+#ifndef VR_SAH_LARGE
+#define VR_SAH_LARGE 2048
+#endif
+constexpr uint32_t LARGE = VR_SAH_LARGE;
 
 // a reference: the primitive's box and its index (sah.h:181-193)
 struct alignas(32) sref {
@@ -607,97 +610,132 @@ __global__ __launch_bounds__(BLOCK) void k_sah_swap_large(const uint32_t* __rest
 // small nodes: one workgroup per node
 // ---------------------------------------------------------------------------
 
+// the wave's inclusive scan of one bin per lane (lanes 0..15) towards higher
+// lanes (UP) or lower ones: exact minima, maxima and sums
+template <bool UP>
+__device__ inline sbin wave_bin_scan(sbin v, uint32_t lane)
+{
+    for (int d = 1; d < NUM_BINS; d <<= 1) {
+        sbin o;
+        float* of = reinterpret_cast<float*>(&o);
+        const float* vf = reinterpret_cast<const float*>(&v);
+        for (int k = 0; k < 12; ++k) of[k] = UP ? __shfl_up(vf[k], d) : __shfl_down(vf[k], d);
+        o.n = UP ? __shfl_up(v.n, d) : __shfl_down(v.n, d);
+        const bool take = UP ? lane >= (uint32_t)d : lane + d < (uint32_t)NUM_BINS;
+        if (take) merge(v, o);
+    }
+    return v;
+}
+
+__device__ inline sbin shfl_bin(const sbin& v, int src)
+{
+    sbin o;
+    float* of = reinterpret_cast<float*>(&o);
+    const float* vf = reinterpret_cast<const float*>(&v);
+    for (int k = 0; k < 12; ++k) of[k] = __shfl(vf[k], src);
+    o.n = __shfl(v.n, src);
+    return o;
+}
+
+// Small nodes: one wave per node.  Bins in the wave's LDS; the sweep of
+// find_split with one bin per lane (prefix and suffix unions by shuffles,
+// every split cost by its own lane with the reference's expression, the
+// lowest cost winning and the rightmost of equal ones, as the reference's
+// strict < from the right); the partition's f and t references ranked by
+// ballots, in range order.
 __global__ __launch_bounds__(BLOCK) void k_sah_small(const uint32_t* __restrict__ list, uint32_t m,
                                                      sref* __restrict__ refs, snode* __restrict__ nodes,
                                                      uint32_t* __restrict__ node_of, counters* C, uint32_t max_leaf,
                                                      uint32_t* large_next, slot_t* slots_next, uint32_t* small_next)
 {
-    __shared__ uint32_t lb[NODE_BINS];
-    __shared__ uint16_t s_pf[LARGE], s_pt[LARGE];
-    __shared__ uint32_t wf[BLOCK / 64], wt[BLOCK / 64];
-    __shared__ int s_best;
-    __shared__ uint32_t s_pivot, s_nf, s_nt, s_child;
-    __shared__ split_out s_so;
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ uint32_t lbs[WAVES][NODE_BINS];
+    __shared__ uint16_t s_pfs[WAVES][LARGE], s_pts[WAVES][LARGE];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x / 64;
-    for (uint32_t nid = blockIdx.x; nid < m; nid += gridDim.x) {
+    uint32_t* lb = lbs[w];
+    uint16_t* s_pf = s_pfs[w];
+    uint16_t* s_pt = s_pts[w];
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (uint32_t nid = blockIdx.x * WAVES + w; nid < m; nid += gridDim.x * WAVES) {
         const uint32_t node = list[nid];
         const snode nd = nodes[node];
         const f3 clo = mk3(nd.cb[0], nd.cb[1], nd.cb[2]), chi = mk3(nd.cb[3], nd.cb[4], nd.cb[5]);
         const int axis = max_index3(sub3(chi, clo));
         const float k0 = axis3(clo, axis), k1 = (float)NUM_BINS / (axis3(chi, axis) - k0);
-        for (int i = threadIdx.x; i < NODE_BINS; i += BLOCK) bins_identity(lb, i);
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < nd.count; e += BLOCK) {
+        for (int i = lane; i < NODE_BINS; i += 64) bins_identity(lb, i);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (uint32_t e = lane; e < nd.count; e += 64) {
             const sref r = refs[nd.first + e];
             const f3 c = center_of(r);
             bin_add(lb + project(k0, k1, axis, c) * BIN_WORDS, r, c);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const split_out so = sah_sweep(lb, mk3(nd.pb[0], nd.pb[1], nd.pb[2]), mk3(nd.pb[3], nd.pb[4], nd.pb[5]),
-                                           nd.count);
-            s_best = so.best;
-            s_so = so;
-            s_pivot = so.best < 0 ? 0u : nd.first + so.L.n;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // the sweep, one bin per lane
+        sbin b;
+        if (lane < (uint32_t)NUM_BINS) b = read_bin(lb + lane * BIN_WORDS);
+        else b = read_bin(lb);  // any bin: lanes >= 16 take no part
+        const sbin accl = wave_bin_scan<true>(b, lane);
+        const sbin accr = wave_bin_scan<false>(b, lane);
+        const sbin left = shfl_bin(accl, (int)(lane > 0 ? lane - 1 : 0));  // acc_l[i - 1] on lane i
+        const float hsa_parent = safe_hsa(mk3(nd.pb[0], nd.pb[1], nd.pb[2]), mk3(nd.pb[3], nd.pb[4], nd.pb[5]));
+        float cost = 1.0f + (safe_hsa(left.p.lo, left.p.hi) / hsa_parent) * (3.0f * (float)left.n) +
+                     (safe_hsa(accr.p.lo, accr.p.hi) / hsa_parent) * (3.0f * (float)accr.n);
+        // a split must cost less than FLT_MAX (the sweep's start) to count
+        const bool valid = lane >= 1 && lane < (uint32_t)NUM_BINS && cost < FLT_MAX;
+        uint32_t key_lane = lane;
+        float key = valid ? cost : FLT_MAX;
+        for (int d = 32; d >= 1; d >>= 1) {
+            const float ok = __shfl_xor(key, d);
+            const uint32_t ol = __shfl_xor(key_lane, d);
+            if (ok < key || (ok == key && ol > key_lane)) {
+                key = ok;
+                key_lane = ol;
+            }
         }
-        __syncthreads();
-        const int best = s_best;
-        if (best < 0) {
-            __syncthreads();
-            continue;
-        }
+        const int best = key < FLT_MAX && !(key > 3.0f * (float)nd.count) ? (int)key_lane : -1;
+        if (best < 0) continue;  // a leaf: the node keeps its record
+        const sbin L = shfl_bin(accl, best - 1), R = shfl_bin(accr, best);
+        const uint32_t pivot = nd.first + L.n;
         // mark and rank the f and t references in range order
-        uint32_t cf = 0, ct = 0;  // running totals of the earlier rounds
-        for (uint32_t e0 = 0; e0 < nd.count; e0 += BLOCK) {
-            const uint32_t e = e0 + threadIdx.x;
+        uint32_t cf = 0, ct = 0;
+        for (uint32_t e0 = 0; e0 < nd.count; e0 += 64) {
+            const uint32_t e = e0 + lane;
             uint32_t f = 0, t = 0;
             if (e < nd.count) {
-                const bool left = nd.first + e < s_pivot;
+                const bool l = nd.first + e < pivot;
                 const bool pred = project_unsafe(k0, k1, axis, center_of(refs[nd.first + e])) < best;
-                f = left && !pred;
-                t = !left && pred;
+                f = l && !pred;
+                t = !l && pred;
             }
             const uint64_t bf = __ballot(f), bt = __ballot(t);
-            const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-            if (lane == 0) {
-                wf[w] = (uint32_t)__popcll(bf);
-                wt[w] = (uint32_t)__popcll(bt);
-            }
-            __syncthreads();
-            uint32_t of = cf, ot = ct;
-            for (uint32_t q = 0; q < w; ++q) {
-                of += wf[q];
-                ot += wt[q];
-            }
-            if (f) s_pf[of + (uint32_t)__popcll(bf & below)] = (uint16_t)e;
-            if (t) s_pt[ot + (uint32_t)__popcll(bt & below)] = (uint16_t)e;
-            for (int q = 0; q < BLOCK / 64; ++q) {
-                cf += wf[q];
-                ct += wt[q];
-            }
-            __syncthreads();
+            if (f) s_pf[cf + (uint32_t)__popcll(bf & below)] = (uint16_t)e;
+            if (t) s_pt[ct + (uint32_t)__popcll(bt & below)] = (uint16_t)e;
+            cf += (uint32_t)__popcll(bf);
+            ct += (uint32_t)__popcll(bt);
         }
-        for (uint32_t k = threadIdx.x; k < min(cf, ct); k += BLOCK) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = lane; k < min(cf, ct); k += 64) {
             const uint32_t p = nd.first + s_pf[k], q = nd.first + s_pt[ct - 1u - k];
-            const sref a = refs[p], b = refs[q];
-            refs[p] = b;
+            const sref a = refs[p], bb = refs[q];
+            refs[p] = bb;
             refs[q] = a;
         }
-        if (threadIdx.x == 0) {
-            const split_out& so = s_so;
-            const uint32_t c = atomicAdd(&C->nodes, 2u);
-            const uint32_t sign = axis3(so.L.p.lo, axis) < axis3(so.R.p.lo, axis) ? 0u : 1u;
+        uint32_t c = 0;
+        if (lane == 0) {
+            c = atomicAdd(&C->nodes, 2u);
+            const uint32_t sign = axis3(L.p.lo, axis) < axis3(R.p.lo, axis) ? 0u : 1u;
             nodes[node].child = c;
             nodes[node].info = (nd.info & ~0x1FFFFu) | 1u | ((uint32_t)axis << 8) | (sign << 16);
             int32_t slot;
-            make_child(nodes, c, node, so.L, nd.first, max_leaf, C, large_next, slots_next, small_next, slot);
-            make_child(nodes, c + 1u, node, so.R, s_pivot, max_leaf, C, large_next, slots_next, small_next, slot);
-            s_child = c;
+            make_child(nodes, c, node, L, nd.first, max_leaf, C, large_next, slots_next, small_next, slot);
+            make_child(nodes, c + 1u, node, R, pivot, max_leaf, C, large_next, slots_next, small_next, slot);
         }
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < nd.count; e += BLOCK)
-            node_of[nd.first + e] = nd.first + e < s_pivot ? s_child : s_child + 1u;
-        __syncthreads();
+        c = __shfl(c, 0);
+        for (uint32_t e = lane; e < nd.count; e += 64) node_of[nd.first + e] = nd.first + e < pivot ? c : c + 1u;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -928,7 +966,7 @@ int sah_build_device(const vr_triangle* d_prims, size_t n, int max_leaf_size, bv
             std::swap(slot_of, slot_next);
         }
         if (n_small)
-            hipLaunchKernelGGL(k_sah_small, dim3(std::min(n_small, 65535u)), dim3(BLOCK), 0, st, small, n_small, refs,
+            hipLaunchKernelGGL(k_sah_small, dim3(std::min(div_up(n_small, BLOCK / 64), 65535u)), dim3(BLOCK), 0, st, small, n_small, refs,
                                nodes, node_of, C, max_leaf, large_next, slots_next, small_next);
         VR_HIP_CHECK(hipGetLastError());
         VR_HIP_CHECK(hipMemcpyAsync(&h, C, sizeof(h), hipMemcpyDeviceToHost, st));
