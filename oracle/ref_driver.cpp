@@ -29,7 +29,7 @@
 //                                       the instances, closest_hit -> {prim_id, t, u, v, inst_id, geom_id}
 //   wide    NODES NN TRIS NP IDX W OUTNODES OUTCOUNT OUTCOMP
 //                                       bvh_collapser into index_bvh4/8 (W), the wide nodes
-//                                       as stored and bvh_compressed_node<W>::init of each
+//                                       as stored, and bvh_compressor::compress of them
 //   tracew  NODES NN TRIS NP IDX W COMP RAYS NR OUT
 //                                       intersect_ray1_bvhN (COMP 0) or
 //                                       intersect_ray1_bvhN_compressed (COMP 1) closest hits
@@ -62,6 +62,7 @@
 #include <visionaray/traverse.h>
 #include <visionaray/get_normal.h>
 #include <visionaray/detail/bvh/collapse.h>
+#include <visionaray/detail/bvh/compress.h>
 
 // the reference's 8-wide traversal needs its 8-lane SIMD types (AVX)
 #if defined(__AVX__)
@@ -409,10 +410,7 @@ int main(int argc, char** argv)
             using CT = decltype(comp);
             thread_pool pool(1);
             bvh_collapser().collapse(tree, wide, pool);
-            comp.nodes().resize(wide.nodes().size());
-            for (size_t i = 0; i < wide.nodes().size(); ++i) comp.nodes()[i].init(wide.nodes()[i]);
-            comp.primitives() = wide.primitives();
-            comp.indices() = wide.indices();
+            bvh_compressor().compress(wide, comp);
             if (mode == "wide") {
                 save(a[6], wide.nodes().data(), wide.nodes().size());
                 uint64_t count = wide.nodes().size();

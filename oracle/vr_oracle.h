@@ -28,7 +28,7 @@
  *   render_ao/render_pt golden images from the reference tiled_sched path
  *   collapse_wide       the reference's bvh_collapser run on the same binary
  *                       trees (oracle/_ref), node for node, W = 4 and 8
- *   compress_wide       the reference's bvh_compressed_node::init on the same
+ *   compress_wide       the reference's bvh_compressor (bvh_compressed_node::init) on the same
  *                       wide nodes (oracle/_ref), byte for byte on used slots
  */
 #ifndef VR_ORACLE_H

@@ -160,7 +160,8 @@ def instances():
 
 def wide():
     """bvh_collapser into 4- and 8-wide trees (detail/bvh/collapse.h:19-169),
-    bvh_compressed_node::init of every wide node (bvh.h:352-400), and the
+    bvh_compressor::compress (detail/bvh/compress.h, bvh_compressed_node::init,
+    bvh.h:352-400) of the wide nodes, and the
     closest hits of the reference's wide traversals (intersect_ray1_bvhN,
     4-wide compressed too) over the GPU-layout LBVH and the reference's own
     binned-SAH tree of the 344-triangle config-0 scene."""
