@@ -60,6 +60,10 @@ def test_null_arguments_are_rejected(lib):
     assert lib.vr_trace(None, 0, None, 1, None, None) == VR_ERR_INVALID_ARGUMENT
     assert lib.vr_render_pt(None, None, 0, None, None, None, None, None, None, None) == VR_ERR_INVALID_ARGUMENT
     assert lib.vr_camera_look_at(None, None, None, 1.0, 1.0, None) == VR_ERR_INVALID_ARGUMENT
+    # the wide-node and device SAH entry points check before touching a device
+    assert lib.vr_bvh_set_width(None, 8, 0, None) == VR_ERR_INVALID_ARGUMENT
+    assert lib.vr_bvh_download_wide(None, None, 0, None, None) == VR_ERR_INVALID_ARGUMENT
+    assert lib.vr_sah_build_device(None, 10, 4, None, None) == VR_ERR_INVALID_ARGUMENT
 
 
 def test_bench_refuses_more_than_one_gpu():
