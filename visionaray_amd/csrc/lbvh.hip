@@ -425,7 +425,7 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
 constexpr int KWIN = 256;
 
 __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ codes, int n,
-                                                  pair_node* __restrict__ pairs, uint32_t* __restrict__ leaf_parent)
+                                                  uint4* __restrict__ ent, uint32_t* __restrict__ leaf_parent)
 {
     __shared__ uint32_t s_codes[BLOCK + 2 * KWIN];
     const int w0 = (int)(blockIdx.x * BLOCK) - KWIN;
@@ -469,23 +469,25 @@ __global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ c
     const int last = d > 0 ? j : i;
     const int lc = split, rc = split + 1;
     if (lc < 0 || rc >= n) return;  // unreachable for sorted keys; never write out of bounds
-    pair_node& P = pairs[i];
+    // the pair's entries in the dense array (words 0, 1, 3 here; word 2,
+    // the parent link, from the parent's thread)
+    uint32_t* E = reinterpret_cast<uint32_t*>(ent + i);
     if (lc == first) {
-        P.e[0] = leaf_entry((uint32_t)lc, 1u);
+        E[0] = leaf_entry((uint32_t)lc, 1u);
         leaf_parent[lc] = ((uint32_t)i << 1) | 0u;
     } else {
-        P.e[0] = (uint32_t)lc;
-        pairs[lc].e[2] = ((uint32_t)i << 1) | 0u;
+        E[0] = (uint32_t)lc;
+        reinterpret_cast<uint32_t*>(ent + lc)[2] = ((uint32_t)i << 1) | 0u;
     }
     if (rc == last) {
-        P.e[1] = leaf_entry((uint32_t)rc, 1u);
+        E[1] = leaf_entry((uint32_t)rc, 1u);
         leaf_parent[rc] = ((uint32_t)i << 1) | 1u;
     } else {
-        P.e[1] = (uint32_t)rc;
-        pairs[rc].e[2] = ((uint32_t)i << 1) | 1u;
+        E[1] = (uint32_t)rc;
+        reinterpret_cast<uint32_t*>(ent + rc)[2] = ((uint32_t)i << 1) | 1u;
     }
-    if (i == 0) P.e[2] = 0xFFFFFFFFu;
-    P.e[3] = (uint32_t)j;  // the range is [min(i, j), max(i, j)]
+    if (i == 0) E[2] = 0xFFFFFFFFu;
+    E[3] = (uint32_t)j;  // the range is [min(i, j), max(i, j)]
 }
 
 __device__ inline void store_box(float* f, f3 lo, f3 hi)
@@ -581,7 +583,8 @@ struct lds_pairs {
 __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __restrict__ prims, size_t n,
                                                            const int32_t* __restrict__ sorted_ids,
                                                            const uint32_t* __restrict__ leaf_parent,
-                                                           pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
+                                                           pair_node* __restrict__ pairs, const uint4* __restrict__ ent,
+                                                           uint32_t* __restrict__ flags,
                                                            uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
                                                            vr_bvh_node* __restrict__ ref_nodes,
                                                            float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
@@ -599,7 +602,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     const size_t c0 = (size_t)blockIdx.x * CHUNK;
     const size_t c1 = min(n, c0 + (size_t)CHUNK);
     s_flag[tid] = 0;
-    if (c0 + tid + 1 < n) s_ent[tid] = *reinterpret_cast<const uint4*>(pairs[c0 + tid].e);
+    if (c0 + tid + 1 < n) s_ent[tid] = ent[c0 + tid];
     __syncthreads();
     const size_t j = c0 + tid;
     // the leaf and its climb; every thread then meets at the barrier below
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
             const uint32_t p = link >> 1, side = link & 1u;
             if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
             const bool inside = p >= c0 && p < c1;
-            const uint32_t other = inside ? s_ent[p - c0].w : pairs[p].e[3];
+            const uint32_t other = inside ? s_ent[p - c0].w : ent[p].w;
             if (!(inside && other >= c0 && other < c1)) {
                 publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
                 if (atomicAdd(&flags[p], 1u) == 1u) {
@@ -649,10 +652,12 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             float b[12];
             for (int q = 0; q < 12; ++q) b[q] = s_box[k * 12 + q];
+            // the whole 64-byte record at once: both boxes and the entries
             float4* dst = reinterpret_cast<float4*>(pairs[p].f);
             dst[0] = make_float4(b[0], b[1], b[2], b[3]);
             dst[1] = make_float4(b[4], b[5], b[6], b[7]);
             dst[2] = make_float4(b[8], b[9], b[10], b[11]);
+            *reinterpret_cast<uint4*>(pairs[p].e) = s_ent[k];
             lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
             hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
             h = max(s_h[2 * k], s_h[2 * k + 1]) + 1u;
@@ -680,7 +685,8 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
 // the whole tree) the next node leaves the block: that node's arrival is
 // counted for the next pass and, as the second, appended to its ready list.
 __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size_t c1, bool whole,
-                                    pair_node* __restrict__ pairs, uint32_t* __restrict__ flags,
+                                    pair_node* __restrict__ pairs, const uint4* __restrict__ ent,
+                                    uint32_t* __restrict__ flags,
                                     uint32_t* __restrict__ heights, vr_bvh_node* __restrict__ ref_nodes,
                                     float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
                                     uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
@@ -693,6 +699,9 @@ __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size
         const f3 lo = min3v(mk3(q0.x, q0.y, q0.z), mk3(q1.z, q1.w, q2.x));
         const f3 hi = max3v(mk3(q0.w, q1.x, q1.y), mk3(q2.y, q2.z, q2.w));
         const uint32_t h = max(heights[2 * p], heights[2 * p + 1]) + 1u;
+        // the record is complete: both child boxes and now the entries
+        const uint4 ep = ent[p];
+        *reinterpret_cast<uint4*>(pairs[p].e) = ep;
         // p spans chunks (far above the leaf size) and its subtree is
         // complete: its quad after the workgroup's climbs (or now, when the
         // list is full)
@@ -705,12 +714,12 @@ __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size
             finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
             return;
         }
-        const uint32_t link = pairs[p].e[2];
+        const uint32_t link = ep.z;
         const uint32_t pp = link >> 1, side = link & 1u;
         if (pp >= npairs) return;
         publish_child(pairs, heights, ref_nodes, pp, side, lo, hi, h, 1u + 2u * p, 0u);
         if (!whole) {
-            const uint32_t other = pairs[pp].e[3];
+            const uint32_t other = ent[pp].w;
             if (!(pp >= c0 && pp < c1 && other >= c0 && other < c1)) {
                 if (atomicAdd(&flags[pp], 1u) == 1u) next[atomicAdd(next_count, 1u)] = pp;
                 return;
@@ -725,6 +734,7 @@ __device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size
 
 // pass 2: workgroup b finishes the nodes inside leaves [b * SUPER, (b + 1) * SUPER)
 __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, pair_node* __restrict__ pairs,
+                                                             const uint4* __restrict__ ent,
                                                              uint32_t* __restrict__ flags,
                                                              uint32_t* __restrict__ heights,
                                                              vr_bvh_node* __restrict__ ref_nodes,
@@ -744,13 +754,14 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, pair_node
     __syncthreads();
     const uint32_t count = min(bucket_count[blockIdx.x], SUPER);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_global(ready[c0 + i], npairs, c0, c1, false, pairs, flags, heights, ref_nodes, root_box, depth_out,
+        climb_global(ready[c0 + i], npairs, c0, c1, false, pairs, ent, flags, heights, ref_nodes, root_box, depth_out,
                      top, top_count, quads, leaf_max, s_done, &s_ndone);
     quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
 }
 
 // pass 3: one workgroup finishes the nodes that span SUPER blocks
 __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, pair_node* __restrict__ pairs,
+                                                             const uint4* __restrict__ ent,
                                                              uint32_t* __restrict__ flags,
                                                              uint32_t* __restrict__ heights,
                                                              vr_bvh_node* __restrict__ ref_nodes,
@@ -767,7 +778,7 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, pair_node
     __syncthreads();
     const uint32_t count = min(*top_count, npairs);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_global(top[i], npairs, 0, n, true, pairs, flags, heights, ref_nodes, root_box, depth_out, nullptr,
+        climb_global(top[i], npairs, 0, n, true, pairs, ent, flags, heights, ref_nodes, root_box, depth_out, nullptr,
                      nullptr, quads, leaf_max, s_done, &s_ndone);
     quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
 }
@@ -888,14 +899,14 @@ void launch_bottom_up(const bvh_state& bs, const build_scratch& s, const vr_tria
     if (!build)
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
     hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, prims, n, bs.sorted_ids,
-                       s.leaf_parent, bs.pairs, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
+                       s.leaf_parent, bs.pairs, bs.entries, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
                        s.keys0, s.counts, (int)(build && bs.want_rotate), bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)CHUNK)
-        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags,
+        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.entries, bs.flags,
                            bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb,
                            bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)SUPER)
-        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.flags, bs.heights,
+        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.entries, bs.flags, bs.heights,
                            ref_nodes, bs.root_box, depth, s.vals0, s.counts + nb, bs.quads, LBVH_LEAF_MAX);
 }
 
@@ -938,7 +949,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         // caller's triangles (valid until its work on the stream is done), a
         // refit the new ones, and an LBVH's index list is the identity
         const size_t sizes[] = {np1 * sizeof(pair_node), np1 * sizeof(quad_node), n * sizeof(tri_node),
-                                0, n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
+                                np1 * sizeof(uint4), n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
                                 0, bs.num_ref_nodes * sizeof(vr_bvh_node), plan.bytes};
         size_t total = 0;
         for (size_t b : sizes) total += align_up(b, 256);
@@ -951,7 +962,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         carve(bs.pairs, sizes[0]);
         carve(bs.quads, sizes[1]);
         carve(bs.tris, sizes[2]);
-        carve(bs.prims, sizes[3]);
+        carve(bs.entries, sizes[3]);
         carve(bs.sorted_codes, sizes[4]);
         carve(bs.sorted_ids, sizes[5]);
         carve(bs.flags, sizes[6]);
@@ -996,7 +1007,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     // 4: hierarchy
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
-                           bs.pairs, s.leaf_parent);
+                           bs.entries, s.leaf_parent);
     // 5: boxes, traversal-order triangles, reference layout, index list,
     // and the 4-wide traversal nodes (each pair's quad as soon as its
     // subtree is complete)

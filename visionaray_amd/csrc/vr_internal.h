@@ -68,6 +68,7 @@ struct bvh_state {
     uint32_t max_leaf = 1;
     bool is_lbvh = false;
     pair_node* pairs = nullptr;      // num_pairs
+    uint4* entries = nullptr;        // LBVH: each pair's entries (children, parent link, range end), dense
     quad_node* quads = nullptr;      // traversal nodes, one per pair (only those below quad 0 are reached)
     uint32_t collapse_leaf_max = 0;  // leaf size the quads were collapsed with
     tri_node* tris = nullptr;        // num_prims, traversal order
