@@ -36,6 +36,15 @@ constexpr int WAVES = BLOCK / 64;
 constexpr int RADIX = 256;
 constexpr int ITEMS = 8;
 constexpr int TILE = BLOCK * ITEMS;  // keys per sort tile
+// leaves per workgroup of the first bottom-up pass
+#ifndef VR_LBVH_CHUNK
+#define VR_LBVH_CHUNK 1024
+#endif
+constexpr int CHUNK = VR_LBVH_CHUNK;
+// split slots: no child has arrived / both have (LDS only); otherwise the
+// first arrival's outer bound (< n)
+constexpr uint32_t SLOT_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t SLOT_DONE = 0xFFFFFFFEu;
 
 // float <-> order-preserving unsigned (for integer atomic min/max)
 __device__ inline uint32_t f2o(float f)
@@ -96,14 +105,15 @@ __device__ inline f3 tri_centroid(const tri_in& t)
     return muls3(add3(hi, lo), 0.5f);
 }
 
-// The primitive copy (index_bvh_t(P*, n), bvh.h:891-896) and the centroid
-// box in one pass over the triangles: a block stages CB_BLOCK triangles
-// (11 words each) in LDS with 16-byte loads when the source is 16-byte
-// aligned, writes them to the copy with 16-byte stores and reduces their
-// centroids; every block leaves its six extremes in `part` (no contended
-// atomics), and k_bounds_final folds them.  min / max are exact, so any
-// reduction order gives the same box.
+// The centroid box in one pass over the triangles (the build reads the
+// caller's array; index_bvh_t's copy, bvh.h:891-896, is not made): a block
+// stages CB_U * CB_BLOCK triangles (11 words each) in LDS, all of a thread's
+// 16-byte loads issued before the first store when the source is 16-byte
+// aligned, and reduces their centroids; every block leaves its six extremes
+// in `part` (no contended atomics), and k_bounds_final folds them.  min / max
+// are exact, so any reduction order gives the same box.
 constexpr int CB_BLOCK = 256;
+constexpr int CB_U = 4;
 constexpr int CB_GRID = 2048;
 constexpr int TRI_WORDS = 11;
 
@@ -126,37 +136,39 @@ __device__ inline void block_minmax6(float v[6], float* __restrict__ out)
     }
 }
 
-__global__ __launch_bounds__(CB_BLOCK) void k_copy_bounds(const vr_triangle* __restrict__ src, size_t n,
-                                                          vr_triangle* __restrict__ dst, int vec,
-                                                          float* __restrict__ part)
+__global__ __launch_bounds__(CB_BLOCK) void k_centroid_bounds(const vr_triangle* __restrict__ src, size_t n,
+                                                              int vec, float* __restrict__ part)
 {
-    __shared__ float s_tri[CB_BLOCK * TRI_WORDS];
+    constexpr uint32_t GROUP = CB_U * CB_BLOCK;            // triangles per step
+    constexpr uint32_t VEC = GROUP * TRI_WORDS / 4;        // their 16-byte words
+    constexpr uint32_t PER_THREAD = VEC / CB_BLOCK;        // 11: VEC is a multiple of CB_BLOCK
+    __shared__ float s_tri[GROUP * TRI_WORDS];
     float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-    const size_t chunks = (n + CB_BLOCK - 1) / CB_BLOCK;
-    for (size_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const size_t t0 = c * CB_BLOCK;
-        const uint32_t cnt = (uint32_t)min((size_t)CB_BLOCK, n - t0);
+    const uint32_t tid = threadIdx.x;
+    const size_t groups = (n + GROUP - 1) / GROUP;
+    for (size_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const size_t t0 = g * GROUP;
+        const uint32_t cnt = (uint32_t)min((size_t)GROUP, n - t0);
         const float* sp = reinterpret_cast<const float*>(src + t0);
-        float* dp = dst ? reinterpret_cast<float*>(dst + t0) : nullptr;
-        if (vec && cnt == CB_BLOCK) {
-            // 256 triangles = 704 16-byte words; the chunk starts on 16 bytes
-            for (uint32_t q = threadIdx.x; q < CB_BLOCK * TRI_WORDS / 4; q += CB_BLOCK) {
-                const float4 x = reinterpret_cast<const float4*>(sp)[q];
-                if (dp) reinterpret_cast<float4*>(dp)[q] = x;
-                reinterpret_cast<float4*>(s_tri)[q] = x;
-            }
+        if (vec && cnt == GROUP) {
+            // a whole group starts on 16 bytes (GROUP * 44 is a multiple of 16)
+            float4 x[PER_THREAD];
+#pragma unroll
+            for (uint32_t q = 0; q < PER_THREAD; ++q) x[q] = reinterpret_cast<const float4*>(sp)[tid + q * CB_BLOCK];
+#pragma unroll
+            for (uint32_t q = 0; q < PER_THREAD; ++q) reinterpret_cast<float4*>(s_tri)[tid + q * CB_BLOCK] = x[q];
         } else {
-            for (uint32_t q = threadIdx.x; q < cnt * TRI_WORDS; q += CB_BLOCK) {
-                const float x = sp[q];
-                if (dp) dp[q] = x;
-                s_tri[q] = x;
-            }
+            for (uint32_t q = tid; q < cnt * TRI_WORDS; q += CB_BLOCK) s_tri[q] = sp[q];
         }
         __syncthreads();
-        if (threadIdx.x < cnt) {
-            const f3 m = tri_centroid(tri_from_words(s_tri + threadIdx.x * TRI_WORDS));
-            v[0] = fminf(v[0], m.x); v[1] = fminf(v[1], m.y); v[2] = fminf(v[2], m.z);
-            v[3] = fmaxf(v[3], m.x); v[4] = fmaxf(v[4], m.y); v[5] = fmaxf(v[5], m.z);
+#pragma unroll
+        for (uint32_t u = 0; u < CB_U; ++u) {
+            const uint32_t t = u * CB_BLOCK + tid;
+            if (t < cnt) {
+                const f3 m = tri_centroid(tri_from_words(s_tri + t * TRI_WORDS));
+                v[0] = fminf(v[0], m.x); v[1] = fminf(v[1], m.y); v[2] = fminf(v[2], m.z);
+                v[3] = fmaxf(v[3], m.x); v[4] = fmaxf(v[4], m.y); v[5] = fmaxf(v[5], m.z);
+            }
         }
         __syncthreads();
     }
@@ -277,11 +289,21 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     for (int q = 0; q < WAVES; ++q) s_wcnt[q][tid] = 0;
     __syncthreads();
 
+    // every key and value of the tile in flight at once, before the barriers
+    uint32_t keys[ITEMS], vals[ITEMS];
+#pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const size_t idx = base + (size_t)k * BLOCK + tid;
         const bool valid = idx < n;
-        const uint32_t key = valid ? keys_in[idx] : 0u;
-        const uint32_t val = valid ? (first_pass ? (uint32_t)idx : vals_in[idx]) : 0u;
+        keys[k] = valid ? keys_in[idx] : 0u;
+        vals[k] = valid ? (first_pass ? (uint32_t)idx : vals_in[idx]) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const size_t idx = base + (size_t)k * BLOCK + tid;
+        const bool valid = idx < n;
+        const uint32_t key = keys[k];
+        const uint32_t val = vals[k];
         const uint32_t d = (key >> shift) & 0xFFu;
         const uint64_t m = match_digit(d, valid);
         const uint32_t rank = mbcnt64(m);
@@ -352,12 +374,12 @@ __global__ __launch_bounds__(BLOCK) void k_radix_spine(const uint32_t* __restric
 // One block per sort tile (TILE keys), staged through LDS in BLOCK-triangle
 // chunks (16-byte loads when the triangles are 16-byte aligned).  In a build
 // the same kernel leaves the first radix pass's per-tile digit counts and
-// clears the hierarchy's link and counter arrays.
+// clears the bottom-up pass's ready-list counters and the split slots on the
+// chunks' edges.
 struct morton_extras {
     uint32_t* hist;         // digit-0 counts per tile (first radix pass), or null
     uint32_t tiles;
-    uint32_t* leaf_parent;  // n words set to ~0 (no parent yet), or null
-    uint32_t* flags;        // n - 1 words cleared
+    uint32_t* edges;        // split slots: those of splits CHUNK * k - 1 set empty, or null
     uint32_t* counts;       // ncounts words cleared
     uint32_t ncounts;
 };
@@ -399,10 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
             code = separate_bits3((uint32_t)c.x) | (separate_bits3((uint32_t)c.y) << 1) |
                    (separate_bits3((uint32_t)c.z) << 2);
             codes[i] = code;
-            if (x.leaf_parent) {
-                x.leaf_parent[i] = 0xFFFFFFFFu;
-                if (i + 1 < n) x.flags[i] = 0u;
-            }
+            if (x.edges && i % CHUNK == CHUNK - 1 && i + 1 < n) x.edges[i] = SLOT_EMPTY;
         }
         if (x.hist) {
             const uint32_t d = code & 0xFFu;
@@ -416,78 +435,71 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Karras hierarchy (detail/bvh/lbvh.h:120-185, 281-330)
+// Hierarchy and boxes, bottom-up (detail/bvh/lbvh.h:120-185, 281-372)
 // ---------------------------------------------------------------------------
+//
+// The binary radix tree over the sorted keys, with equal codes ordered by
+// index (the reference's delta: common-prefix length of the codes, or 32 +
+// that of the indices when the codes are equal), is unique.  The reference
+// finds each node's range and split top-down (determine_range and a binary
+// search, one thread per node), then climbs from the leaves with float
+// atomics for the boxes.  Here one climb from the leaves does both: a node
+// covering [l, r] has its parent on the side where its boundary keys share
+// the longer prefix, delta(l - 1, l) against delta(r, r + 1) (the two are
+// never equal for a node of the tree); the parent's split is then r (this
+// node is its left child) or l - 1 (its right child).  The first child to
+// reach a split leaves its box, height and outer bound there; the second
+// completes the parent.  The reference's node numbering follows from the
+// side alone: a left child is node `split`, which is its own r, a right child
+// node split + 1, its own l, and the root node 0, so every record, entry and
+// box equals the top-down construction's.
+//
+// Every box is a min / max union computed once, so the boxes are the
+// reference's exactly.  Three passes, each finishing the nodes inside blocks
+// of a growing size with workgroup-scope ordering only (on gfx950 a
+// device-scope fence writes back the XCD's L2, far too slow to pay per node):
+//   1. k_bottom_up_local: one workgroup per CHUNK leaves; the splits whose
+//      two children both lie in the chunk meet in LDS;
+//   2. k_bottom_up_mid: one workgroup per SUPER leaves, in global memory;
+//   3. k_bottom_up_top: one workgroup for the rest.
+// A climb stops at a split outside its pass's block: it leaves its data in
+// the split's global stage and exchanges its bound into the split's global
+// slot; the second arrival there appends the split to the next pass's ready
+// list (bucketed by the block the split is inside of).  A node's parent link
+// (entry word 2) is known only when its parent's own side is: pass 1 sets it
+// in LDS before writing the chunk's records, a later pass writes it into
+// the child's record.  Kernel boundaries make every earlier write visible to
+// the next pass.
+constexpr int TOP_BLOCK = 1024;
+// subtrees of at most this many triangles become one traversal leaf
+#ifndef VR_LBVH_LEAF_MAX
+#define VR_LBVH_LEAF_MAX 2
+#endif
+constexpr uint32_t LBVH_LEAF_MAX = VR_LBVH_LEAF_MAX;
+constexpr uint32_t SUPER = 64u * CHUNK;
+// pairs completed by one workgroup of the mid or top pass whose quads are
+// made after its climbs, one per thread
+constexpr uint32_t DONE_CAP = 4096;
 
-// Karras' delta over a block's window of the sorted keys: most ranges and
-// splits are short, so nearly every probe is an LDS read and only the long
-// ranges of the upper levels go to global memory
-constexpr int KWIN = 256;
+// One split's hand-over in global memory: each child's box, height and
+// outer bound (the left child's l, the right child's r) on its side.
+struct alignas(64) split_stage {
+    float box[12];
+    uint32_t h[2];
+    uint32_t bound[2];
+};
+static_assert(sizeof(split_stage) == 64, "split_stage is one 64-byte record");
 
-__global__ __launch_bounds__(BLOCK) void k_karras(const uint32_t* __restrict__ codes, int n,
-                                                  uint4* __restrict__ ent, uint32_t* __restrict__ leaf_parent)
+// common-prefix length of the sorted keys k and k + 1 (codes a and b)
+__device__ inline int key_delta(uint32_t a, uint32_t b, uint32_t k)
 {
-    __shared__ uint32_t s_codes[BLOCK + 2 * KWIN];
-    const int w0 = (int)(blockIdx.x * BLOCK) - KWIN;
-    for (int t = threadIdx.x; t < BLOCK + 2 * KWIN; t += BLOCK) {
-        const int g = w0 + t;
-        s_codes[t] = g >= 0 && g < n ? codes[g] : 0u;
-    }
-    __syncthreads();
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n - 1) return;
-    const uint32_t ki = s_codes[i - w0];
-    auto kdelta = [&](int j) -> int {
-        if (j < 0 || j >= n) return -1;
-        const int t = j - w0;
-        const uint32_t x = ki ^ (t >= 0 && t < BLOCK + 2 * KWIN ? s_codes[t] : codes[j]);
-        if (x == 0) return (int)__clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
-        return (int)__clz((int)x);
-    };
-    const int d = kdelta(i + 1) >= kdelta(i - 1) ? 1 : -1;
-    const int dmin = kdelta(i - d);
-    int lmax = 2;
-    while (kdelta(i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (kdelta(i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = kdelta(j);
-    // split search with steps t = ceil(l / 2^k): the reference evaluates
-    // ceil(l / divf) in float (exact for l < 2^24) and keeps probing with
-    // t = 1 afterwards; those extra probes cannot move s (the prefix length
-    // is monotone in the distance from i), so stopping after the first t = 1
-    // gives the same split.
-    int s = 0;
-    for (int k = 1;; ++k) {
-        const int t = (l + (1 << k) - 1) >> k;
-        if (kdelta(i + (s + t) * d) > dnode) s += t;
-        if (t <= 1) break;
-    }
-    const int split = i + s * d + (d < 0 ? d : 0);
-    const int first = d > 0 ? i : j;
-    const int last = d > 0 ? j : i;
-    const int lc = split, rc = split + 1;
-    if (lc < 0 || rc >= n) return;  // unreachable for sorted keys; never write out of bounds
-    // the pair's entries in the dense array (words 0, 1, 3 here; word 2,
-    // the parent link, from the parent's thread)
-    uint32_t* E = reinterpret_cast<uint32_t*>(ent + i);
-    if (lc == first) {
-        E[0] = leaf_entry((uint32_t)lc, 1u);
-        leaf_parent[lc] = ((uint32_t)i << 1) | 0u;
-    } else {
-        E[0] = (uint32_t)lc;
-        reinterpret_cast<uint32_t*>(ent + lc)[2] = ((uint32_t)i << 1) | 0u;
-    }
-    if (rc == last) {
-        E[1] = leaf_entry((uint32_t)rc, 1u);
-        leaf_parent[rc] = ((uint32_t)i << 1) | 1u;
-    } else {
-        E[1] = (uint32_t)rc;
-        reinterpret_cast<uint32_t*>(ent + rc)[2] = ((uint32_t)i << 1) | 1u;
-    }
-    if (i == 0) E[2] = 0xFFFFFFFFu;
-    E[3] = (uint32_t)j;  // the range is [min(i, j), max(i, j)]
+    const uint32_t x = a ^ b;
+    return x ? (int)__clz((int)x) : 32 + (int)__clz((int)(k ^ (k + 1u)));
+}
+
+__device__ inline uint32_t entry_index(uint32_t e)
+{
+    return e & LEAF_BIT ? e & 0x03FFFFFFu : e;
 }
 
 __device__ inline void store_box(float* f, f3 lo, f3 hi)
@@ -504,58 +516,29 @@ __device__ inline void store_ref_node(vr_bvh_node* o, f3 lo, f3 hi, uint32_t fir
     q[1] = make_uint4(__float_as_uint(hi.y), __float_as_uint(hi.z), first, num);
 }
 
-// Bottom-up boxes.  The second thread to reach a node sees both child boxes,
-// stores them in the node's traversal record and climbs with their union, so
-// every box is computed exactly once.
-//
-// A Karras node covers a leaf range that has its own index as one end, so a
-// node whose range lies inside an aligned block of leaves has its index in
-// that block too.  Three passes, each finishing the nodes inside blocks of a
-// growing size with workgroup-scope ordering only (on gfx950 a device-scope
-// fence writes back the XCD's L2, far too slow to pay per node):
-//   1. k_bottom_up_local: one workgroup per CHUNK leaves, counters and child
-//      boxes in LDS;
-//   2. k_bottom_up_mid: one workgroup per SUPER leaves, in global memory;
-//   3. k_bottom_up_top: one workgroup for the rest.
-// A thread stops at the first node whose range leaves its pass's block: it
-// stores its box in that node's global record and counts the arrival; the
-// second arrival appends the node to the next pass's ready list, bucketed by
-// the block of the node's index.  Kernel boundaries make every earlier write
-// visible to the next pass.
-#ifndef VR_LBVH_CHUNK
-#define VR_LBVH_CHUNK 1024
-#endif
-constexpr int CHUNK = VR_LBVH_CHUNK;
-constexpr int TOP_BLOCK = 1024;
-// subtrees of at most this many triangles become one traversal leaf
-#ifndef VR_LBVH_LEAF_MAX
-#define VR_LBVH_LEAF_MAX 2
-#endif
-constexpr uint32_t LBVH_LEAF_MAX = VR_LBVH_LEAF_MAX;
-constexpr uint32_t SUPER = 64u * CHUNK;
-// pairs completed by one workgroup of the mid or top pass whose quads are
-// made after its climbs, one per thread
-constexpr uint32_t DONE_CAP = 4096;
-
-// quads of the pairs a workgroup completed, in parallel after its climbs
-__device__ inline void quads_after_climbs(const pair_node* __restrict__ pairs, uint32_t npairs, uint32_t leaf_max,
-                                          quad_node* __restrict__ quads, const uint32_t* s_done,
-                                          const uint32_t* s_ndone)
+// the reference-layout nodes of pair p's two children (slots 1 + 2p + side):
+// an inner child c points at its own children (1 + 2c), a leaf at its
+// primitive's original index
+__device__ inline void store_ref_children(vr_bvh_node* ref_nodes, const int32_t* __restrict__ sorted_ids, uint32_t p,
+                                          const float* b, uint32_t e0, uint32_t e1)
 {
-    __syncthreads();
-    if (!quads) return;
-    const uint32_t count = min(*s_ndone, DONE_CAP);
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
-        make_quad<true>(global_pairs{pairs}, s_done[i], npairs, leaf_max, quads);
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t e = side ? e1 : e0;
+        const float* c = b + 6 * side;
+        const bool leaf = e & LEAF_BIT;
+        store_ref_node(&ref_nodes[1 + 2 * p + side], mk3(c[0], c[1], c[2]), mk3(c[3], c[4], c[5]),
+                       leaf ? (uint32_t)sorted_ids[entry_index(e)] : 1u + 2u * e, leaf ? 1u : 0u);
+    }
 }
 
-__device__ inline void publish_child(pair_node* pairs, uint32_t* heights, vr_bvh_node* ref_nodes, uint32_t p,
-                                     uint32_t side, f3 lo, f3 hi, uint32_t h, uint32_t ref_first,
-                                     uint32_t ref_num)
+// one pair record: child boxes b[12] and entries
+__device__ inline void store_pair(pair_node* __restrict__ pairs, uint32_t p, const float* b, uint4 e)
 {
-    store_box(&pairs[p].f[side * 6], lo, hi);
-    heights[2 * p + side] = h;
-    if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
+    float4* dst = reinterpret_cast<float4*>(pairs[p].f);
+    dst[0] = make_float4(b[0], b[1], b[2], b[3]);
+    dst[1] = make_float4(b[4], b[5], b[6], b[7]);
+    dst[2] = make_float4(b[8], b[9], b[10], b[11]);
+    *reinterpret_cast<uint4*>(pairs[p].e) = e;
 }
 
 __device__ inline void finish_root(float* root_box, vr_bvh_node* ref_nodes, uint32_t* depth_out, f3 lo, f3 hi,
@@ -566,51 +549,75 @@ __device__ inline void finish_root(float* root_box, vr_bvh_node* ref_nodes, uint
     *depth_out = h;
 }
 
-// the block's pair records in LDS: child boxes (12 floats) and entries
-struct lds_pairs {
-    const float* box_;
+// the split's node is complete (both children arrived) and goes to the pass
+// that owns the block the split is inside of: its SUPER block's list, or the
+// top list for a split on the edge of two SUPER blocks
+__device__ inline void push_ready(uint32_t s, uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count,
+                                  uint32_t* __restrict__ top, uint32_t* __restrict__ top_count)
+{
+    const uint32_t b = s / SUPER;
+    if ((s + 1u) / SUPER == b)
+        ready[(size_t)b * SUPER + atomicAdd(&bucket_count[b], 1u)] = s;
+    else
+        top[atomicAdd(top_count, 1u)] = s;
+}
+
+// the chunk's pair records in LDS: entries by pair index, child boxes by the
+// pair's split (the index in its first entry)
+struct lds_split_pairs {
+    const float* stage_;
     const uint4* ent_;
     uint32_t c0;
     __device__ uint4 ent(uint32_t q) const { return ent_[q - c0]; }
     __device__ void box(uint32_t loc, f3& lo, f3& hi) const
     {
-        const float* b = box_ + (size_t)((loc >> 1) - c0) * 12 + 6 * (loc & 1u);
+        const uint32_t s = entry_index(ent_[(loc >> 1) - c0].x);
+        const float* b = stage_ + (size_t)(s - c0) * 12 + 6 * (loc & 1u);
         lo = mk3(b[0], b[1], b[2]);
         hi = mk3(b[3], b[4], b[5]);
     }
 };
 
 __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __restrict__ prims, size_t n,
+                                                           const uint32_t* __restrict__ codes,
                                                            const int32_t* __restrict__ sorted_ids,
-                                                           const uint32_t* __restrict__ leaf_parent,
-                                                           pair_node* __restrict__ pairs, const uint4* __restrict__ ent,
-                                                           uint32_t* __restrict__ flags,
+                                                           pair_node* __restrict__ pairs,
+                                                           split_stage* __restrict__ gstage,
+                                                           uint32_t* __restrict__ gslot,
                                                            uint32_t* __restrict__ heights, tri_node* __restrict__ tris,
                                                            vr_bvh_node* __restrict__ ref_nodes,
                                                            float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
                                                            uint32_t* __restrict__ ready, uint32_t* __restrict__ bucket_count,
+                                                           uint32_t* __restrict__ top, uint32_t* __restrict__ top_count,
                                                            int slot_heights, quad_node* __restrict__ quads,
                                                            uint32_t leaf_max)
 {
-    __shared__ float s_box[CHUNK * 12];
-    __shared__ uint32_t s_h[CHUNK * 2];
-    __shared__ uint32_t s_flag[CHUNK];
-    // the block's pair entries (children, parent link, range end): the climb
-    // and the quads of the block's pairs read no global records
+    // by split - c0: child boxes, heights, slot
+    __shared__ float s_stage[CHUNK * 12];
+    __shared__ uint8_t s_h[CHUNK * 2];
+    __shared__ uint32_t s_slot[CHUNK];
+    // by pair index - c0: entries of the pairs completed here (w = SLOT_EMPTY:
+    // not completed here)
     __shared__ uint4 s_ent[CHUNK];
+    // keys c0 - 1 .. c1
+    __shared__ uint32_t s_code[CHUNK + 2];
     const uint32_t tid = threadIdx.x;
-    const size_t c0 = (size_t)blockIdx.x * CHUNK;
-    const size_t c1 = min(n, c0 + (size_t)CHUNK);
-    s_flag[tid] = 0;
-    if (c0 + tid + 1 < n) s_ent[tid] = ent[c0 + tid];
+    const uint32_t N = (uint32_t)n;
+    const uint32_t c0 = blockIdx.x * (uint32_t)CHUNK;
+    const uint32_t c1 = min(N, c0 + (uint32_t)CHUNK);
+    s_slot[tid] = SLOT_EMPTY;
+    s_ent[tid] = make_uint4(0u, 0u, SLOT_EMPTY, SLOT_EMPTY);
+    for (uint32_t t = tid; t < CHUNK + 2; t += CHUNK) {
+        const int64_t g = (int64_t)c0 - 1 + t;
+        if (g >= 0 && g < (int64_t)N) s_code[t] = codes[g];
+    }
     __syncthreads();
-    const size_t j = c0 + tid;
+    // keys k and k + 1 for c0 - 1 <= k < c1
+    auto delta = [&](uint32_t k) { return key_delta(s_code[k + 1 - c0], s_code[k + 2 - c0], k); };
+    const uint32_t j = c0 + tid;
     // the leaf and its climb; every thread then meets at the barrier below
     auto leaf_and_climb = [&]() {
-        // both per-leaf words before the triangle gather and its stores: a
-        // wait for a load issued after the stores would wait for the stores too
         const int32_t id = sorted_ids[j];
-        uint32_t link = n > 1 ? leaf_parent[j] : 0u;
         const tri_in t = load_tri(prims, (size_t)id);
         tri_node tn;
         tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
@@ -619,174 +626,337 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
         tris[j] = tn;
         f3 lo, hi;
         tri_box(t, lo, hi);
-        if (n == 1) {
+        if (N == 1) {
             store_box(root_box, lo, hi);
             if (ref_nodes) store_ref_node(&ref_nodes[0], lo, hi, (uint32_t)id, 1u);
             *depth_out = 1;
             return;
         }
-        const uint32_t npairs = (uint32_t)(n - 1);
-        uint32_t h = 1, ref_first = (uint32_t)id, ref_num = 1;
-        // a Karras tree has at most 65 levels; the bound only guards against
-        // a corrupt hierarchy
+        uint32_t l = j, r = j, h = 1, split = 0, e0 = 0, e1 = 0;
+        bool inner = false;
+        // a radix tree over n < 2^26 keys of 30 bits has fewer than 64 levels;
+        // the bound only guards against a corrupt key array
         for (int level = 0; level < 128; ++level) {
-            const uint32_t p = link >> 1, side = link & 1u;
-            if (p >= npairs) return;  // unset link: corrupt hierarchy, never write out of bounds
-            const bool inside = p >= c0 && p < c1;
-            const uint32_t other = inside ? s_ent[p - c0].w : ent[p].w;
-            if (!(inside && other >= c0 && other < c1)) {
-                publish_child(pairs, heights, ref_nodes, p, side, lo, hi, h, ref_first, ref_num);
-                if (atomicAdd(&flags[p], 1u) == 1u) {
-                    const uint32_t b = p / SUPER;
-                    ready[(size_t)b * SUPER + atomicAdd(&bucket_count[b], 1u)] = p;
-                }
+            const bool root = l == 0 && r == N - 1;
+            bool right = false;
+            if (!root) {
+                const int dl = l > 0 ? delta(l - 1) : -1;
+                const int dr = r + 1 < N ? delta(r) : -1;
+                right = dr > dl;  // parent's split r: this node is its left child
+            }
+            if (inner) {
+                // this node's number, its entries, and its children's links
+                const uint32_t p = root ? 0u : (right ? r : l);
+                s_ent[p - c0] = make_uint4(e0, e1, SLOT_EMPTY, p == l ? r : l);
+                if (!(e0 & LEAF_BIT)) s_ent[split - c0].z = (p << 1) | 0u;
+                if (!(e1 & LEAF_BIT)) s_ent[split + 1 - c0].z = (p << 1) | 1u;
+            }
+            if (root) {
+                finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
                 return;
             }
-            const uint32_t k = p - (uint32_t)c0;
-            if (ref_nodes) store_ref_node(&ref_nodes[1 + 2 * p + side], lo, hi, ref_first, ref_num);
-            store_box(&s_box[k * 12 + side * 6], lo, hi);
-            s_h[2 * k + side] = h;
-            if (slot_heights) heights[2 * p + side] = h;  // every child slot's height, for the rotation pass
+            const uint32_t s = right ? r : l - 1;
+            const uint32_t side = right ? 0u : 1u;
+            const uint32_t bound = right ? l : r;
+            if (s < c0 || s + 2 > c1) {
+                // the parent spans the chunk's edge: hand over in global memory
+                split_stage* g = gstage + s;
+                store_box(g->box + 6 * side, lo, hi);
+                g->h[side] = h;
+                g->bound[side] = bound;
+                if (__hip_atomic_exchange(&gslot[s], bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                    SLOT_EMPTY)
+                    push_ready(s, ready, bucket_count, top, top_count);
+                return;
+            }
+            const uint32_t k = s - c0;
+            store_box(&s_stage[k * 12 + side * 6], lo, hi);
+            s_h[2 * k + side] = (uint8_t)h;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (atomicAdd(&s_flag[k], 1u) == 0u) return;
+            const uint32_t other = atomicExch(&s_slot[k], bound);
+            if (other == SLOT_EMPTY) return;  // first: the sibling's thread completes the parent
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            s_slot[k] = SLOT_DONE;
+            if (right) r = other;
+            else l = other;
             float b[12];
-            for (int q = 0; q < 12; ++q) b[q] = s_box[k * 12 + q];
-            // the whole 64-byte record at once: both boxes and the entries
+            for (int q = 0; q < 12; ++q) b[q] = s_stage[k * 12 + q];
+            lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
+            hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
+            h = max((uint32_t)s_h[2 * k], (uint32_t)s_h[2 * k + 1]) + 1u;
+            split = s;
+            e0 = l == s ? leaf_entry(s, 1u) : s;
+            e1 = s + 1 == r ? leaf_entry(s + 1, 1u) : s + 1;
+            inner = true;
+        }
+    };
+    if (j < N) leaf_and_climb();
+    __syncthreads();
+    if (N < 2) return;
+    // split c0 + tid inside the chunk: a lone arrival goes to global memory
+    // for the pass that completes its sibling; the slot is set either way
+    if (tid + 2 <= c1 - c0) {
+        const uint32_t s = c0 + tid, v = s_slot[tid];
+        uint32_t g = SLOT_EMPTY;
+        if (v != SLOT_EMPTY && v != SLOT_DONE) {
+            const uint32_t side = v > s ? 1u : 0u;  // a right child's bound is its r > s
+            split_stage* st = gstage + s;
+            for (int q = 0; q < 6; ++q) st->box[6 * side + q] = s_stage[tid * 12 + 6 * side + q];
+            st->h[side] = s_h[2 * tid + side];
+            st->bound[side] = v;
+            g = v;
+        }
+        gslot[s] = g;
+    }
+    // the records of the pairs completed here: 16-byte word f of the
+    // chunk's records from thread f mod CHUNK, so each store instruction
+    // writes a contiguous kilobyte per wave
+    for (uint32_t f = tid; f < 4u * CHUNK; f += CHUNK) {
+        const uint32_t qq = f >> 2, part = f & 3u;
+        if (c0 + qq + 1 >= N) break;
+        const uint4 e = s_ent[qq];
+        if (e.w == SLOT_EMPTY) continue;
+        uint4 v = e;
+        if (part < 3) {
+            const float* b = s_stage + (entry_index(e.x) - c0) * 12 + 4 * part;
+            v = make_uint4(__float_as_uint(b[0]), __float_as_uint(b[1]), __float_as_uint(b[2]),
+                           __float_as_uint(b[3]));
+        }
+        reinterpret_cast<uint4*>(pairs)[(size_t)(c0 + qq) * 4 + part] = v;
+    }
+    // then per pair: slot heights, reference nodes, and the quads (every
+    // pair below a completed pair is in the chunk)
+    const uint32_t q = j;
+    if (q + 1 >= N) return;
+    const uint4 e = s_ent[tid];
+    if (e.w == SLOT_EMPTY) return;
+    const uint32_t k = entry_index(e.x) - c0;
+    if (slot_heights) {
+        heights[2 * q] = s_h[2 * k];
+        heights[2 * q + 1] = s_h[2 * k + 1];
+    }
+    if (ref_nodes) store_ref_children(ref_nodes, sorted_ids, q, s_stage + k * 12, e.x, e.y);
+    if (quads && pair_gets_quad(q, e.w, leaf_max))
+        make_quad<true>(lds_split_pairs{s_stage, s_ent, c0}, q, N - 1, leaf_max, quads);
+}
+
+// quads of the pairs a workgroup completed (the first DONE_CAP listed in
+// LDS, the rest in its global overflow list), in parallel after its climbs:
+// the barrier orders every record the workgroup wrote before them
+__device__ inline void quads_after_climbs(const pair_node* __restrict__ pairs, uint32_t npairs, uint32_t leaf_max,
+                                          quad_node* __restrict__ quads, const uint32_t* s_done,
+                                          const uint32_t* s_ndone, const uint32_t* ovf)
+{
+    __syncthreads();
+    if (!quads) return;
+    const uint32_t count = *s_ndone;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
+        make_quad<true>(global_pairs{pairs}, i < DONE_CAP ? s_done[i] : ovf[i - DONE_CAP], npairs, leaf_max, quads);
+}
+
+// Complete the node at split s, whose two children have arrived, and climb
+// until an arrival is first at its split, the root is done, or (when the
+// block [c0, c1) is not the whole tree) the next split is not inside the
+// block: that arrival is exchanged for the next pass and, as the second,
+// appended to the top list.
+__device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t c1, bool whole,
+                                   const uint32_t* __restrict__ codes, const int32_t* __restrict__ sorted_ids,
+                                   pair_node* __restrict__ pairs, split_stage* __restrict__ gstage,
+                                   uint32_t* __restrict__ gslot, uint32_t* __restrict__ heights, int slot_heights,
+                                   vr_bvh_node* __restrict__ ref_nodes, float* __restrict__ root_box,
+                                   uint32_t* __restrict__ depth_out, uint32_t* __restrict__ top,
+                                   uint32_t* __restrict__ top_count, quad_node* __restrict__ quads,
+                                   uint32_t leaf_max, uint32_t* s_done, uint32_t* s_ndone, uint32_t* ovf)
+{
+    auto delta = [&](uint32_t k) { return key_delta(codes[k], codes[k + 1], k); };
+    // after a second arrival: this thread's child (box, height, side) and the
+    // parent's range from the exchanged bound, so only the sibling's half of
+    // the stage is read, together with the keys at the range's edges
+    bool carried = false;
+    f3 clo = {}, chi = {};
+    uint32_t ch = 0, cside = 0, cl = 0, cr = 0;
+    for (int level = 0; level < 128 && s + 1 < N; ++level) {
+        const split_stage* g = gstage + s;
+        float b[12];
+        uint32_t h0, h1, l, r;
+        if (!carried) {
+            const float4* gb = reinterpret_cast<const float4*>(g->box);
+            const float4 q0 = gb[0], q1 = gb[1], q2 = gb[2];
+            const uint4 meta = *reinterpret_cast<const uint4*>(g->h);
+            b[0] = q0.x; b[1] = q0.y; b[2] = q0.z; b[3] = q0.w;
+            b[4] = q1.x; b[5] = q1.y; b[6] = q1.z; b[7] = q1.w;
+            b[8] = q2.x; b[9] = q2.y; b[10] = q2.z; b[11] = q2.w;
+            h0 = meta.x;
+            h1 = meta.y;
+            l = meta.z;
+            r = meta.w;
+        } else {
+            const uint32_t os = 1u - cside;
+            const float2* ob = reinterpret_cast<const float2*>(g->box + 6 * os);
+            const float2 a = ob[0], c = ob[1], d = ob[2];
+            const uint32_t oh = g->h[os];
+            const float mine[6] = {clo.x, clo.y, clo.z, chi.x, chi.y, chi.z};
+            const float theirs[6] = {a.x, a.y, c.x, c.y, d.x, d.y};
+            for (int q = 0; q < 6; ++q) {
+                b[q] = cside ? theirs[q] : mine[q];
+                b[6 + q] = cside ? mine[q] : theirs[q];
+            }
+            h0 = cside ? oh : ch;
+            h1 = cside ? ch : oh;
+            l = cl;
+            r = cr;
+        }
+        const f3 lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
+        const f3 hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
+        const uint32_t h = max(h0, h1) + 1u;
+        const uint32_t e0 = l == s ? leaf_entry(s, 1u) : s;
+        const uint32_t e1 = s + 1 == r ? leaf_entry(s + 1, 1u) : s + 1;
+        const bool root = l == 0 && r == N - 1;
+        bool right = false;
+        if (!root) {
+            const int dl = l > 0 ? delta(l - 1) : -1;
+            const int dr = r + 1 < N ? delta(r) : -1;
+            right = dr > dl;
+        }
+        const uint32_t p = root ? 0u : (right ? r : l);
+        const uint32_t e3 = p == l ? r : l;
+        // this thread's arrival at the parent's split first (its release
+        // waits only for the hand-over's stores), then the node's record and
+        // the rest
+        uint32_t s2 = 0, side = 0, bound = 0, other = SLOT_EMPTY;
+        bool climb = false;
+        if (!root) {
+            s2 = right ? r : l - 1;
+            side = right ? 0u : 1u;
+            bound = right ? l : r;
+            split_stage* g2 = gstage + s2;
+            store_box(g2->box + 6 * side, lo, hi);
+            g2->h[side] = h;
+            g2->bound[side] = bound;
+            if (!whole && (s2 < c0 || s2 + 2 > c1)) {
+                if (__hip_atomic_exchange(&gslot[s2], bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                    SLOT_EMPTY)
+                    top[atomicAdd(top_count, 1u)] = s2;
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                other = __hip_atomic_exchange(&gslot[s2], bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                climb = other != SLOT_EMPTY;
+            }
+        }
+        // the record but for word 14, its parent link, which the pass that
+        // completes the parent writes (the root's stays empty); the
+        // children's links now.  No word is written twice, so these stores
+        // need no order among the passes' threads.
+        {
             float4* dst = reinterpret_cast<float4*>(pairs[p].f);
             dst[0] = make_float4(b[0], b[1], b[2], b[3]);
             dst[1] = make_float4(b[4], b[5], b[6], b[7]);
             dst[2] = make_float4(b[8], b[9], b[10], b[11]);
-            *reinterpret_cast<uint4*>(pairs[p].e) = s_ent[k];
-            lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
-            hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
-            h = max(s_h[2 * k], s_h[2 * k + 1]) + 1u;
-            ref_first = 1u + 2u * p;
-            ref_num = 0;
-            if (p == 0) {
-                finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
-                return;
-            }
-            link = s_ent[k].z;
+            *reinterpret_cast<uint2*>(pairs[p].e) = make_uint2(e0, e1);
+            pairs[p].e[3] = e3;
+            if (root) pairs[p].e[2] = SLOT_EMPTY;
         }
-    };
-    if (j < n) leaf_and_climb();
-    // every pair whose leaf range lies in the block is complete now, with
-    // both child boxes in LDS: its quad, one pair per thread
-    __syncthreads();
-    if (!quads || n < 2 || j + 1 >= n) return;
-    const uint32_t q = (uint32_t)j, other = s_ent[tid].w;
-    if (other >= c0 && other < c1 && pair_gets_quad(q, other, leaf_max))
-        make_quad<true>(lds_pairs{s_box, s_ent, (uint32_t)c0}, q, (uint32_t)(n - 1), leaf_max, quads);
-}
-
-// Climb from node p, whose children are both published, until an arrival
-// is first at its node, the root is done, or (when the block [c0, c1) is not
-// the whole tree) the next node leaves the block: that node's arrival is
-// counted for the next pass and, as the second, appended to its ready list.
-__device__ inline void climb_global(uint32_t p, uint32_t npairs, size_t c0, size_t c1, bool whole,
-                                    pair_node* __restrict__ pairs, const uint4* __restrict__ ent,
-                                    uint32_t* __restrict__ flags,
-                                    uint32_t* __restrict__ heights, vr_bvh_node* __restrict__ ref_nodes,
-                                    float* __restrict__ root_box, uint32_t* __restrict__ depth_out,
-                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
-                                    quad_node* __restrict__ quads, uint32_t leaf_max, uint32_t* s_done,
-                                    uint32_t* s_ndone)
-{
-    for (int level = 0; level < 128 && p < npairs; ++level) {
-        const float4* src = reinterpret_cast<const float4*>(pairs[p].f);
-        const float4 q0 = src[0], q1 = src[1], q2 = src[2];
-        const f3 lo = min3v(mk3(q0.x, q0.y, q0.z), mk3(q1.z, q1.w, q2.x));
-        const f3 hi = max3v(mk3(q0.w, q1.x, q1.y), mk3(q2.y, q2.z, q2.w));
-        const uint32_t h = max(heights[2 * p], heights[2 * p + 1]) + 1u;
-        // the record is complete: both child boxes and now the entries
-        const uint4 ep = ent[p];
-        *reinterpret_cast<uint4*>(pairs[p].e) = ep;
+        if (!(e0 & LEAF_BIT)) pairs[s].e[2] = (p << 1) | 0u;
+        if (!(e1 & LEAF_BIT)) pairs[s + 1].e[2] = (p << 1) | 1u;
+        if (slot_heights) {
+            heights[2 * p] = h0;
+            heights[2 * p + 1] = h1;
+        }
+        if (ref_nodes) store_ref_children(ref_nodes, sorted_ids, p, b, e0, e1);
         // p spans chunks (far above the leaf size) and its subtree is
-        // complete: its quad after the workgroup's climbs (or now, when the
-        // list is full)
+        // complete: its quad after the workgroup's climbs
         if (quads) {
             const uint32_t slot = atomicAdd(s_ndone, 1u);
             if (slot < DONE_CAP) s_done[slot] = p;
-            else make_quad<true>(global_pairs{pairs}, p, npairs, leaf_max, quads);
+            else ovf[slot - DONE_CAP] = p;
         }
-        if (p == 0) {
+        if (root) {
             finish_root(root_box, ref_nodes, depth_out, lo, hi, h);
             return;
         }
-        const uint32_t link = ep.z;
-        const uint32_t pp = link >> 1, side = link & 1u;
-        if (pp >= npairs) return;
-        publish_child(pairs, heights, ref_nodes, pp, side, lo, hi, h, 1u + 2u * p, 0u);
-        if (!whole) {
-            const uint32_t other = ent[pp].w;
-            if (!(pp >= c0 && pp < c1 && other >= c0 && other < c1)) {
-                if (atomicAdd(&flags[pp], 1u) == 1u) next[atomicAdd(next_count, 1u)] = pp;
-                return;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (__hip_atomic_fetch_add(&flags[pp], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+        if (!climb) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        p = pp;
+        carried = true;
+        clo = lo;
+        chi = hi;
+        ch = h;
+        cside = side;
+        cl = right ? l : other;
+        cr = right ? other : r;
+        s = s2;
     }
 }
 
 // pass 2: workgroup b finishes the nodes inside leaves [b * SUPER, (b + 1) * SUPER)
-__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, pair_node* __restrict__ pairs,
-                                                             const uint4* __restrict__ ent,
-                                                             uint32_t* __restrict__ flags,
-                                                             uint32_t* __restrict__ heights,
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, const uint32_t* __restrict__ codes,
+                                                             const int32_t* __restrict__ sorted_ids,
+                                                             pair_node* __restrict__ pairs,
+                                                             split_stage* __restrict__ gstage,
+                                                             uint32_t* __restrict__ gslot,
+                                                             uint32_t* __restrict__ heights, int slot_heights,
                                                              vr_bvh_node* __restrict__ ref_nodes,
                                                              float* __restrict__ root_box,
                                                              uint32_t* __restrict__ depth_out,
                                                              const uint32_t* __restrict__ ready,
                                                              const uint32_t* __restrict__ bucket_count,
                                                              uint32_t* __restrict__ top, uint32_t* __restrict__ top_count,
-                                                             quad_node* __restrict__ quads, uint32_t leaf_max)
+                                                             uint32_t* __restrict__ ovf, quad_node* __restrict__ quads,
+                                                             uint32_t leaf_max)
 {
-    const uint32_t npairs = (uint32_t)(n - 1);
-    const size_t c0 = (size_t)blockIdx.x * SUPER;
-    const size_t c1 = min(n, c0 + (size_t)SUPER);
+    const uint32_t N = (uint32_t)n;
+    const uint32_t c0 = blockIdx.x * SUPER;
+    const uint32_t c1 = min(N, c0 + SUPER);
     __shared__ uint32_t s_done[DONE_CAP];
     __shared__ uint32_t s_ndone;
     if (threadIdx.x == 0) s_ndone = 0u;
     __syncthreads();
     const uint32_t count = min(bucket_count[blockIdx.x], SUPER);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_global(ready[c0 + i], npairs, c0, c1, false, pairs, ent, flags, heights, ref_nodes, root_box, depth_out,
-                     top, top_count, quads, leaf_max, s_done, &s_ndone);
-    quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
+        climb_split(ready[(size_t)c0 + i], N, c0, c1, false, codes, sorted_ids, pairs, gstage, gslot, heights,
+                    slot_heights, ref_nodes, root_box, depth_out, top, top_count, quads, leaf_max, s_done, &s_ndone,
+                    ovf + c0);
+    quads_after_climbs(pairs, N - 1, leaf_max, quads, s_done, &s_ndone, ovf + c0);
 }
 
 // pass 3: one workgroup finishes the nodes that span SUPER blocks
-__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, pair_node* __restrict__ pairs,
-                                                             const uint4* __restrict__ ent,
-                                                             uint32_t* __restrict__ flags,
-                                                             uint32_t* __restrict__ heights,
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, const uint32_t* __restrict__ codes,
+                                                             const int32_t* __restrict__ sorted_ids,
+                                                             pair_node* __restrict__ pairs,
+                                                             split_stage* __restrict__ gstage,
+                                                             uint32_t* __restrict__ gslot,
+                                                             uint32_t* __restrict__ heights, int slot_heights,
                                                              vr_bvh_node* __restrict__ ref_nodes,
                                                              float* __restrict__ root_box,
                                                              uint32_t* __restrict__ depth_out,
                                                              const uint32_t* __restrict__ top,
                                                              const uint32_t* __restrict__ top_count,
-                                                             quad_node* __restrict__ quads, uint32_t leaf_max)
+                                                             uint32_t* __restrict__ ovf, quad_node* __restrict__ quads,
+                                                             uint32_t leaf_max)
 {
-    const uint32_t npairs = (uint32_t)(n - 1);
+    const uint32_t N = (uint32_t)n;
     __shared__ uint32_t s_done[DONE_CAP];
     __shared__ uint32_t s_ndone;
     if (threadIdx.x == 0) s_ndone = 0u;
     __syncthreads();
-    const uint32_t count = min(*top_count, npairs);
+    const uint32_t count = min(*top_count, N - 1);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_global(top[i], npairs, 0, n, true, pairs, ent, flags, heights, ref_nodes, root_box, depth_out, nullptr,
-                     nullptr, quads, leaf_max, s_done, &s_ndone);
-    quads_after_climbs(pairs, npairs, leaf_max, quads, s_done, &s_ndone);
+        climb_split(top[i], N, 0, N, true, codes, sorted_ids, pairs, gstage, gslot, heights, slot_heights, ref_nodes,
+                    root_box, depth_out, nullptr, nullptr, quads, leaf_max, s_done, &s_ndone, ovf);
+    quads_after_climbs(pairs, N - 1, leaf_max, quads, s_done, &s_ndone, ovf);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
 {
     const size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// the split slots on the chunks' edges (CHUNK * k - 1 for 0 < k, below n - 1) set empty
+__global__ void k_clear_edges(uint32_t* slots, size_t n)
+{
+    const size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x + 1;
+    const size_t s = k * CHUNK - 1;
+    if (s + 1 < n) slots[s] = SLOT_EMPTY;
 }
 
 
@@ -840,9 +1010,10 @@ struct build_scratch {
     uint32_t* totals; // per-digit totals
     uint32_t* cb;     // 6 words
     float* part;      // CB_GRID x 6 per-block centroid extremes
-    uint32_t* leaf_parent;
+    split_stage* stage;  // bottom-up hand-overs in global memory, by split (n - 1)
     uint32_t* counts;  // bottom-up ready-list counters: one per SUPER block, then the top list's
-                       // (the ready lists reuse keys0 and vals0 after the sort)
+                       // (the ready lists reuse keys0 and vals0 after the sort, the mid and
+                       // top passes' quad overflow lists keys1)
     size_t bytes;
 };
 
@@ -866,48 +1037,52 @@ build_scratch plan_scratch(size_t n, char* base)
     s.totals = take(RADIX);
     s.cb = take(8);
     s.part = reinterpret_cast<float*>(take((size_t)CB_GRID * 6));
-    s.leaf_parent = take(n);
+    s.stage = reinterpret_cast<split_stage*>(take(n * (sizeof(split_stage) / 4)));
     s.counts = take((n + SUPER - 1) / SUPER + 1);
     s.bytes = off;
     return s;
 }
 
-// centroid box (and, when dst is set, the primitive copy), then the codes
-// (read from the copy when there is one) with the kernel's extras
-int launch_morton(const vr_triangle* prims, size_t n, vr_triangle* dst, float* part, uint32_t* cb, uint32_t* codes,
+// centroid box, then the codes with the kernel's extras
+int launch_morton(const vr_triangle* prims, size_t n, float* part, uint32_t* cb, uint32_t* codes,
                   const morton_extras& x, hipStream_t st)
 {
-    const unsigned grid = std::max(1u, std::min(div_up(n, CB_BLOCK), (unsigned)CB_GRID));
-    auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
-    const int vec = aligned(prims) && (!dst || aligned(dst));
-    hipLaunchKernelGGL(k_copy_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, dst, vec, part);
+    const unsigned grid = std::max(1u, std::min(div_up(n, CB_U * CB_BLOCK), (unsigned)CB_GRID));
+    const int vec = reinterpret_cast<uintptr_t>(prims) % 16 == 0;
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, vec, part);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(CB_BLOCK), 0, st, part, grid, cb);
-    const vr_triangle* src = dst ? dst : prims;
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, TILE)), dim3(BLOCK), 0, st, src, n, (int)aligned(src), cb, codes, x);
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, TILE)), dim3(BLOCK), 0, st, prims, n, vec, cb, codes, x);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
 
-// the bottom-up passes; a build has cleared the ready-list counters already
-// (k_morton) and writes the identity index list on the way (a refit keeps it)
+// the bottom-up passes over the sorted keys (hierarchy, boxes, records,
+// quads); a build has cleared the ready-list counters and the chunk-edge
+// split slots already (k_morton), a refit clears them here
 void launch_bottom_up(const bvh_state& bs, const build_scratch& s, const vr_triangle* prims, vr_bvh_node* ref_nodes,
                       bool build, hipStream_t st)
 {
     const size_t n = bs.num_prims;
     const size_t nb = (n + SUPER - 1) / SUPER;
     uint32_t* depth = reinterpret_cast<uint32_t*>(bs.root_box + 6);
-    if (!build)
+    uint32_t* slots = bs.flags;
+    const int slot_heights = (int)(build && bs.want_rotate);
+    if (!build) {
         hipLaunchKernelGGL(k_fill_u32, dim3(div_up(nb + 1, BLOCK)), dim3(BLOCK), 0, st, s.counts, nb + 1, 0u);
-    hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, prims, n, bs.sorted_ids,
-                       s.leaf_parent, bs.pairs, bs.entries, bs.flags, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
-                       s.keys0, s.counts, (int)(build && bs.want_rotate), bs.quads, LBVH_LEAF_MAX);
+        if (n > (size_t)CHUNK)
+            hipLaunchKernelGGL(k_clear_edges, dim3(div_up(n / CHUNK, BLOCK)), dim3(BLOCK), 0, st, slots, n);
+    }
+    hipLaunchKernelGGL(k_bottom_up_local, dim3(div_up(n, CHUNK)), dim3(CHUNK), 0, st, prims, n, bs.sorted_codes,
+                       bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
+                       s.keys0, s.counts, s.vals0, s.counts + nb, slot_heights, bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)CHUNK)
-        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.entries, bs.flags,
-                           bs.heights, ref_nodes, bs.root_box, depth, s.keys0, s.counts, s.vals0, s.counts + nb,
-                           bs.quads, LBVH_LEAF_MAX);
+        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.sorted_codes,
+                           bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box,
+                           depth, s.keys0, s.counts, s.vals0, s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)SUPER)
-        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.pairs, bs.entries, bs.flags, bs.heights,
-                           ref_nodes, bs.root_box, depth, s.vals0, s.counts + nb, bs.quads, LBVH_LEAF_MAX);
+        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.sorted_codes, bs.sorted_ids,
+                           bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box, depth, s.vals0,
+                           s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
 }
 
 }  // namespace
@@ -949,7 +1124,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         // caller's triangles (valid until its work on the stream is done), a
         // refit the new ones, and an LBVH's index list is the identity
         const size_t sizes[] = {np1 * sizeof(pair_node), np1 * sizeof(quad_node), n * sizeof(tri_node),
-                                np1 * sizeof(uint4), n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
+                                0, n * 4, n * 4, np1 * 4, 2 * np1 * 4, 8 * sizeof(float),
                                 0, bs.num_ref_nodes * sizeof(vr_bvh_node), plan.bytes};
         size_t total = 0;
         for (size_t b : sizes) total += align_up(b, 256);
@@ -962,7 +1137,7 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
         carve(bs.pairs, sizes[0]);
         carve(bs.quads, sizes[1]);
         carve(bs.tris, sizes[2]);
-        carve(bs.entries, sizes[3]);
+        carve(bs.prims, sizes[3]);
         carve(bs.sorted_codes, sizes[4]);
         carve(bs.sorted_ids, sizes[5]);
         carve(bs.flags, sizes[6]);
@@ -975,12 +1150,12 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     if (n == 0) return VR_OK;
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
 
-    // 1-2: copy of the primitives with the centroid bounds, Morton codes
-    // (with the first sort pass's digit counts and the cleared link arrays)
+    // 1-2: centroid bounds, Morton codes (with the first sort pass's digit
+    // counts, the cleared ready-list counters and chunk-edge split slots)
     const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
     const size_t nb = (n + SUPER - 1) / SUPER;
-    const morton_extras extras{s.hist, tiles, s.leaf_parent, bs.flags, s.counts, (uint32_t)(nb + 1)};
-    int rc = launch_morton(d_prims, n, nullptr, s.part, s.cb, s.keys0, extras, st);
+    const morton_extras extras{s.hist, tiles, bs.flags, s.counts, (uint32_t)(nb + 1)};
+    int rc = launch_morton(d_prims, n, s.part, s.cb, s.keys0, extras, st);
     if (rc) return rc;
 
     // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
@@ -1004,13 +1179,9 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     }
     VR_HIP_CHECK(hipGetLastError());
 
-    // 4: hierarchy
-    if (n > 1)
-        hipLaunchKernelGGL(k_karras, dim3(div_up(npairs, BLOCK)), dim3(BLOCK), 0, st, bs.sorted_codes, (int)n,
-                           bs.entries, s.leaf_parent);
-    // 5: boxes, traversal-order triangles, reference layout, index list,
-    // and the 4-wide traversal nodes (each pair's quad as soon as its
-    // subtree is complete)
+    // 4: hierarchy and boxes from the leaves up, traversal-order triangles,
+    // reference layout, and the 4-wide traversal nodes (each pair's quad as
+    // soon as its subtree is complete)
     launch_bottom_up(bs, s, d_prims, keep_nodes ? bs.ref_nodes : nullptr, true, st);
     VR_HIP_CHECK(hipGetLastError());
     bs.root = n == 1 ? leaf_entry(0, 1) : 0u;
@@ -1041,13 +1212,9 @@ int lbvh_refit(bvh_state& bs, const vr_triangle* d_prims, hipStream_t st)
     if (n == 0) return VR_OK;
     VR_CHECK(!bs.rotated, VR_ERR_UNSUPPORTED, "refit: a rotated LBVH is rebuilt (vr_lbvh_rebuild), not refitted");
     build_scratch s = plan_scratch(n, reinterpret_cast<char*>(bs.scratch));
-    // the hierarchy (child entries, parent links, ranges, leaf parents) is
-    // unchanged since the build; only the boxes are recomputed
-    if (n > 1)
-        hipLaunchKernelGGL(k_fill_u32, dim3(div_up(bs.num_pairs, BLOCK)), dim3(BLOCK), 0, st, bs.flags,
-                           bs.num_pairs, 0u);
-    // same topology: the bottom-up pass makes the quads from the new boxes
-    // (the greedy choices may move)
+    // the sorted keys are kept, so the bottom-up pass rebuilds the same
+    // hierarchy with the new boxes, and the quads from them (the greedy
+    // choices may move)
     launch_bottom_up(bs, s, d_prims, bs.ref_nodes, false, st);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
@@ -1100,7 +1267,7 @@ int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hi
     VR_HIP_CHECK(hipMallocAsync(&tmp, 256 + (size_t)CB_GRID * 6 * sizeof(float), st));
     uint32_t* cb = static_cast<uint32_t*>(tmp);
     float* part = reinterpret_cast<float*>(static_cast<char*>(tmp) + 256);
-    int rc = launch_morton(d_prims, n, nullptr, part, cb, d_codes, morton_extras{}, st);
+    int rc = launch_morton(d_prims, n, part, cb, d_codes, morton_extras{}, st);
     (void)hipFreeAsync(tmp, st);
     return rc;
 }

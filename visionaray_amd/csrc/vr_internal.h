@@ -68,7 +68,6 @@ struct bvh_state {
     uint32_t max_leaf = 1;
     bool is_lbvh = false;
     pair_node* pairs = nullptr;      // num_pairs
-    uint4* entries = nullptr;        // LBVH: each pair's entries (children, parent link, range end), dense
     quad_node* quads = nullptr;      // traversal nodes, one per pair (only those below quad 0 are reached)
     uint32_t collapse_leaf_max = 0;  // leaf size the quads were collapsed with
     tri_node* tris = nullptr;        // num_prims, traversal order
@@ -76,7 +75,7 @@ struct bvh_state {
     uint32_t* sorted_codes = nullptr;
     int32_t* sorted_ids = nullptr;   // traversal position -> original index
     int32_t* parents = nullptr;      // LBVH: parent pair of every node (2n-1)
-    uint32_t* flags = nullptr;       // bottom-up arrival counters (num_pairs)
+    uint32_t* flags = nullptr;       // LBVH: bottom-up split slots (num_pairs)
     uint32_t* heights = nullptr;     // bottom-up subtree heights (num_pairs)
     vr_bvh_node* ref_nodes = nullptr;   // optional reference layout
     uint32_t* indices = nullptr;        // tree.indices()
