@@ -10,13 +10,14 @@
 //      ballot-match ranking and LDS staging so the scatter writes runs of one
 //      digit contiguously (the reference calls CUB's merge sort, lbvh.h:740-755;
 //      a stable sort of ids 0..n-1 has exactly one result)
-//   4. Karras 2012 hierarchy (lbvh.h:120-185, 281-330) with exact integer
-//      step sizes, writing children straight into the traversal records
-//   5. bottom-up boxes where the second thread to reach a node computes its
-//      box (one atomic per node instead of lbvh.h:332-372's six float CAS
-//      atomics per level per leaf) -- the boxes are the same exact min/max
-//      unions -- fused with the collapse to the reference bvh_node layout
-//      (lbvh.h:374-451) and the copy of triangles into traversal order.
+//   4. the Karras 2012 hierarchy (lbvh.h:120-185, 281-330) found bottom-up:
+//      a climb from each leaf takes the parent on the side whose boundary
+//      keys share the longer prefix, and the second thread to reach a node
+//      computes its box (one atomic per node instead of lbvh.h:332-372's six
+//      float CAS atomics per level per leaf) -- the same nodes, numbers and
+//      exact min/max boxes -- fused with the collapse to the reference
+//      bvh_node layout (lbvh.h:374-451), the copy of triangles into
+//      traversal order and the 4-wide traversal nodes.
 //      Nodes inside a 1024-leaf chunk meet in LDS, nodes inside a 65536-leaf
 //      block in one workgroup, the rest in a single workgroup: no
 //      device-scope fence is needed.
