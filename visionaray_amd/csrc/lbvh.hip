@@ -483,12 +483,19 @@ constexpr uint32_t SUPER = 64u * CHUNK;
 constexpr uint32_t DONE_CAP = 4096;
 
 // One split's hand-over in global memory: each child's box, height and
-// outer bound (the left child's l, the right child's r) on its side.
+// outer bound (the left child's l, the right child's r) on its side, and the
+// prefix length at that outer edge (delta(l - 1) for the left child,
+// delta(r) for the right one: the two the parent's own side is decided by),
+// so a later pass climbs without reading keys.
 struct alignas(64) split_stage {
     float box[12];
-    uint32_t h[2];
+    uint32_t hd[2];  // height | (outer delta + 1) << 8
     uint32_t bound[2];
 };
+
+__device__ inline uint32_t pack_hd(uint32_t h, int d) { return h | ((uint32_t)(d + 1) << 8); }
+__device__ inline uint32_t hd_height(uint32_t hd) { return hd & 0xFFu; }
+__device__ inline int hd_delta(uint32_t hd) { return (int)(hd >> 8) - 1; }
 static_assert(sizeof(split_stage) == 64, "split_stage is one 64-byte record");
 
 // common-prefix length of the sorted keys k and k + 1 (codes a and b)
@@ -640,9 +647,10 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
         for (int level = 0; level < 128; ++level) {
             const bool root = l == 0 && r == N - 1;
             bool right = false;
+            int dl = -1, dr = -1;
             if (!root) {
-                const int dl = l > 0 ? delta(l - 1) : -1;
-                const int dr = r + 1 < N ? delta(r) : -1;
+                dl = l > 0 ? delta(l - 1) : -1;
+                dr = r + 1 < N ? delta(r) : -1;
                 right = dr > dl;  // parent's split r: this node is its left child
             }
             if (inner) {
@@ -663,7 +671,7 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
                 // the parent spans the chunk's edge: hand over in global memory
                 split_stage* g = gstage + s;
                 store_box(g->box + 6 * side, lo, hi);
-                g->h[side] = h;
+                g->hd[side] = pack_hd(h, right ? dl : dr);
                 g->bound[side] = bound;
                 if (__hip_atomic_exchange(&gslot[s], bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
                     SLOT_EMPTY)
@@ -703,7 +711,10 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
             const uint32_t side = v > s ? 1u : 0u;  // a right child's bound is its r > s
             split_stage* st = gstage + s;
             for (int q = 0; q < 6; ++q) st->box[6 * side + q] = s_stage[tid * 12 + 6 * side + q];
-            st->h[side] = s_h[2 * tid + side];
+            // the outer edge's prefix length: delta(l - 1) of a left child
+            // (l = v), delta(r) of a right one (r = v); both keys are in s_code
+            const int od = side ? (v + 1 < N ? delta(v) : -1) : (v > 0 ? delta(v - 1) : -1);
+            st->hd[side] = pack_hd(s_h[2 * tid + side], od);
             st->bound[side] = v;
             g = v;
         }
@@ -761,7 +772,7 @@ __device__ inline void quads_after_climbs(const pair_node* __restrict__ pairs, u
 // block: that arrival is exchanged for the next pass and, as the second,
 // appended to the top list.
 __device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t c1, bool whole,
-                                   const uint32_t* __restrict__ codes, const int32_t* __restrict__ sorted_ids,
+                                   const int32_t* __restrict__ sorted_ids,
                                    pair_node* __restrict__ pairs, split_stage* __restrict__ gstage,
                                    uint32_t* __restrict__ gslot, uint32_t* __restrict__ heights, int slot_heights,
                                    vr_bvh_node* __restrict__ ref_nodes, float* __restrict__ root_box,
@@ -769,56 +780,54 @@ __device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t
                                    uint32_t* __restrict__ top_count, quad_node* __restrict__ quads,
                                    uint32_t leaf_max, uint32_t* s_done, uint32_t* s_ndone, uint32_t* ovf)
 {
-    auto delta = [&](uint32_t k) { return key_delta(codes[k], codes[k + 1], k); };
-    // after a second arrival: this thread's child (box, height, side) and the
-    // parent's range from the exchanged bound, so only the sibling's half of
-    // the stage is read, together with the keys at the range's edges
+    // after a second arrival: this thread's child (box, height, outer prefix
+    // length, side) and the parent's range from the exchanged bound, so only
+    // the sibling's half of the stage is read
     bool carried = false;
     f3 clo = {}, chi = {};
-    uint32_t ch = 0, cside = 0, cl = 0, cr = 0;
+    uint32_t chd = 0, cside = 0, cl = 0, cr = 0;
     for (int level = 0; level < 128 && s + 1 < N; ++level) {
         const split_stage* g = gstage + s;
         float b[12];
-        uint32_t h0, h1, l, r;
+        uint32_t hd0, hd1, l, r;
         if (!carried) {
             const float4* gb = reinterpret_cast<const float4*>(g->box);
             const float4 q0 = gb[0], q1 = gb[1], q2 = gb[2];
-            const uint4 meta = *reinterpret_cast<const uint4*>(g->h);
+            const uint4 meta = *reinterpret_cast<const uint4*>(g->hd);
             b[0] = q0.x; b[1] = q0.y; b[2] = q0.z; b[3] = q0.w;
             b[4] = q1.x; b[5] = q1.y; b[6] = q1.z; b[7] = q1.w;
             b[8] = q2.x; b[9] = q2.y; b[10] = q2.z; b[11] = q2.w;
-            h0 = meta.x;
-            h1 = meta.y;
+            hd0 = meta.x;
+            hd1 = meta.y;
             l = meta.z;
             r = meta.w;
         } else {
             const uint32_t os = 1u - cside;
             const float2* ob = reinterpret_cast<const float2*>(g->box + 6 * os);
             const float2 a = ob[0], c = ob[1], d = ob[2];
-            const uint32_t oh = g->h[os];
+            const uint32_t ohd = g->hd[os];
             const float mine[6] = {clo.x, clo.y, clo.z, chi.x, chi.y, chi.z};
             const float theirs[6] = {a.x, a.y, c.x, c.y, d.x, d.y};
             for (int q = 0; q < 6; ++q) {
                 b[q] = cside ? theirs[q] : mine[q];
                 b[6 + q] = cside ? mine[q] : theirs[q];
             }
-            h0 = cside ? oh : ch;
-            h1 = cside ? ch : oh;
+            hd0 = cside ? ohd : chd;
+            hd1 = cside ? chd : ohd;
             l = cl;
             r = cr;
         }
+        const uint32_t h0 = hd_height(hd0), h1 = hd_height(hd1);
         const f3 lo = min3v(mk3(b[0], b[1], b[2]), mk3(b[6], b[7], b[8]));
         const f3 hi = max3v(mk3(b[3], b[4], b[5]), mk3(b[9], b[10], b[11]));
         const uint32_t h = max(h0, h1) + 1u;
         const uint32_t e0 = l == s ? leaf_entry(s, 1u) : s;
         const uint32_t e1 = s + 1 == r ? leaf_entry(s + 1, 1u) : s + 1;
         const bool root = l == 0 && r == N - 1;
-        bool right = false;
-        if (!root) {
-            const int dl = l > 0 ? delta(l - 1) : -1;
-            const int dr = r + 1 < N ? delta(r) : -1;
-            right = dr > dl;
-        }
+        // the prefix lengths at the node's edges: delta(l - 1) is the left
+        // child's outer one, delta(r) the right child's
+        const int dl = hd_delta(hd0), dr = hd_delta(hd1);
+        const bool right = !root && dr > dl;
         const uint32_t p = root ? 0u : (right ? r : l);
         const uint32_t e3 = p == l ? r : l;
         // this thread's arrival at the parent's split first (its release
@@ -832,7 +841,7 @@ __device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t
             bound = right ? l : r;
             split_stage* g2 = gstage + s2;
             store_box(g2->box + 6 * side, lo, hi);
-            g2->h[side] = h;
+            g2->hd[side] = pack_hd(h, right ? dl : dr);
             g2->bound[side] = bound;
             if (!whole && (s2 < c0 || s2 + 2 > c1)) {
                 if (__hip_atomic_exchange(&gslot[s2], bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
@@ -880,7 +889,7 @@ __device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t
         carried = true;
         clo = lo;
         chi = hi;
-        ch = h;
+        chd = pack_hd(h, right ? dl : dr);
         cside = side;
         cl = right ? l : other;
         cr = right ? other : r;
@@ -889,8 +898,7 @@ __device__ inline void climb_split(uint32_t s, uint32_t N, uint32_t c0, uint32_t
 }
 
 // pass 2: workgroup b finishes the nodes inside leaves [b * SUPER, (b + 1) * SUPER)
-__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, const uint32_t* __restrict__ codes,
-                                                             const int32_t* __restrict__ sorted_ids,
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, const int32_t* __restrict__ sorted_ids,
                                                              pair_node* __restrict__ pairs,
                                                              split_stage* __restrict__ gstage,
                                                              uint32_t* __restrict__ gslot,
@@ -913,15 +921,14 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_mid(size_t n, const uin
     __syncthreads();
     const uint32_t count = min(bucket_count[blockIdx.x], SUPER);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_split(ready[(size_t)c0 + i], N, c0, c1, false, codes, sorted_ids, pairs, gstage, gslot, heights,
+        climb_split(ready[(size_t)c0 + i], N, c0, c1, false, sorted_ids, pairs, gstage, gslot, heights,
                     slot_heights, ref_nodes, root_box, depth_out, top, top_count, quads, leaf_max, s_done, &s_ndone,
                     ovf + c0);
     quads_after_climbs(pairs, N - 1, leaf_max, quads, s_done, &s_ndone, ovf + c0);
 }
 
 // pass 3: one workgroup finishes the nodes that span SUPER blocks
-__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, const uint32_t* __restrict__ codes,
-                                                             const int32_t* __restrict__ sorted_ids,
+__global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, const int32_t* __restrict__ sorted_ids,
                                                              pair_node* __restrict__ pairs,
                                                              split_stage* __restrict__ gstage,
                                                              uint32_t* __restrict__ gslot,
@@ -941,7 +948,7 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, const uin
     __syncthreads();
     const uint32_t count = min(*top_count, N - 1);
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
-        climb_split(top[i], N, 0, N, true, codes, sorted_ids, pairs, gstage, gslot, heights, slot_heights, ref_nodes,
+        climb_split(top[i], N, 0, N, true, sorted_ids, pairs, gstage, gslot, heights, slot_heights, ref_nodes,
                     root_box, depth_out, nullptr, nullptr, quads, leaf_max, s_done, &s_ndone, ovf);
     quads_after_climbs(pairs, N - 1, leaf_max, quads, s_done, &s_ndone, ovf);
 }
@@ -1077,11 +1084,10 @@ void launch_bottom_up(const bvh_state& bs, const build_scratch& s, const vr_tria
                        bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, bs.tris, ref_nodes, bs.root_box, depth,
                        s.keys0, s.counts, s.vals0, s.counts + nb, slot_heights, bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)CHUNK)
-        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.sorted_codes,
-                           bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box,
+        hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box,
                            depth, s.keys0, s.counts, s.vals0, s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
     if (n > (size_t)SUPER)
-        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.sorted_codes, bs.sorted_ids,
+        hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.sorted_ids,
                            bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box, depth, s.vals0,
                            s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
 }
