@@ -91,6 +91,65 @@ def test_ray_limits_and_axis_aligned_dirs(V, O):
         assert_hits_match(got, want, tris, rays, any_hit)
 
 
+@pytest.mark.parametrize("scene", ["heightfield", "clusters"])
+def test_extreme_rays_match_oracle(V, O, scene):
+    """Rays at the edges of the slab arithmetic: direction components of
+    exactly +-0 (inv = FLT_MAX) and of 1e-30, 1e-19, 1e-17, 1e-7 (inv up to
+    1e30), and tmin / tmax windows: hits equal the oracle's traversal of the
+    same tree.  Origins 1e4 away aimed back at the scene: there (p - o) * inv
+    rounds coarsely (ulp(6000) = 2^-11), and the reference's own traversal
+    can reject a single triangle's leaf box that the ray does cross, missing
+    the closest hit that its brute-force closest_hit finds; the GPU's leaves
+    of two triangles test that triangle without its own box.  So a far ray
+    either equals the oracle's hit or is the brute-force closest hit (the
+    same arithmetic over every triangle), and never lies beyond the oracle's."""
+    tris = {"heightfield": lambda: S.heightfield(96, seed=1),
+            "clusters": lambda: S.random_clusters(40_000, seed=2)}[scene]()
+    print(f"[case] extreme rays {scene} n={tris.size}", flush=True)
+    bvh = V.lbvh_builder().build(tris)
+    nodes, idx, _, _ = O.lbvh_build(tris)
+    rng = np.random.default_rng(21)
+    n = 24_000
+    rays = random_rays(n, seed=22)
+    d = rays["dir"].copy()
+    k = n // 8
+    for j, scale in enumerate((0.0, -0.0, 1e-30, 1e-19, 1e-17, 1e-7)):
+        sl = slice(j * k, (j + 1) * k)
+        axis = rng.integers(0, 3, size=k)
+        d[sl][np.arange(k), axis] = np.float32(scale) * rng.choice([-1.0, 1.0], size=k).astype(np.float32) \
+            if scale != 0.0 else np.float32(scale)
+    rays["dir"] = d
+    far = slice(6 * k, 7 * k)  # origins 1e4 away, aimed back at the scene
+    o = rng.normal(size=(k, 3))
+    o = (o / np.linalg.norm(o, axis=1, keepdims=True) * 1e4).astype(np.float32)
+    tgt = rng.uniform(-0.5, 0.5, (k, 3)).astype(np.float32)
+    dd = tgt - o
+    rays["ori"][far] = o
+    rays["dir"][far] = (dd / np.linalg.norm(dd, axis=1, keepdims=True)).astype(np.float32)
+    win = slice(7 * k, n)  # tmin / tmax windows
+    rays["tmin"][win] = rng.uniform(0.0, 0.6, n - 7 * k).astype(np.float32)
+    rays["tmax"][win] = rng.uniform(0.6, 2.0, n - 7 * k).astype(np.float32)
+    near = np.r_[0:6 * k, 7 * k:n]
+    for any_hit in (False, True):
+        got = V.trace(bvh, rays, any_hit)
+        want = O.trace(nodes, tris, idx, rays, any_hit)
+        assert_hits_match(got[near], want[near], tris, rays[near], any_hit)
+        g, w = got[far], want[far]
+        assert np.array_equal(g["prim_id"] >= 0, w["prim_id"] >= 0)
+        if any_hit:
+            continue
+        hit = w["prim_id"] >= 0
+        assert np.all(g["t"][hit] <= w["t"][hit])
+        other = np.nonzero((g["prim_id"] != w["prim_id"]) | (g["t"] != w["t"]))[0]
+        print(f"[far] {k} rays, {other.size} where the reference's traversal misses its closest hit", flush=True)
+        assert other.size <= k // 100
+        if other.size:
+            fr = rays[far][other]
+            brute = O.trace_brute(tris, fr)
+            O.check_hits(g[other], brute, tris, fr)
+    assert np.sum(V.closest_hit(bvh, rays)["prim_id"] >= 0) > n // 10
+
+
 def test_intersect_kat_ray1tri4(V):
     """test/unittests/math/intersect.cpp:60-121: ray along +x from (-2,0,0)
     through four x = const triangles; the hits are prim 1 (t 0), 2 (t 1),
