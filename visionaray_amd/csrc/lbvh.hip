@@ -206,6 +206,20 @@ __device__ inline uint32_t separate_bits3(uint32_t n)
 // stable LSD radix sort, 8-bit digits
 // ---------------------------------------------------------------------------
 
+// The sort tile of this workgroup.  Workgroups go to the 8 XCDs round
+// robin (workgroup b on XCD b mod 8), and the per-tile digit counts are
+// digit-major words (hist[d * tiles + t]): with tile = b, the 16 tiles whose
+// counts share a 64-byte line ran on 8 XCDs, and each XCD's L2 read and
+// wrote that line on its own.  Here each XCD takes one contiguous range of
+// tiles (a bijection of [0, tiles)), so a line's tiles share an L2.
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t tiles)
+{
+    constexpr uint32_t X = 8;
+    const uint32_t xcd = b % X, local = b / X;
+    const uint32_t base = tiles / X, rem = tiles % X;  // the first rem XCDs take base + 1 tiles
+    return xcd < rem ? xcd * (base + 1) + local : rem * (base + 1) + (xcd - rem) * base + local;
+}
+
 // lanes of this wave holding the same 8-bit digit (among `valid` lanes)
 __device__ inline uint64_t match_digit(uint32_t d, bool valid)
 {
@@ -226,7 +240,8 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict
     __shared__ uint32_t h[RADIX];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * TILE;
+    const uint32_t tile = xcd_tile(blockIdx.x, num_tiles);
+    const size_t base = (size_t)tile * TILE;
     for (int k = 0; k < ITEMS; ++k) {
         const size_t idx = base + (size_t)k * BLOCK + threadIdx.x;
         const bool valid = idx < n;
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict
         if (valid && mbcnt64(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
+    hist[(size_t)threadIdx.x * num_tiles + tile] = h[threadIdx.x];
 }
 
 // block-wide exclusive scan of one value per thread (BLOCK threads)
@@ -278,13 +293,14 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     __shared__ uint32_t s_tmp[WAVES + 1];
 
     const uint32_t tid = threadIdx.x, w = tid / 64;
-    const size_t base = (size_t)blockIdx.x * TILE;
-    const uint32_t cnt = hist[(size_t)tid * num_tiles + blockIdx.x];
+    const uint32_t tile = xcd_tile(blockIdx.x, num_tiles);
+    const size_t base = (size_t)tile * TILE;
+    const uint32_t cnt = hist[(size_t)tid * num_tiles + tile];
     uint32_t total;
     // digit tid's first output slot: all smaller digits, then this digit in
     // the tiles before this one
     const uint32_t dbase = block_exclusive_scan(totals[tid], s_tmp, total);
-    s_goff[tid] = dbase + offsets[(size_t)tid * num_tiles + blockIdx.x];
+    s_goff[tid] = dbase + offsets[(size_t)tid * num_tiles + tile];
     s_dstart[tid] = block_exclusive_scan(cnt, s_tmp, total);
     s_run[tid] = 0;
     for (int q = 0; q < WAVES; ++q) s_wcnt[q][tid] = 0;
@@ -397,7 +413,8 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
     const f3 hi = mk3(o2f(cb[3]), o2f(cb[4]), o2f(cb[5]));
     const f3 center = muls3(add3(hi, lo), 0.5f);
     const f3 size = sub3(hi, lo);
-    const size_t base = (size_t)blockIdx.x * TILE;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);  // (gridDim.x = x.tiles when x.hist is set)
+    const size_t base = (size_t)tile * TILE;
     for (int k = 0; k < ITEMS; ++k) {
         const size_t t0 = base + (size_t)k * BLOCK;
         if (t0 >= n) break;  // uniform over the block
@@ -431,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
         }
         __syncthreads();
     }
-    if (x.hist) x.hist[(size_t)tid * x.tiles + blockIdx.x] = h[tid];
+    if (x.hist) x.hist[(size_t)tid * x.tiles + tile] = h[tid];
     for (size_t c = (size_t)blockIdx.x * BLOCK + tid; c < x.ncounts; c += (size_t)gridDim.x * BLOCK) x.counts[c] = 0u;
 }
 
