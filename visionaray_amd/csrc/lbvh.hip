@@ -955,19 +955,33 @@ __global__ __launch_bounds__(TOP_BLOCK) void k_bottom_up_top(size_t n, const int
                                                              uint32_t* __restrict__ depth_out,
                                                              const uint32_t* __restrict__ top,
                                                              const uint32_t* __restrict__ top_count,
-                                                             uint32_t* __restrict__ ovf, quad_node* __restrict__ quads,
-                                                             uint32_t leaf_max)
+                                                             uint32_t* __restrict__ done, uint32_t* __restrict__ done_count,
+                                                             quad_node* __restrict__ quads, uint32_t leaf_max)
 {
     const uint32_t N = (uint32_t)n;
-    __shared__ uint32_t s_done[DONE_CAP];
     __shared__ uint32_t s_ndone;
     if (threadIdx.x == 0) s_ndone = 0u;
     __syncthreads();
     const uint32_t count = min(*top_count, N - 1);
+    // the completed pairs go straight to the global list (`done`, no LDS
+    // part): their quads are made by k_list_quads, many workgroups wide
     for (uint32_t i = threadIdx.x; i < count; i += TOP_BLOCK)
         climb_split(top[i], N, 0, N, true, sorted_ids, pairs, gstage, gslot, heights, slot_heights, ref_nodes,
-                    root_box, depth_out, nullptr, nullptr, quads, leaf_max, s_done, &s_ndone, ovf);
-    quads_after_climbs(pairs, N - 1, leaf_max, quads, s_done, &s_ndone, ovf);
+                    root_box, depth_out, nullptr, nullptr, quads, leaf_max, done, &s_ndone, done + DONE_CAP);
+    __syncthreads();
+    if (threadIdx.x == 0) *done_count = s_ndone;
+}
+
+// the quads of the pairs the top pass completed (its one workgroup would
+// make them in rounds of dependent record reads, one chain per thread)
+__global__ __launch_bounds__(BLOCK) void k_list_quads(const pair_node* __restrict__ pairs, uint32_t npairs,
+                                                      uint32_t leaf_max, quad_node* __restrict__ quads,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count)
+{
+    const uint32_t c = min(*count, npairs);
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < c; i += gridDim.x * BLOCK)
+        make_quad<true>(global_pairs{pairs}, list[i], npairs, leaf_max, quads);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
@@ -1036,9 +1050,10 @@ struct build_scratch {
     uint32_t* cb;     // 6 words
     float* part;      // CB_GRID x 6 per-block centroid extremes
     split_stage* stage;  // bottom-up hand-overs in global memory, by split (n - 1)
-    uint32_t* counts;  // bottom-up ready-list counters: one per SUPER block, then the top list's
-                       // (the ready lists reuse keys0 and vals0 after the sort, the mid and
-                       // top passes' quad overflow lists keys1)
+    uint32_t* counts;  // bottom-up ready-list counters: one per SUPER block, then the top list's,
+                       // then the number of pairs the top pass completed (the ready lists reuse
+                       // keys0 and vals0 after the sort; the mid pass's quad overflow lists and
+                       // the top pass's completed pairs keys1)
     size_t bytes;
 };
 
@@ -1063,7 +1078,7 @@ build_scratch plan_scratch(size_t n, char* base)
     s.cb = take(8);
     s.part = reinterpret_cast<float*>(take((size_t)CB_GRID * 6));
     s.stage = reinterpret_cast<split_stage*>(take(n * (sizeof(split_stage) / 4)));
-    s.counts = take((n + SUPER - 1) / SUPER + 1);
+    s.counts = take((n + SUPER - 1) / SUPER + 2);  // + the top pass's done count
     s.bytes = off;
     return s;
 }
@@ -1103,10 +1118,14 @@ void launch_bottom_up(const bvh_state& bs, const build_scratch& s, const vr_tria
     if (n > (size_t)CHUNK)
         hipLaunchKernelGGL(k_bottom_up_mid, dim3((unsigned)nb), dim3(TOP_BLOCK), 0, st, n, bs.sorted_ids, bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box,
                            depth, s.keys0, s.counts, s.vals0, s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
-    if (n > (size_t)SUPER)
+    if (n > (size_t)SUPER) {
         hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(TOP_BLOCK), 0, st, n, bs.sorted_ids,
                            bs.pairs, s.stage, slots, bs.heights, slot_heights, ref_nodes, bs.root_box, depth, s.vals0,
-                           s.counts + nb, s.keys1, bs.quads, LBVH_LEAF_MAX);
+                           s.counts + nb, s.keys1, s.counts + nb + 1, bs.quads, LBVH_LEAF_MAX);
+        if (bs.quads)
+            hipLaunchKernelGGL(k_list_quads, dim3(64), dim3(BLOCK), 0, st, bs.pairs, (uint32_t)(n - 1),
+                               LBVH_LEAF_MAX, bs.quads, s.keys1, s.counts + nb + 1);
+    }
 }
 
 }  // namespace
