@@ -138,7 +138,8 @@ __device__ inline void block_minmax6(float v[6], float* __restrict__ out)
 }
 
 __global__ __launch_bounds__(CB_BLOCK) void k_centroid_bounds(const vr_triangle* __restrict__ src, size_t n,
-                                                              int vec, float* __restrict__ part)
+                                                              int vec, float* __restrict__ part,
+                                                              float* __restrict__ cen)
 {
     constexpr uint32_t GROUP = CB_U * CB_BLOCK;            // triangles per step
     constexpr uint32_t VEC = GROUP * TRI_WORDS / 4;        // their 16-byte words
@@ -169,6 +170,12 @@ __global__ __launch_bounds__(CB_BLOCK) void k_centroid_bounds(const vr_triangle*
                 const f3 m = tri_centroid(tri_from_words(s_tri + t * TRI_WORDS));
                 v[0] = fminf(v[0], m.x); v[1] = fminf(v[1], m.y); v[2] = fminf(v[2], m.z);
                 v[3] = fmaxf(v[3], m.x); v[4] = fmaxf(v[4], m.y); v[5] = fmaxf(v[5], m.z);
+                if (cen) {
+                    float* o = cen + 3 * (t0 + t);
+                    o[0] = m.x;
+                    o[1] = m.y;
+                    o[2] = m.z;
+                }
             }
         }
         __syncthreads();
@@ -401,9 +408,9 @@ struct morton_extras {
     uint32_t ncounts;
 };
 
-__global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict__ prims, size_t n, int vec,
-                                                  const uint32_t* __restrict__ cb, uint32_t* __restrict__ codes,
-                                                  morton_extras x)
+__global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict__ prims, const float* __restrict__ cen,
+                                                  size_t n, int vec, const uint32_t* __restrict__ cb,
+                                                  uint32_t* __restrict__ codes, morton_extras x)
 {
     __shared__ float s_tri[BLOCK * TRI_WORDS];
     __shared__ uint32_t h[RADIX];
@@ -419,19 +426,24 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
         const size_t t0 = base + (size_t)k * BLOCK;
         if (t0 >= n) break;  // uniform over the block
         const uint32_t cnt = (uint32_t)min((size_t)BLOCK, n - t0);
-        const float* sp = reinterpret_cast<const float*>(prims + t0);
-        if (vec && cnt == BLOCK) {
-            for (uint32_t q = tid; q < BLOCK * TRI_WORDS / 4; q += BLOCK)
-                reinterpret_cast<float4*>(s_tri)[q] = reinterpret_cast<const float4*>(sp)[q];
-        } else {
-            for (uint32_t q = tid; q < cnt * TRI_WORDS; q += BLOCK) s_tri[q] = sp[q];
+        if (!cen) {
+            const float* sp = reinterpret_cast<const float*>(prims + t0);
+            if (vec && cnt == BLOCK) {
+                for (uint32_t q = tid; q < BLOCK * TRI_WORDS / 4; q += BLOCK)
+                    reinterpret_cast<float4*>(s_tri)[q] = reinterpret_cast<const float4*>(sp)[q];
+            } else {
+                for (uint32_t q = tid; q < cnt * TRI_WORDS; q += BLOCK) s_tri[q] = sp[q];
+            }
         }
         __syncthreads();
         const bool valid = tid < cnt;
         const size_t i = t0 + tid;
         uint32_t code = 0u;
         if (valid) {
-            f3 c = tri_centroid(tri_from_words(s_tri + tid * TRI_WORDS));
+            // the centroid the bounds pass kept (the same float operations),
+            // or from the triangle
+            f3 c = cen ? mk3(cen[3 * i], cen[3 * i + 1], cen[3 * i + 2])
+                       : tri_centroid(tri_from_words(s_tri + tid * TRI_WORDS));
             c = sub3(c, center);
             c = add3(c, muls3(size, 0.5f));
             c = f3{c.x / size.x, c.y / size.y, c.z / size.z};
@@ -1085,13 +1097,13 @@ build_scratch plan_scratch(size_t n, char* base)
 
 // centroid box, then the codes with the kernel's extras
 int launch_morton(const vr_triangle* prims, size_t n, float* part, uint32_t* cb, uint32_t* codes,
-                  const morton_extras& x, hipStream_t st)
+                  const morton_extras& x, float* cen, hipStream_t st)
 {
     const unsigned grid = std::max(1u, std::min(div_up(n, CB_U * CB_BLOCK), (unsigned)CB_GRID));
     const int vec = reinterpret_cast<uintptr_t>(prims) % 16 == 0;
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, vec, part);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(grid), dim3(CB_BLOCK), 0, st, prims, n, vec, part, cen);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(CB_BLOCK), 0, st, part, grid, cb);
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, TILE)), dim3(BLOCK), 0, st, prims, n, vec, cb, codes, x);
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, TILE)), dim3(BLOCK), 0, st, prims, cen, n, vec, cb, codes, x);
     VR_HIP_CHECK(hipGetLastError());
     return VR_OK;
 }
@@ -1198,7 +1210,10 @@ int lbvh_build_into(bvh_state& bs, const vr_triangle* d_prims, size_t n, bool ke
     const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
     const size_t nb = (n + SUPER - 1) / SUPER;
     const morton_extras extras{s.hist, tiles, bs.flags, s.counts, (uint32_t)(nb + 1)};
-    int rc = launch_morton(d_prims, n, s.part, s.cb, s.keys0, extras, st);
+    // the centroids go to the bottom-up passes' stage (64 bytes per split,
+    // unused until then): 12 bytes per primitive written and read once
+    // instead of the triangles (44 bytes) read again
+    int rc = launch_morton(d_prims, n, s.part, s.cb, s.keys0, extras, reinterpret_cast<float*>(s.stage), st);
     if (rc) return rc;
 
     // 3: radix sort (code, id) -- 30-bit keys, four 8-bit passes
@@ -1310,7 +1325,7 @@ int lbvh_morton_only(const vr_triangle* d_prims, size_t n, uint32_t* d_codes, hi
     VR_HIP_CHECK(hipMallocAsync(&tmp, 256 + (size_t)CB_GRID * 6 * sizeof(float), st));
     uint32_t* cb = static_cast<uint32_t*>(tmp);
     float* part = reinterpret_cast<float*>(static_cast<char*>(tmp) + 256);
-    int rc = launch_morton(d_prims, n, part, cb, d_codes, morton_extras{}, st);
+    int rc = launch_morton(d_prims, n, part, cb, d_codes, morton_extras{}, nullptr, st);
     (void)hipFreeAsync(tmp, st);
     return rc;
 }
