@@ -442,13 +442,44 @@ __device__ __forceinline__ bool end_sample(const scene_desc& S, path_t<ADJ>& P)
 #define VR_PT_REGEN_LANES 32
 #endif
 
+// Hand-over of samples inside a workgroup once the pixel queue is empty.  A
+// pixel's samples run in sequence on one lane (one random stream per pixel),
+// so at the end of the frame lanes idle while their waves finish the last
+// pixels.  A wave (other than the workgroup's first) with fewer than
+// VR_PT_DONATE_LANES lanes holding a pixel then hands each of its pixels over
+// at the pixel's next sample boundary: pixel, next sample index, random
+// state and sum so far go to the workgroup's pool in LDS, and the wave ends
+// once all its lanes are idle.  Idle lanes of the other waves take them and
+// go on with the next sample exactly as the giving lane would have (same
+// random numbers, same sum), so the image does not change; the work gathers
+// in fewer, fuller waves.  The first wave never gives and ends last, after
+// every other wave has ended and the pool is empty, so no sample is left
+// behind.  0 disables it.
+#ifndef VR_PT_DONATE_LANES
+#define VR_PT_DONATE_LANES 24
+#endif
+struct handed_sample {
+    uint4 a;     // pixel, next sample, random state, scene
+    float4 sum;  // the pixel's samples so far
+};
+
 template <int MODE, bool BATCH, int BLK, int LDSD, int NODE = NODE_QUAD>
 __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 {
     constexpr bool COUNT = (MODE & ~PT_NEE) == PT_COUNT;
     constexpr bool ADJ = (MODE & ~PT_NEE) == PT_ADJOINT;
     constexpr bool NEE = (MODE & PT_NEE) != 0;
+    // the adjoint keeps the plain end of frame (its gradient partials are
+    // summed per workgroup, which a hand-over would reorder), and so does the
+    // NEE frame (measured: no gain, its shadow-ray state leaves no registers)
+    constexpr bool DON = !ADJ && !NEE && VR_PT_DONATE_LANES > 0 && BLK > 64;
+    // each lane of the waves other than the first gives at most once (a
+    // giving wave takes nothing afterwards)
+    constexpr uint32_t POOL = DON ? BLK - 64 : 1;
     __shared__ uint32_t stack[LDSD * BLK];
+    __shared__ handed_sample s_pool[POOL];
+    __shared__ uint32_t s_pool_ready[POOL];
+    __shared__ uint32_t s_given, s_taken, s_waves;
     extern __shared__ float s_grad[];  // ADJ: 6 * num_mats
     uint32_t* st = stack + threadIdx.x;
     const uint32_t npix = BATCH ? A.total_pixels : padded_pixels(A.single.sp.width, A.single.sp.height, A.tile);
@@ -458,6 +489,20 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
         for (uint32_t i = threadIdx.x; i < 6 * A.grad_slots; i += BLK) s_grad[i] = 0.0f;
         __syncthreads();
     }
+    const uint32_t wid = threadIdx.x / 64u;
+    if (DON) {
+        for (uint32_t i = threadIdx.x; i < POOL; i += BLK) s_pool_ready[i] = 0u;
+        if (threadIdx.x == 0) {
+            s_given = 0u;
+            s_taken = 0u;
+            s_waves = BLK / 64u;
+        }
+        __syncthreads();
+    }
+    // wave-uniform: the pixel queue gave out its last pixel to this wave's
+    // view; this wave hands its pixels over
+    bool exhausted = false, giver = false;
+    uint32_t idle_rounds = 0;
     uint32_t sid = 0;  // scene of this lane's pixel (BATCH)
 #define SCN (BATCH ? A.scenes[sid] : A.single)
 
@@ -484,6 +529,7 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
 
     for (;;) {
         const bool was_tracing = tracing;
+        bool give = false;  // DON: this lane hands its pixel's next sample over
         // 1. shade the finished queries; each such lane starts its next query
         //    or finishes its sample or pixel
         if (pending) {
@@ -526,6 +572,8 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                     if (end_sample(S, P)) {
                         if (!ADJ) write_pixel(S, (uint32_t)px, sample_sum(S, P.sum));
                         px = -1;
+                    } else if (DON && giver) {
+                        give = true;
                     } else {
                         const uint32_t w = S.sp.width, h = S.sp.height;
                         const int x = (int)((uint32_t)px % w), y = (int)((uint32_t)px / w);
@@ -533,16 +581,35 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                         start_sample(P);
                     }
                 }
-                if (px >= 0) {
+                if (px >= 0 && !give) {
                     trav_begin(T, S.b, no, nd, 0.0f, FLT_MAX);
                     tracing = true;
                 }
             }
         }
 
+        if (DON) {
+            // the samples handed over in this round, one LDS atomic per wave
+            const uint64_t gm = __ballot(give);
+            if (gm) {
+                const int leader = __ffsll((unsigned long long)gm) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(&s_given, (uint32_t)__popcll(gm));
+                base = __shfl(base, leader);
+                if (give) {
+                    const uint32_t i = min(base + (uint32_t)lane_mask_lt_count(gm), POOL - 1u);
+                    s_pool[i].a = make_uint4((uint32_t)px, P.s, P.g.x, sid);
+                    s_pool[i].sum = P.sum;
+                    __hip_atomic_store(&s_pool_ready[i], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    px = -1;
+                }
+            }
+        }
+
         // 2. lanes without a pixel take the next ones, one atomic per wave
         const bool need = !done && px < 0;
-        const uint64_t needm = __ballot(need);
+        const uint64_t needm = DON && exhausted ? 0ull : __ballot(need);
+        bool dry = false;  // DON: the queue had no pixel left for this lane
         if (needm) {
             const int leader = __ffsll((unsigned long long)needm) - 1;
             uint32_t base = 0;
@@ -585,12 +652,92 @@ __global__ __launch_bounds__(BLK) VR_PT_OCCUPANCY void k_pathtrace(pt_args A)
                         trav_begin(T, S.b, o, d, 0.0f, FLT_MAX);
                         tracing = true;
                     }
+                } else if (DON) {
+                    dry = true;
                 } else {
                     done = true;
                 }
             }
         }
-        if (__ballot(!done) == 0) break;
+        if (DON) {
+            if (__ballot(dry)) exhausted = true;
+            if (exhausted) {
+                const uint32_t live = (uint32_t)__popcll(__ballot(px >= 0));
+                if (wid != 0 && live < (uint32_t)VR_PT_DONATE_LANES) giver = true;
+                if (giver) {
+                    // every pixel handed over or finished: this wave ends
+                    // (its hand-overs are in the pool before the count drops)
+                    if (live == 0) {
+                        if (lane == 0) __hip_atomic_fetch_add(&s_waves, -1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                } else {
+                    // idle lanes take handed-over samples: the leader reserves
+                    // up to one per idle lane of those given so far
+                    const uint64_t im = __ballot(px < 0);
+                    uint32_t got = 0, base = 0;
+                    if (im) {
+                        const int leader = __ffsll((unsigned long long)im) - 1;
+                        if ((int)lane == leader) {
+                            for (int tries = 0; tries < 16; ++tries) {
+                                uint32_t p = __hip_atomic_load(&s_taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                const uint32_t q = __hip_atomic_load(&s_given, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (p >= q) break;
+                                const uint32_t k = min(q - p, (uint32_t)__popcll(im));
+                                if (__hip_atomic_compare_exchange_strong(&s_taken, &p, p + k, __ATOMIC_RELAXED,
+                                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                                    base = p;
+                                    got = k;
+                                    break;
+                                }
+                            }
+                        }
+                        got = __shfl(got, leader);
+                        base = __shfl(base, leader);
+                        const uint32_t r = (uint32_t)lane_mask_lt_count(im);
+                        if (px < 0 && r < got) {
+                            const uint32_t i = min(base + r, POOL - 1u);
+                            // the giver writes a record right after reserving it
+                            for (uint32_t wt = 0; wt < (1u << 20); ++wt) {
+                                if (__hip_atomic_load(&s_pool_ready[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                            const uint4 a = s_pool[i].a;
+                            P.sum = s_pool[i].sum;
+                            px = (int32_t)a.x;
+                            P.s = a.y;
+                            P.g.x = a.z;
+                            if (BATCH) {
+                                sid = a.w;
+                                bv = A.scenes[sid].b;
+                            }
+                            const scene_desc& S = SCN;
+                            const uint32_t w = S.sp.width, h = S.sp.height;
+                            const int x = (int)(a.x % w), y = (int)(a.x / w);
+                            f3 o, d;
+                            primary_ray(S.cam, P.g, A.arg_order, x, y, (int)w, (int)h, o, d);
+                            start_sample(P);
+                            trav_begin(T, S.b, o, d, 0.0f, FLT_MAX);
+                            tracing = true;
+                        }
+                    }
+                    if (__ballot(px >= 0) == 0) {
+                        // only the first wave gets here with nothing to do
+                        // (the others hand over once they run low): it ends
+                        // when every other wave has ended and nothing is
+                        // left in the pool, and waits otherwise
+                        const uint32_t waves = __hip_atomic_load(&s_waves, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint32_t q = __hip_atomic_load(&s_given, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint32_t p = __hip_atomic_load(&s_taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if ((waves == 1u && p >= q) || ++idle_rounds > (1u << 22)) break;
+                        __builtin_amdgcn_s_sleep(32);
+                        continue;
+                    }
+                }
+            }
+        } else if (__ballot(!done) == 0) {
+            break;
+        }
         if (A.ray_counts) {
             // every lane that holds a query now and did not before this round
             // started one: primary if its path is at bounce 0 (a shadow ray
