@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(bvh_view b, const vr_ray* __res
             }
         }
         if (__ballot(!done) == 0) break;
-        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK, COUNT ? TRAV_LANE : TRAV_VOTED, default_isect, NODE>(
+        if (tracing && trav_iter<ANY, COUNT, LDS_STACK, BLOCK, COUNT ? TRAV_LANE : TRAV_VOTED, paired_isect, NODE>(
                            T, b, st, spill, &c)) {
             float4 o;
             if (T.r.tri >= 0) {
