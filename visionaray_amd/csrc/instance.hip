@@ -310,9 +310,20 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A, uint32_t* __r
         const uint32_t k = lf++;
         --ln;
         if (bottom) {
+            // the leaf's next triangle in the same iteration (in leaf order;
+            // a lane with one left tests it twice, and the second test of the
+            // same triangle never replaces the hit: not strictly closer)
+            const uint32_t k2 = ln > 0 ? k + 1u : k;
+            if (ln > 0) {
+                ++lf;
+                --ln;
+            }
             const float4* tp = reinterpret_cast<const float4*>(tris + k);
+            const float4* sp2 = reinterpret_cast<const float4*>(tris + k2);
             const float4 ta = tp[0], tb = tp[1], tq = tp[2];
+            const float4 sa = sp2[0], sb = sp2[1], sq = sp2[2];
             float t, u, v;
+            bool ended = false;
             if (intersect_tri(o, d, mk3(ta.x, ta.y, ta.z), mk3(tb.x, tb.y, tb.z), mk3(tq.x, tq.y, tq.z), t, u, v) &&
                 t < best_t && t >= tmin && t <= tmax) {
                 best_t = t;
@@ -320,12 +331,23 @@ __global__ __launch_bounds__(BLOCK) void k_tlas_trace(tlas_args A, uint32_t* __r
                 best_v = v;
                 best_tri = (int32_t)k;
                 best_inst = cur_inst;
-                if (ANY_HIT) {  // any hit ends the query
-                    cur = TRAV_DONE;
-                    ln = 0;
-                    sp = 0;
-                    to_top();
-                }
+                ended = ANY_HIT;
+            }
+            if (!ended &&
+                intersect_tri(o, d, mk3(sa.x, sa.y, sa.z), mk3(sb.x, sb.y, sb.z), mk3(sq.x, sq.y, sq.z), t, u, v) &&
+                t < best_t && t >= tmin && t <= tmax) {
+                best_t = t;
+                best_u = u;
+                best_v = v;
+                best_tri = (int32_t)k2;
+                best_inst = cur_inst;
+                ended = ANY_HIT;
+            }
+            if (ended) {  // any hit ends the query
+                cur = TRAV_DONE;
+                ln = 0;
+                sp = 0;
+                to_top();
             }
             continue;
         }
