@@ -642,6 +642,11 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     const uint32_t N = (uint32_t)n;
     const uint32_t c0 = blockIdx.x * (uint32_t)CHUNK;
     const uint32_t c1 = min(N, c0 + (uint32_t)CHUNK);
+    const uint32_t j = c0 + tid;
+    // the leaf's triangle, requested before the chunk's keys and the barrier
+    // so that the two dependent reads overlap them
+    const int32_t id = j < N ? sorted_ids[j] : 0;
+    const tri_in t = load_tri(prims, (size_t)id);
     s_slot[tid] = SLOT_EMPTY;
     s_ent[tid] = make_uint4(0u, 0u, SLOT_EMPTY, SLOT_EMPTY);
     for (uint32_t t = tid; t < CHUNK + 2; t += CHUNK) {
@@ -651,11 +656,8 @@ __global__ __launch_bounds__(CHUNK) void k_bottom_up_local(const vr_triangle* __
     __syncthreads();
     // keys k and k + 1 for c0 - 1 <= k < c1
     auto delta = [&](uint32_t k) { return key_delta(s_code[k + 1 - c0], s_code[k + 2 - c0], k); };
-    const uint32_t j = c0 + tid;
     // the leaf and its climb; every thread then meets at the barrier below
     auto leaf_and_climb = [&]() {
-        const int32_t id = sorted_ids[j];
-        const tri_in t = load_tri(prims, (size_t)id);
         tri_node tn;
         tn.a = make_float4(t.v1.x, t.v1.y, t.v1.z, __uint_as_float(t.prim_id));
         tn.b = make_float4(t.e1.x, t.e1.y, t.e1.z, __uint_as_float(t.geom_id));
