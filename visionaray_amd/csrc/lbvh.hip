@@ -249,12 +249,21 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict
     __syncthreads();
     const uint32_t tile = xcd_tile(blockIdx.x, num_tiles);
     const size_t base = (size_t)tile * TILE;
-    for (int k = 0; k < ITEMS; ++k) {
-        const size_t idx = base + (size_t)k * BLOCK + threadIdx.x;
-        const bool valid = idx < n;
-        const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
-        const uint64_t m = match_digit(d, valid);
-        if (valid && mbcnt64(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    // one LDS increment per key: a count does not depend on the order of its
+    // increments (ranking the wave's equal digits first, as the scatter must,
+    // cost eight ballots per key and left the pass VALU-bound)
+    static_assert(ITEMS == 8, "two 16-byte loads per thread");
+    if (base + TILE <= n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
+        const uint4* p = reinterpret_cast<const uint4*>(keys + base) + 2u * threadIdx.x;
+        const uint4 a = p[0], b = p[1];
+        const uint32_t k[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicAdd(&h[(k[j] >> shift) & 0xFFu], 1u);
+    } else {
+        for (int k = 0; k < ITEMS; ++k) {
+            const size_t idx = base + (size_t)k * BLOCK + threadIdx.x;
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 0xFFu], 1u);
+        }
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * num_tiles + tile] = h[threadIdx.x];
@@ -453,11 +462,7 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const vr_triangle* __restrict_
             codes[i] = code;
             if (x.edges && i % CHUNK == CHUNK - 1 && i + 1 < n) x.edges[i] = SLOT_EMPTY;
         }
-        if (x.hist) {
-            const uint32_t d = code & 0xFFu;
-            const uint64_t m = match_digit(d, valid);
-            if (valid && mbcnt64(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
-        }
+        if (x.hist && valid) atomicAdd(&h[code & 0xFFu], 1u);
         __syncthreads();
     }
     if (x.hist) x.hist[(size_t)tid * x.tiles + tile] = h[tid];
