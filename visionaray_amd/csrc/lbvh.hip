@@ -304,8 +304,9 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     __shared__ uint32_t s_val[TILE];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint32_t s_goff[RADIX];
-    __shared__ uint32_t s_run[RADIX];
-    __shared__ uint32_t s_wcnt[WAVES][RADIX];
+    // per (item, wave) row and digit: the keys of that digit the wave holds
+    // in that item, then (in place) their first slot in the tile
+    __shared__ uint16_t s_cnt[ITEMS * WAVES][RADIX];
     __shared__ uint32_t s_tmp[WAVES + 1];
 
     const uint32_t tid = threadIdx.x, w = tid / 64;
@@ -317,10 +318,10 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
     // the tiles before this one
     const uint32_t dbase = block_exclusive_scan(totals[tid], s_tmp, total);
     s_goff[tid] = dbase + offsets[(size_t)tid * num_tiles + tile];
-    s_dstart[tid] = block_exclusive_scan(cnt, s_tmp, total);
-    s_run[tid] = 0;
-    for (int q = 0; q < WAVES; ++q) s_wcnt[q][tid] = 0;
-    __syncthreads();
+    const uint32_t dstart = block_exclusive_scan(cnt, s_tmp, total);
+    s_dstart[tid] = dstart;
+#pragma unroll
+    for (int r = 0; r < ITEMS * WAVES; ++r) s_cnt[r][tid] = 0;
 
     // every key and value of the tile in flight at once, before the barriers
     uint32_t keys[ITEMS], vals[ITEMS];
@@ -331,32 +332,40 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(const uint32_t* __restr
         keys[k] = valid ? keys_in[idx] : 0u;
         vals[k] = valid ? (first_pass ? (uint32_t)idx : vals_in[idx]) : 0u;
     }
+    __syncthreads();
+    // tile order is (item, wave, lane): a key's slot is its digit's prefix
+    // over the earlier (item, wave) rows plus its rank among the wave's
+    // equal digits of the item (stable)
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const size_t idx = base + (size_t)k * BLOCK + tid;
-        const bool valid = idx < n;
-        const uint32_t key = keys[k];
-        const uint32_t val = vals[k];
-        const uint32_t d = (key >> shift) & 0xFFu;
+        const bool valid = base + (size_t)k * BLOCK + tid < n;
+        const uint32_t d = (keys[k] >> shift) & 0xFFu;
         const uint64_t m = match_digit(d, valid);
-        const uint32_t rank = mbcnt64(m);
-        if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_dstart[d] + s_run[d] + rank;
-            for (uint32_t q = 0; q < w; ++q) pos += s_wcnt[q][d];
-            s_key[pos] = key;
-            s_val[pos] = val;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int q = 0; q < WAVES; ++q) {
-            add += s_wcnt[q][tid];
-            s_wcnt[q][tid] = 0;
-        }
-        s_run[tid] += add;
-        __syncthreads();
+        rank[k] = mbcnt64(m);
+        if (valid && rank[k] == 0) s_cnt[k * WAVES + w][d] = (uint16_t)__popcll(m);
     }
+    __syncthreads();
+    {
+        uint32_t run = dstart;  // at most TILE
+#pragma unroll
+        for (int r = 0; r < ITEMS * WAVES; ++r) {
+            const uint32_t c = s_cnt[r][tid];
+            s_cnt[r][tid] = (uint16_t)run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (base + (size_t)k * BLOCK + tid < n) {
+            const uint32_t d = (keys[k] >> shift) & 0xFFu;
+            const uint32_t pos = min((uint32_t)s_cnt[k * WAVES + w][d] + rank[k], (uint32_t)TILE - 1u);
+            s_key[pos] = keys[k];
+            s_val[pos] = vals[k];
+        }
+    }
+    __syncthreads();
     const size_t tile_n = std::min<size_t>(TILE, n - base);
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t p = (uint32_t)k * BLOCK + tid;
